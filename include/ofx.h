@@ -1,0 +1,206 @@
+/*
+ * ofx.h -- C ABI of the MI355X unstructured finite-volume hot path.
+ *
+ * The library (oneflow_amd/libofx.so) evaluates OneFLOW's unstructured-grid
+ * compressible Navier-Stokes + Spalart-Allmaras residual and its implicit
+ * LU-SGS update on one AMD Instinct MI355X (gfx950) with hand-written HIP
+ * kernels.  Every entry point below replaces one CPU task of the reference
+ * solver; the citation on each names the reference function it stands in for
+ * (paths relative to the OneFLOW source tree).
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative OFX_E* code on error; the message of
+ *     the last error on this host thread is returned by ofx_last_error();
+ *   - nothing throws across the boundary;
+ *   - device buffers are allocated and owned by the context; the host passes
+ *     plain host pointers for upload / download;
+ *   - one context = one HIP stream; calls on one context are serialised.
+ *
+ * Field layout (host and device): structure-of-arrays, one contiguous array
+ * per equation, cells 0..nCells-1 followed by one ghost cell per boundary
+ * face (ghost of boundary face f is cell nCells+f), faces ordered
+ * boundary-first exactly as in the grid file (codes/geometry/src/UnsGrid.cpp:123-129).
+ * Primitive variables are rho,u,v,w,p (codes/ns/include/NsIdx.h:33-44).
+ */
+#ifndef OFX_H
+#define OFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------- */
+#define OFX_OK            0
+#define OFX_EINVAL       -1   /* bad argument / unsupported option        */
+#define OFX_ESTATE       -2   /* call out of order (e.g. no mesh yet)     */
+#define OFX_EHIP         -3   /* HIP runtime error                        */
+#define OFX_ENOMEM       -4   /* device allocation failed                 */
+#define OFX_ENAN         -5   /* NaN in a residual (reference: Stop())    */
+
+/* ---- case configuration --------------------------------------------- */
+/* The subset of the reference's script keys read on this path
+ * (test/<case>/script/{cfd,inflow,turb,vismodel}.txt; readers
+ * codes/ns/src/NsCom.cpp:44-122, codes/turb/src/TurbCom.cpp:43-380,
+ * codes/global/src/Interation.cpp:52-64).  Plain old data, no pointers. */
+typedef struct ofx_config {
+    int32_t dim;               /* 2 or 3                                      */
+    int32_t ischeme;           /* 1 = Roe, 12 = SLAU2                         */
+    int32_t ilim;              /* -1 zero order, 0 none, 1 Barth, 2 Venkat    */
+    int32_t ieigenfix;         /* Roe entropy fix on/off                      */
+    int32_t ivischeme;         /* NS face gradient: 0 STD, 1 AVER, 2 TEST     */
+    int32_t vismodel;          /* 0 Euler, 1 laminar, 3 SA                    */
+    int32_t timestepModel;     /* 0 local time step                           */
+    int32_t visSRModel;        /* viscous spectral radius model (1,2,3)       */
+    int32_t visTimeStepModel;  /* >0: viscous time-step limit                 */
+    int32_t ncfl;              /* CFL ramp length                             */
+    int32_t isowallbc;         /* 0 adiabatic wall, 1 isothermal              */
+    int32_t iturb_visflux;     /* SA face gradient method (0,1,2)             */
+    int32_t iprod_sa;          /* SA production form                          */
+    int32_t turb_ilim;         /* SA reconstruction (only -1 supported)       */
+    int32_t tns_ilim;          /* NS state used by SA convection (only -1)    */
+    int32_t flux_fp32;         /* 1: FP32 face fluxes, FP64 accumulation      */
+    double  vencat_coef;
+    double  centropy1, centropy2;
+    double  cflst, cfled, max_time_ratio;
+    double  mach_ref, aoa_degree, sideslip_degree, reynolds;
+    double  gama_ref, prl, prt, tref_dim, twall_dim;
+    double  ref_sa, max_vis_ratio, rprod, turb_cfl_ratio;
+} ofx_config;
+
+/* Field identifiers for ofx_get_field / ofx_set_field.  Sizes in doubles. */
+enum ofx_field {
+    OFX_F_Q = 0,        /* primitive state        [5][nTCell]               */
+    OFX_F_TURBQ,        /* SA nu_tilde            [nTCell]                  */
+    OFX_F_RES,          /* NS residual            [5][nTCell]               */
+    OFX_F_TURBRES,      /* SA residual            [nTCell]                  */
+    OFX_F_DQDX,         /* primitive gradients    [5][nTCell] x3 (x,y,z)   */
+    OFX_F_DTDX,         /* temperature gradient   [nTCell] x3               */
+    OFX_F_LIMITER,      /* NS limiter             [5][nTCell]               */
+    OFX_F_TIMESTEP,     /* local time step        [nTCell]                  */
+    OFX_F_INVSR,        /* inviscid spectral rad. [nTCell]                  */
+    OFX_F_VISSR,        /* viscous spectral rad.  [nTCell]                  */
+    OFX_F_DQ,           /* NS LU-SGS increment    [5][nTCell]               */
+    OFX_F_TURBDQ,       /* SA LU-SGS increment    [nTCell]                  */
+    OFX_F_TEMPR,        /* temperature            [nTCell]                  */
+    OFX_F_VISL,         /* laminar viscosity      [nTCell]                  */
+    OFX_F_VIST,         /* eddy viscosity         [nTCell]                  */
+    OFX_F_DIST,         /* wall distance          [nCells]                  */
+    OFX_F_VOL,          /* cell volume            [nTCell]                  */
+    OFX_F_XCC,          /* cell centre            [nTCell] x3               */
+    OFX_F_FACE_NORMAL,  /* unit normal + area     [F] x4 (nx,ny,nz,area)    */
+    OFX_F_FACE_CENTER,  /* face centre            [F] x3                    */
+    OFX_F_TURBDQDX,     /* SA gradient            [nTCell] x3               */
+    OFX_F_TURBIMPSR,    /* SA implicit diagonal   [nTCell]                  */
+    OFX_F_IMPSR,        /* NS implicit diagonal   [nTCell]                  */
+    OFX_F_COUNT
+};
+
+/* Residual norms of one pseudo-time step, as written to res.dat/turbres.dat
+ * (codes/residual/src/Residual.cpp:52-72): sqrt(sum_c res^2 / nCells). */
+typedef struct ofx_norms {
+    double ns[5];
+    double sa;
+} ofx_norms;
+
+typedef struct ofx_ctx ofx_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char* ofx_last_error(void);
+
+/* Create / destroy a context bound to one HIP device and one stream. */
+int ofx_ctx_create(ofx_ctx** out, int device);
+int ofx_ctx_destroy(ofx_ctx* ctx);
+
+/* Upload one zone's grid exactly as stored in a .ofl file and build the
+ * geometry on the device: face normals/areas/centres and cell centres and
+ * volumes (codes/geometry/src/UnsGrid.cpp:469-760, FaceMesh.cpp:112-287),
+ * ghost-cell mirrors, the cell->face lists (codes/geometry/src/CellTopo.cpp:95-119),
+ * the LU-SGS level schedule and the wall distance
+ * (codes/geometry/src/WallDist.cpp:136-358).  rc[f] < 0 marks a boundary
+ * face; boundary faces must come first.  faceNodeCount[nFaces],
+ * faceNodes[sum(faceNodeCount)], bcType[nBFaces] (codes/geometry/src/Boundary.cpp:35-46). */
+int ofx_mesh_upload(ofx_ctx* ctx, int dim, int nNodes,
+                    const double* x, const double* y, const double* z,
+                    int nFaces, int nCells, int nBFaces,
+                    const int* faceNodeCount, const int* faceNodes,
+                    const int* lc, const int* rc, const int* bcType);
+
+/* Mesh sizes: out[0..4] = nCells, nFaces, nBFaces, nTCell, nLevels (LU-SGS). */
+int ofx_mesh_info(ofx_ctx* ctx, int* out5);
+
+/* Case constants (codes/ns/src/NsCom.cpp:44-122, codes/chemical/src/Chemical.cpp:280-338,
+ * codes/turb/src/TurbCom.cpp:110-380). */
+int ofx_config_set(ofx_ctx* ctx, const ofx_config* cfg);
+
+/* Freestream start: INIT_FLOWFIELD + INIT_FINAL of the NS and SA solvers
+ * (codes/uns/src/UNsRestart.cpp:48-62, codes/ns/src/NsSolverImp.cpp:58-77,
+ *  codes/uturb/src/UTurbRestart.cpp:48-60, codes/turb/src/TurbSolverImp.cpp:49-54). */
+int ofx_init_freestream(ofx_ctx* ctx);
+
+/* Host <-> device copies of any field (enum ofx_field, sizes above). */
+int ofx_get_field(ofx_ctx* ctx, int field, double* host);
+int ofx_set_field(ofx_ctx* ctx, int field, const double* host);
+
+/* NsCalcBoundary: gamma/T, Sutherland viscosity, ghost-cell BCs
+ * (codes/ns/src/NsSolverImp.cpp:72-79, codes/uns/src/UNsBcSolver.cpp:55-168). */
+int ofx_ns_boundary(ofx_ctx* ctx);
+/* TurbCalcBoundary: eddy viscosity + SA ghost cells
+ * (codes/turb/src/TurbSolverImp.cpp:55-60). */
+int ofx_sa_boundary(ofx_ctx* ctx);
+
+/* CALC_TIME_STEP: UTimeStep::CalcTimeStep at pseudo-step `iter`
+ * (codes/uns/src/UTimeStep.cpp:190-416, codes/ns/src/TimeStep.cpp:43-133). */
+int ofx_timestep(ofx_ctx* ctx, int iter);
+
+/* UPDATE_RESIDUALS of the NS solver: NsCalcRHS = UNsInvFlux::CalcFlux +
+ * UNsVisFlux::CalcFlux with the ordered, atomic-free residual gather
+ * (codes/ns/src/NsRhs.cpp:106-128, codes/uns/src/UNsInvFlux.cpp:123-192,
+ *  codes/uns/src/UNsVisFlux.cpp:90-127, codes/usolver/src/UCom.cpp:144-171).
+ * res is overwritten (ZERO_RESIDUALS + LOAD_RESIDUALS happen inside). */
+int ofx_residual_ns(ofx_ctx* ctx);
+
+/* UPDATE_RESIDUALS of the SA solver: TurbCalcRHS
+ * (codes/turb/src/TurbRhs.cpp:57-71). */
+int ofx_residual_sa(ofx_ctx* ctx);
+
+/* INIT_LUSGS + LUSGS_LOWER_SWEEP + LUSGS_UPPER_SWEEP, level scheduled
+ * (codes/uns/src/UNsLusgs.cpp:75-138, codes/ns/src/NsLusgs.cpp:147-330;
+ *  codes/uturb/src/UTurbLusgs.cpp:90-304, codes/turb/src/TurbLusgs.cpp:123-254). */
+int ofx_lusgs_ns(ofx_ctx* ctx);
+int ofx_lusgs_sa(ofx_ctx* ctx);
+
+/* UPDATE_FLOWFIELD_LUSGS (codes/uns/src/UNsUpdate.cpp:46-146,
+ * codes/uturb/src/UTurbUpdate.cpp:85-120). */
+int ofx_update_ns(ofx_ctx* ctx);
+int ofx_update_sa(ofx_ctx* ctx);
+
+/* Residual norms of the current res / turbres fields (DUMP_RESIDUAL,
+ * codes/residual/src/ResidualTask.cpp:75-126). */
+int ofx_norms_get(ofx_ctx* ctx, ofx_norms* out);
+
+/* n whole pseudo-time steps starting at step `first_iter` (1-based outer
+ * step, drives the CFL ramp), the task sequence of TimeIntegral::Lusgs for
+ * NS then SA (codes/solver/src/TimeIntegral.cpp:113-130,
+ * codes/multigrid/src/Multigrid.cpp:339-375).  norms_out (may be NULL)
+ * receives n entries. */
+int ofx_iterate(ofx_ctx* ctx, int first_iter, int n, ofx_norms* norms_out);
+
+/* Wall force coefficients of the current state: out[0..3] = CL, CD, CDp, CDf
+ * (codes/force/src/AeroForceTask.cpp:186-380). */
+int ofx_aero(ofx_ctx* ctx, double* out4);
+
+/* Average duration of the kernels launched by the last ofx_residual_ns call
+ * on this context, measured with HIP events on the context's stream:
+ * out[0] = whole NS residual (ms), out[1] = NS face-flux kernel (ms). */
+int ofx_profile_last(ofx_ctx* ctx, double* out2);
+
+/* Synchronise the context's stream. */
+int ofx_sync(ofx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OFX_H */
