@@ -1,0 +1,385 @@
+// ofx_api.hip -- the C ABI of include/ofx.h and the host-side task sequence.
+//
+// The host side mirrors the reference's task runtime for one zone: one call
+// per registered task (UPDATE_RESIDUALS, LUSGS_*_SWEEP, UPDATE_FLOWFIELD, ...)
+// and ofx_iterate() for the whole TimeIntegral::Lusgs sequence of the NS then
+// the SA solver (codes/solver/src/TimeIntegral.cpp:113-130,
+// codes/multigrid/src/Multigrid.cpp:339-375).
+#include "ofx_common.h"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace ofx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int alloc_bytes(ofx_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return OFX_ENOMEM;
+    }
+    c->allocs.push_back(*p);
+    e = hipMemsetAsync(*p, 0, bytes, c->stream);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return OFX_EHIP; }
+    return 0;
+}
+
+int upload(ofx_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return 0;
+    OFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int alloc_fields(ofx_ctx* c) {
+    const size_t T = c->m.T, F = c->m.F, C = c->m.C;
+    DevFields& d = c->d;
+    auto A = [&](double** p, size_t n) { return alloc_bytes(c, (void**)p, n * sizeof(double)); };
+    OFX_TRY(A(&d.q, 5 * T));
+    OFX_TRY(A(&d.tempr, T));
+    OFX_TRY(A(&d.visl, T));
+    OFX_TRY(A(&d.vist, T));
+    OFX_TRY(A(&d.turbq, T));
+    OFX_TRY(A(&d.res, 5 * T));
+    OFX_TRY(A(&d.turbres, T));
+    OFX_TRY(A(&d.dq, 5 * T));
+    OFX_TRY(A(&d.turbdq, T));
+    OFX_TRY(A(&d.grad, 18 * T));
+    OFX_TRY(A(&d.tgrad, 3 * T));
+    OFX_TRY(A(&d.minmax, 10 * C));
+    OFX_TRY(A(&d.limiter, 5 * T));
+    OFX_TRY(A(&d.timestep, T));
+    OFX_TRY(A(&d.invsr, T));
+    OFX_TRY(A(&d.vissr, T));
+    OFX_TRY(A(&d.impsr, T));
+    OFX_TRY(A(&d.turbimpsr, T));
+    OFX_TRY(A(&d.invflux, 5 * F));
+    OFX_TRY(A(&d.visflux, 5 * F));
+    OFX_TRY(A(&d.saface, 5 * F));
+    OFX_TRY(A(&d.red, 16));
+    OFX_TRY(A(&d.norms, 8 + 6 * 512));
+    return 0;
+}
+
+// Case constants (codes/ns/src/NsCom.cpp:44-122, codes/chemical/src/Chemical.cpp:280-338,
+// codes/turb/src/TurbCom.cpp:110-380).
+static void set_consts(ofx_ctx* c) {
+    const ofx_config& g = c->cfg;
+    Consts& k = c->k;
+    double aoa = g.aoa_degree * PI / 180.0;
+    double aos = g.sideslip_degree * PI / 180.0;
+    k.gama = g.gama_ref;
+    k.statecoef = 1.0 / (g.gama_ref * (g.mach_ref * g.mach_ref));
+    k.reynolds = g.reynolds;
+    k.oreynolds = 1.0 / g.reynolds;
+    k.oprl = 1.0 / g.prl;
+    k.oprt = 1.0 / g.prt;
+    k.twall = g.twall_dim / g.tref_dim;
+    k.const_cp = 1.0 / ((g.gama_ref - 1.0) * (g.mach_ref * g.mach_ref));
+    k.ts_suth = 110.4 / g.tref_dim;
+    k.inflow[0] = 1.0;
+    k.inflow[1] = 1.0 * std::cos(aoa) * std::cos(aos);
+    k.inflow[2] = 1.0 * std::sin(aoa) * std::cos(aos);
+    k.inflow[3] = 1.0 * std::sin(aos);
+    k.inflow[4] = k.statecoef * 1.0 * 1.0 / 1.0;
+    k.cfl = g.cflst;
+    k.sa_inflow = g.ref_sa;
+    double sigma = 2.0 / 3.0;
+    k.osigma = 1.0 / sigma;
+    double karm = 0.41;
+    k.karm2 = karm * karm;
+    k.okarm2 = 1.0 / k.karm2;
+    k.cb1 = 0.1355;
+    k.cb2 = 0.622;
+    double cv1 = 7.1;
+    k.cv13 = cv1 * cv1 * cv1;
+    k.cw2 = 0.3;
+    k.cw36 = std::pow(2.0, 6);
+    k.cw1 = k.cb1 * k.okarm2 + (1.0 + k.cb2) / sigma;
+    k.sac2 = 0.7;
+    k.sac3 = 0.9;
+    k.cb2s = k.osigma * k.cb2;
+    k.cw1k = k.cw1 * k.karm2;
+    k.max_vis_ratio = g.max_vis_ratio;
+    k.rprod = g.rprod;
+    k.turb_cfl_ratio = g.turb_cfl_ratio;
+    k.vencat_coef = g.vencat_coef;
+    k.centropy1 = g.centropy1;
+    k.centropy2 = g.centropy2;
+    k.max_time_ratio = g.max_time_ratio;
+    k.ischeme = g.ischeme;
+    k.ilim = g.ilim;
+    k.ieigenfix = g.ieigenfix;
+    k.ivischeme = g.ivischeme;
+    k.vismodel = g.vismodel;
+    k.visSRModel = g.visSRModel;
+    k.visTimeStepModel = g.visTimeStepModel;
+    k.isowallbc = g.isowallbc;
+    k.iturb_visflux = g.iturb_visflux;
+    k.iprod_sa = g.iprod_sa;
+    k.flux_fp32 = g.flux_fp32;
+}
+
+__global__ void k_fill_field(double* p, long n, double v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int init_freestream(ofx_ctx* c) {
+    const long T = c->m.T;
+    for (int e = 0; e < 5; ++e)
+        k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.q + e * T, T, c->k.inflow[e]);
+    k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.turbq, T, c->k.sa_inflow);
+    k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.vist, T, 0.0);
+    OFX_HIP(hipGetLastError());
+    OFX_TRY(ns_boundary(c));
+    OFX_TRY(sa_boundary(c));
+    return 0;
+}
+
+}  // namespace ofx
+
+using namespace ofx;
+
+#define CHECK_CTX(c)                                                           \
+    do {                                                                       \
+        if (!(c)) { set_error("null context"); return OFX_EINVAL; }            \
+    } while (0)
+#define CHECK_READY(c)                                                         \
+    do {                                                                       \
+        CHECK_CTX(c);                                                          \
+        if (!(c)->has_mesh || !(c)->has_cfg) {                                 \
+            set_error("mesh and config must be set first"); return OFX_ESTATE; \
+        }                                                                      \
+    } while (0)
+
+extern "C" {
+
+const char* ofx_last_error(void) { return g_err.c_str(); }
+
+int ofx_ctx_create(ofx_ctx** out, int device) {
+    if (!out) { set_error("null out"); return OFX_EINVAL; }
+    *out = nullptr;
+    int n = 0;
+    OFX_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) { set_error("no such HIP device"); return OFX_EINVAL; }
+    ofx_ctx* c = new (std::nothrow) ofx_ctx();
+    if (!c) { set_error("out of host memory"); return OFX_ENOMEM; }
+    c->device = device;
+    OFX_HIP(hipSetDevice(device));
+    OFX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) OFX_HIP(hipEventCreate(&e));
+    *out = c;
+    return 0;
+}
+
+int ofx_ctx_destroy(ofx_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int ofx_mesh_upload(ofx_ctx* c, int dim, int nNodes, const double* x, const double* y, const double* z,
+                    int nFaces, int nCells, int nBFaces, const int* faceNodeCount, const int* faceNodes,
+                    const int* lc, const int* rc, const int* bcType) {
+    CHECK_CTX(c);
+    if (c->has_mesh) { set_error("context already holds a mesh"); return OFX_ESTATE; }
+    if ((dim != 2 && dim != 3) || nNodes <= 0 || nFaces <= 0 || nCells <= 0 || nBFaces < 0 || nBFaces > nFaces ||
+        !x || !y || !z || !faceNodeCount || !faceNodes || !lc || !rc || (nBFaces && !bcType)) {
+        set_error("ofx_mesh_upload: bad arguments"); return OFX_EINVAL;
+    }
+    OFX_HIP(hipSetDevice(c->device));
+    c->m.dim = dim; c->m.nNodes = nNodes; c->m.C = nCells; c->m.F = nFaces; c->m.B = nBFaces;
+    c->m.T = nCells + nBFaces;
+    OFX_TRY(mesh_build(c, x, y, z, faceNodeCount, faceNodes, lc, rc, bcType));
+    OFX_TRY(alloc_fields(c));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    c->has_mesh = true;
+    return 0;
+}
+
+int ofx_mesh_info(ofx_ctx* c, int* out5) {
+    CHECK_CTX(c);
+    if (!c->has_mesh) { set_error("no mesh"); return OFX_ESTATE; }
+    out5[0] = c->m.C; out5[1] = c->m.F; out5[2] = c->m.B; out5[3] = c->m.T;
+    out5[4] = std::max(c->m.nLevLo, c->m.nLevUp);
+    return 0;
+}
+
+int ofx_config_set(ofx_ctx* c, const ofx_config* cfg) {
+    CHECK_CTX(c);
+    if (!cfg) { set_error("null config"); return OFX_EINVAL; }
+    if (cfg->ischeme != 1 && cfg->ischeme != 12) { set_error("ischeme must be 1 (Roe) or 12 (SLAU2)"); return OFX_EINVAL; }
+    if (cfg->ilim < -1 || cfg->ilim > 2) { set_error("ilim must be -1..2"); return OFX_EINVAL; }
+    if (cfg->vismodel != 0 && cfg->vismodel != 1 && cfg->vismodel != 3) { set_error("vismodel must be 0, 1 or 3"); return OFX_EINVAL; }
+    if (cfg->ivischeme < 0 || cfg->ivischeme > 2 || cfg->iturb_visflux < 0 || cfg->iturb_visflux > 2) {
+        set_error("face-gradient methods 0..2 supported"); return OFX_EINVAL;
+    }
+    if (cfg->turb_ilim != -1 || cfg->tns_ilim != -1) { set_error("only turb_ilim = tns_ilim = -1 supported"); return OFX_EINVAL; }
+    if (cfg->timestepModel != 0) { set_error("only local time stepping supported"); return OFX_EINVAL; }
+    if (cfg->flux_fp32 != 0 && cfg->flux_fp32 != 1) { set_error("flux_fp32 must be 0 or 1"); return OFX_EINVAL; }
+    c->cfg = *cfg;
+    set_consts(c);
+    c->has_cfg = true;
+    return 0;
+}
+
+int ofx_init_freestream(ofx_ctx* c) {
+    CHECK_READY(c);
+    OFX_TRY(init_freestream(c));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int field_view(ofx_ctx* c, int field, double** p, size_t* n) {
+    const size_t T = c->m.T, F = c->m.F, C = c->m.C;
+    DevFields& d = c->d;
+    DevMesh& m = c->m;
+    switch (field) {
+        case OFX_F_Q: *p = d.q; *n = 5 * T; return 0;
+        case OFX_F_TURBQ: *p = d.turbq; *n = T; return 0;
+        case OFX_F_RES: *p = d.res; *n = 5 * T; return 0;
+        case OFX_F_TURBRES: *p = d.turbres; *n = T; return 0;
+        case OFX_F_LIMITER: *p = d.limiter; *n = 5 * T; return 0;
+        case OFX_F_TIMESTEP: *p = d.timestep; *n = T; return 0;
+        case OFX_F_INVSR: *p = d.invsr; *n = T; return 0;
+        case OFX_F_VISSR: *p = d.vissr; *n = T; return 0;
+        case OFX_F_DQ: *p = d.dq; *n = 5 * T; return 0;
+        case OFX_F_TURBDQ: *p = d.turbdq; *n = T; return 0;
+        case OFX_F_TEMPR: *p = d.tempr; *n = T; return 0;
+        case OFX_F_VISL: *p = d.visl; *n = T; return 0;
+        case OFX_F_VIST: *p = d.vist; *n = T; return 0;
+        case OFX_F_DIST: *p = m.dist; *n = C; return 0;
+        case OFX_F_VOL: *p = m.vol; *n = T; return 0;
+        case OFX_F_TURBIMPSR: *p = d.turbimpsr; *n = T; return 0;
+        case OFX_F_IMPSR: *p = d.impsr; *n = T; return 0;
+        default: break;
+    }
+    (void)F;
+    return -1;
+}
+
+int ofx_get_field(ofx_ctx* c, int field, double* host) {
+    CHECK_READY(c);
+    if (!host) { set_error("null host pointer"); return OFX_EINVAL; }
+    const size_t T = c->m.T, F = c->m.F;
+    const DevMesh& m = c->m;
+    auto D2H = [&](double* h, const double* dsrc, size_t n) {
+        return hipMemcpyAsync(h, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    };
+    if (field == OFX_F_DQDX) {
+        OFX_TRY(fill_ghost_grad(c));
+        // [3][5][T]: x, y, z of rho,u,v,w,p
+        for (int a = 0; a < 3; ++a) OFX_HIP(D2H(host + a * 5 * T, c->d.grad + a * 6 * T, 5 * T));
+    } else if (field == OFX_F_DTDX) {
+        OFX_TRY(fill_ghost_grad(c));
+        for (int a = 0; a < 3; ++a) OFX_HIP(D2H(host + a * T, c->d.grad + (a * 6 + 5) * T, T));
+    } else if (field == OFX_F_TURBDQDX) {
+        OFX_TRY(fill_ghost_grad(c));
+        OFX_HIP(D2H(host, c->d.tgrad, 3 * T));
+    } else if (field == OFX_F_XCC) {
+        OFX_HIP(D2H(host, m.xcc, T)); OFX_HIP(D2H(host + T, m.ycc, T)); OFX_HIP(D2H(host + 2 * T, m.zcc, T));
+    } else if (field == OFX_F_FACE_NORMAL) {
+        OFX_HIP(D2H(host, m.xfn, F)); OFX_HIP(D2H(host + F, m.yfn, F));
+        OFX_HIP(D2H(host + 2 * F, m.zfn, F)); OFX_HIP(D2H(host + 3 * F, m.area, F));
+    } else if (field == OFX_F_FACE_CENTER) {
+        OFX_HIP(D2H(host, m.xfc, F)); OFX_HIP(D2H(host + F, m.yfc, F)); OFX_HIP(D2H(host + 2 * F, m.zfc, F));
+    } else {
+        double* p; size_t n;
+        if (field_view(c, field, &p, &n)) { set_error("unknown field id"); return OFX_EINVAL; }
+        OFX_HIP(D2H(host, p, n));
+    }
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ofx_set_field(ofx_ctx* c, int field, const double* host) {
+    CHECK_READY(c);
+    if (!host) { set_error("null host pointer"); return OFX_EINVAL; }
+    double* p; size_t n;
+    if (field == OFX_F_DIST || field == OFX_F_VOL || field_view(c, field, &p, &n)) {
+        set_error("field cannot be set"); return OFX_EINVAL;
+    }
+    OFX_HIP(hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ofx_ns_boundary(ofx_ctx* c) { CHECK_READY(c); return ns_boundary(c); }
+int ofx_sa_boundary(ofx_ctx* c) { CHECK_READY(c); return sa_boundary(c); }
+int ofx_timestep(ofx_ctx* c, int iter) { CHECK_READY(c); return ns_timestep(c, iter); }
+int ofx_residual_ns(ofx_ctx* c) { CHECK_READY(c); return ns_residual(c); }
+int ofx_residual_sa(ofx_ctx* c) { CHECK_READY(c); return sa_residual(c); }
+int ofx_lusgs_ns(ofx_ctx* c) { CHECK_READY(c); return ns_lusgs(c); }
+int ofx_lusgs_sa(ofx_ctx* c) { CHECK_READY(c); return sa_lusgs(c); }
+int ofx_update_ns(ofx_ctx* c) { CHECK_READY(c); return ns_update(c); }
+int ofx_update_sa(ofx_ctx* c) { CHECK_READY(c); return sa_update(c); }
+
+int ofx_norms_get(ofx_ctx* c, ofx_norms* out) {
+    CHECK_READY(c);
+    if (!out) { set_error("null out"); return OFX_EINVAL; }
+    OFX_TRY(norms(c, out));
+    for (int e = 0; e < 5; ++e)
+        if (std::isnan(out->ns[e])) { set_error("NotANumber in NS residual"); return OFX_ENAN; }
+    if (std::isnan(out->sa)) { set_error("NotANumber in SA residual"); return OFX_ENAN; }
+    return 0;
+}
+
+// One StrongIter per step: NS then SA, each the TimeIntegral::Lusgs task list
+// ZERO_DQ_FIELD, CALC_TIME_STEP, LOAD_RESIDUALS, UPDATE_RESIDUALS, LU-SGS
+// sweeps, UPDATE_FLOWFIELD_LUSGS, CALC_BOUNDARY.
+int ofx_iterate(ofx_ctx* c, int first_iter, int n, ofx_norms* norms_out) {
+    CHECK_READY(c);
+    if (n < 0) { set_error("n must be >= 0"); return OFX_EINVAL; }
+    for (int s = 0; s < n; ++s) {
+        OFX_TRY(ns_timestep(c, first_iter + s));
+        OFX_TRY(ns_residual(c));
+        OFX_TRY(ns_lusgs(c));
+        OFX_TRY(ns_update(c));
+        OFX_TRY(ns_boundary(c));
+        if (c->k.vismodel == 3) {
+            OFX_TRY(sa_residual(c));
+            OFX_TRY(sa_lusgs(c));
+            OFX_TRY(sa_update(c));
+            OFX_TRY(sa_boundary(c));
+        }
+        if (norms_out) OFX_TRY(ofx_norms_get(c, norms_out + s));
+    }
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ofx_aero(ofx_ctx* c, double* out4) {
+    CHECK_READY(c);
+    if (!out4) { set_error("null out"); return OFX_EINVAL; }
+    return aero(c, out4);
+}
+
+int ofx_profile_last(ofx_ctx* c, double* out2) {
+    CHECK_READY(c);
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    OFX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    out2[0] = 0.0;
+    out2[1] = ms;
+    return 0;
+}
+
+int ofx_sync(ofx_ctx* c) {
+    CHECK_CTX(c);
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

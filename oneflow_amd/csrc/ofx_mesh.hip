@@ -1,0 +1,447 @@
+// ofx_mesh.hip -- grid upload and geometry build on the device.
+//
+// Face normals/areas/centres, cell centres/volumes, ghost-cell mirrors and the
+// wall distance are computed by GPU kernels that keep the reference's
+// arithmetic order (codes/geometry/src/FaceMesh.cpp:112-287,
+// codes/geometry/src/UnsGrid.cpp:588-887, codes/geometry/src/WallDist.cpp:136-358).
+// The cell -> face lists and the LU-SGS level schedule are integer
+// preprocessing done once on the host.
+#include "ofx_common.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace ofx {
+
+// ---------------------------------------------------------------- face metrics
+
+// FaceMesh::CalcFaceNormal2D + CalcFaceCenter2D (codes/geometry/src/FaceMesh.cpp:112-155)
+__global__ void k_face_metrics_2d(DevMesh m) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    int o = m.fnOff[f];
+    int p1 = m.fnNodes[o], p2 = m.fnNodes[o + 1];
+    double xfn = m.yn[p2] - m.yn[p1];
+    double yfn = m.xn[p1] - m.xn[p2];
+    double zfn = 0.0;
+    double area = sqrt(sqr3(xfn, yfn, zfn));
+    double oArea = 1.0 / (area + SMALL);
+    m.xfn[f] = xfn * oArea;
+    m.yfn[f] = yfn * oArea;
+    m.zfn[f] = zfn * oArea;
+    m.area[f] = area;
+    m.xfc[f] = 0.5 * (m.xn[p1] + m.xn[p2]);
+    m.yfc[f] = 0.5 * (m.yn[p1] + m.yn[p2]);
+    m.zfc[f] = 0.5 * (m.zn[p1] + m.zn[p2]);
+}
+
+// FaceMesh::CalcFaceNormal3D + CalcFaceCenter3D (codes/geometry/src/FaceMesh.cpp:157-287)
+__global__ void k_face_metrics_3d(DevMesh m) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    int o = m.fnOff[f], n = m.fnOff[f + 1] - o;
+    const int* fv = m.fnNodes + o;
+    double xfn = 0, yfn = 0, zfn = 0;
+    for (int i = 0; i < n; ++i) {
+        int p1 = fv[i], p2 = fv[(i + 1) % n];
+        double dx1 = m.xn[p1], dy1 = m.yn[p1], dz1 = m.zn[p1];
+        double dx2 = m.xn[p2], dy2 = m.yn[p2], dz2 = m.zn[p2];
+        xfn += 0.5 * (dy1 * dz2 - dy2 * dz1);
+        yfn += 0.5 * (dz1 * dx2 - dz2 * dx1);
+        zfn += 0.5 * (dx1 * dy2 - dx2 * dy1);
+    }
+    double area = sqrt(sqr3(xfn, yfn, zfn));
+    double oArea = 1.0 / (area + SMALL);
+    xfn *= oArea; yfn *= oArea; zfn *= oArea;
+    m.xfn[f] = xfn; m.yfn[f] = yfn; m.zfn[f] = zfn; m.area[f] = area;
+    const double third = 1.0 / 3.0;
+    double x0 = 0, y0 = 0, z0 = 0;
+    for (int i = 0; i < n; ++i) { x0 += m.xn[fv[i]]; y0 += m.yn[fv[i]]; z0 += m.zn[fv[i]]; }
+    double factor = 1.0 / n;
+    x0 *= factor; y0 *= factor; z0 *= factor;
+    double xc = 0, yc = 0, zc = 0, sarea = 0;
+    for (int i = 0; i < n; ++i) {
+        int p1 = fv[i], p2 = fv[(i + 1) % n];
+        double dx1 = m.xn[p1] - x0, dy1 = m.yn[p1] - y0, dz1 = m.zn[p1] - z0;
+        double dx2 = m.xn[p2] - x0, dy2 = m.yn[p2] - y0, dz2 = m.zn[p2] - z0;
+        double x00 = third * (m.xn[p1] + m.xn[p2] + x0);
+        double y00 = third * (m.yn[p1] + m.yn[p2] + y0);
+        double z00 = third * (m.zn[p1] + m.zn[p2] + z0);
+        double anx = dy1 * dz2 - dy2 * dz1;
+        double any = dz1 * dx2 - dz2 * dx1;
+        double anz = dx1 * dy2 - dx2 * dy1;
+        double fa = 0.5 * sqrt(sqr3(anx, any, anz));
+        double norm = anx * xfn + any * yfn + anz * zfn;
+        if (norm < 0.0) fa = -fa;
+        sarea += fa;
+        xc += fa * x00; yc += fa * y00; zc += fa * z00;
+    }
+    double osarea = 1.0 / (sarea + SMALL);
+    m.xfc[f] = xc * osarea; m.yfc[f] = yc * osarea; m.zfc[f] = zc * osarea;
+}
+
+// ---------------------------------------------------------------- cell metrics
+
+// UnsGrid::CalcCellCenterVol2D as a per-cell gather over the cell's faces in
+// ascending face order (codes/geometry/src/UnsGrid.cpp:588-667).
+__global__ void k_cell_metrics_2d(DevMesh m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double xc = 0, yc = 0, zc = 0, v = 0;
+    for (int k = m.c2fOff[i]; k < m.c2fOff[i + 1]; ++k) {
+        int e = m.c2f[k];
+        int f = e >= 0 ? e : ~e;
+        double dot = (m.xfc[f] * m.xfn[f] + m.yfc[f] * m.yfn[f] + m.zfc[f] * m.zfn[f]) * m.area[f];
+        if (e >= 0) {
+            xc += m.xfc[f] * dot; yc += m.yfc[f] * dot; zc += m.zfc[f] * dot; v += dot;
+        } else {
+            xc -= m.xfc[f] * dot; yc -= m.yfc[f] * dot; zc -= m.zfc[f] * dot; v -= dot;
+        }
+    }
+    double tmp = 1.0 / (1.5 * v + SMALL);
+    xc *= tmp; yc *= tmp; zc *= tmp;
+    v *= 0.5;
+    if (v <= 0.0) v = -v;
+    m.xcc[i] = xc; m.ycc[i] = yc; m.zcc[i] = zc; m.vol[i] = v;
+}
+
+// UnsGrid::CalcCellCenterVol3D, per-cell gather (codes/geometry/src/UnsGrid.cpp:740-860)
+__global__ void k_cell_metrics_3d(DevMesh m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double xc = 0, yc = 0, zc = 0, v = 0;
+    for (int k = m.c2fOff[i]; k < m.c2fOff[i + 1]; ++k) {
+        int e = m.c2f[k];
+        int f = e >= 0 ? e : ~e;
+        int o = m.fnOff[f], n = m.fnOff[f + 1] - o;
+        const int* fv = m.fnNodes + o;
+        for (int j = 0; j < n; ++j) {
+            int p2 = fv[j], p3 = fv[(j + 1) % n];
+            double x21 = m.xn[p2] - m.xfc[f], y21 = m.yn[p2] - m.yfc[f], z21 = m.zn[p2] - m.zfc[f];
+            double x31 = m.xn[p3] - m.xfc[f], y31 = m.yn[p3] - m.yfc[f], z31 = m.zn[p3] - m.zfc[f];
+            double nx = y21 * z31 - y31 * z21;
+            double ny = z21 * x31 - z31 * x21;
+            double nz = x21 * y31 - x31 * y21;
+            double trix = m.xfc[f] + m.xn[p2] + m.xn[p3];
+            double triy = m.yfc[f] + m.yn[p2] + m.yn[p3];
+            double triz = m.zfc[f] + m.zn[p2] + m.zn[p3];
+            double tmp = nx * trix + ny * triy + nz * triz;
+            trix *= tmp; triy *= tmp; triz *= tmp;
+            if (e >= 0) { xc += trix; yc += triy; zc += triz; v += tmp; }
+            else { xc -= trix; yc -= triy; zc -= triz; v -= tmp; }
+        }
+    }
+    double tmp = 1.0 / (4.0 * v + SMALL);
+    xc *= tmp; yc *= tmp; zc *= tmp;
+    v /= 18.0;
+    if (v <= 0.0) v = -v;
+    m.xcc[i] = xc; m.ycc[i] = yc; m.zcc[i] = zc; m.vol[i] = v;
+}
+
+// ghost mirror (codes/geometry/src/UnsGrid.cpp:668-690)
+__global__ void k_ghost_centres(DevMesh m) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.B) return;
+    int l = m.lc[f], r = m.C + f;
+    if (m.area[f] > SMALL) {
+        double tmp = 2.0 * ((m.xcc[l] - m.xfc[f]) * m.xfn[f] + (m.ycc[l] - m.yfc[f]) * m.yfn[f] +
+                            (m.zcc[l] - m.zfc[f]) * m.zfn[f]);
+        m.xcc[r] = m.xcc[l] - m.xfn[f] * tmp;
+        m.ycc[r] = m.ycc[l] - m.yfn[f] * tmp;
+        m.zcc[r] = m.zcc[l] - m.zfn[f] * tmp;
+    } else {
+        m.xcc[r] = -m.xcc[l] + 2.0 * m.xfc[f];
+        m.ycc[r] = -m.ycc[l] + 2.0 * m.yfc[f];
+        m.zcc[r] = -m.zcc[l] + 2.0 * m.zfc[f];
+    }
+    m.vol[r] = m.vol[l];
+}
+
+// ---------------------------------------------------------------- wall distance
+
+// gte DCPQuery point/segment, squared distance (codes/gte/include/GteDistPointSegment.h)
+__device__ double seg_dist2(const double p[3], const double a[3], const double b[3]) {
+    double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double df[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+    double t = d[0] * df[0] + d[1] * df[1] + d[2] * df[2];
+    double cl[3];
+    if (t >= 0.0) { cl[0] = b[0]; cl[1] = b[1]; cl[2] = b[2]; }
+    else {
+        df[0] = p[0] - a[0]; df[1] = p[1] - a[1]; df[2] = p[2] - a[2];
+        t = d[0] * df[0] + d[1] * df[1] + d[2] * df[2];
+        if (t <= 0.0) { cl[0] = a[0]; cl[1] = a[1]; cl[2] = a[2]; }
+        else {
+            double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (sl > 0.0) {
+                t /= sl;
+                cl[0] = a[0] + t * d[0]; cl[1] = a[1] + t * d[1]; cl[2] = a[2] + t * d[2];
+            } else { cl[0] = a[0]; cl[1] = a[1]; cl[2] = a[2]; }
+        }
+    }
+    df[0] = p[0] - cl[0]; df[1] = p[1] - cl[1]; df[2] = p[2] - cl[2];
+    return df[0] * df[0] + df[1] * df[1] + df[2] * df[2];
+}
+
+// gte DistancePointTriangleExact, squared distance (codes/gte/include/GteDistPointTriangleExact.h)
+__device__ double tri_dist2(const double P[3], const double V0[3], const double V1[3], const double V2[3]) {
+    double diff[3] = {P[0] - V0[0], P[1] - V0[1], P[2] - V0[2]};
+    double e0[3] = {V1[0] - V0[0], V1[1] - V0[1], V1[2] - V0[2]};
+    double e1[3] = {V2[0] - V0[0], V2[1] - V0[1], V2[2] - V0[2]};
+    double a00 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+    double a01 = e0[0] * e1[0] + e0[1] * e1[1] + e0[2] * e1[2];
+    double a11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    double b0 = -(diff[0] * e0[0] + diff[1] * e0[1] + diff[2] * e0[2]);
+    double b1 = -(diff[0] * e1[0] + diff[1] * e1[1] + diff[2] * e1[2]);
+    double det = a00 * a11 - a01 * a01;
+    double t0 = a01 * b1 - a11 * b0;
+    double t1 = a01 * b0 - a00 * b1;
+    if (t0 + t1 <= det) {
+        if (t0 < 0) {
+            if (t1 < 0) {
+                if (b0 < 0) { t1 = 0; if (-b0 >= a00) t0 = 1; else t0 = -b0 / a00; }
+                else { t0 = 0; if (b1 >= 0) t1 = 0; else if (-b1 >= a11) t1 = 1; else t1 = -b1 / a11; }
+            } else {
+                t0 = 0;
+                if (b1 >= 0) t1 = 0; else if (-b1 >= a11) t1 = 1; else t1 = -b1 / a11;
+            }
+        } else if (t1 < 0) {
+            t1 = 0;
+            if (b0 >= 0) t0 = 0; else if (-b0 >= a00) t0 = 1; else t0 = -b0 / a00;
+        } else {
+            double invDet = 1.0 / det;
+            t0 *= invDet; t1 *= invDet;
+        }
+    } else {
+        double tmp0, tmp1, numer, denom;
+        if (t0 < 0) {
+            tmp0 = a01 + b0; tmp1 = a11 + b1;
+            if (tmp1 > tmp0) {
+                numer = tmp1 - tmp0; denom = a00 - 2 * a01 + a11;
+                if (numer >= denom) { t0 = 1; t1 = 0; } else { t0 = numer / denom; t1 = 1 - t0; }
+            } else {
+                t0 = 0;
+                if (tmp1 <= 0) t1 = 1; else if (b1 >= 0) t1 = 0; else t1 = -b1 / a11;
+            }
+        } else if (t1 < 0) {
+            tmp0 = a01 + b1; tmp1 = a00 + b0;
+            if (tmp1 > tmp0) {
+                numer = tmp1 - tmp0; denom = a00 - 2 * a01 + a11;
+                if (numer >= denom) { t1 = 1; t0 = 0; } else { t1 = numer / denom; t0 = 1 - t1; }
+            } else {
+                t1 = 0;
+                if (tmp1 <= 0) t0 = 1; else if (b0 >= 0) t0 = 0; else t0 = -b0 / a00;
+            }
+        } else {
+            numer = a11 + b1 - a01 - b0;
+            if (numer <= 0) { t0 = 0; t1 = 1; }
+            else {
+                denom = a00 - 2 * a01 + a11;
+                if (numer >= denom) { t0 = 1; t1 = 0; } else { t0 = numer / denom; t1 = 1 - t0; }
+            }
+        }
+    }
+    double cp[3] = {V0[0] + t0 * e0[0] + t1 * e1[0], V0[1] + t0 * e0[1] + t1 * e1[1], V0[2] + t0 * e0[2] + t1 * e1[2]};
+    double d[3] = {P[0] - cp[0], P[1] - cp[1], P[2] - cp[2]};
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// CalcWallDist: brute force over all wall faces, min of squared distance, sqrt
+// (codes/geometry/src/WallDist.cpp:136-358).  One thread per cell; wall-face
+// node coordinates are staged through LDS in tiles shared by the block.
+constexpr int WD_TILE = 256;
+__global__ void __launch_bounds__(256) k_wall_distance(DevMesh m) {
+    __shared__ double sx[WD_TILE * 4], sy[WD_TILE * 4], sz[WD_TILE * 4];
+    __shared__ int sn[WD_TILE];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = i < m.C;
+    double p[3] = {0, 0, 0};
+    if (live) { p[0] = m.xcc[i]; p[1] = m.ycc[i]; p[2] = m.zcc[i]; }
+    double best = LARGE;
+    for (int base = 0; base < m.nWall; base += WD_TILE) {
+        int nt = min(WD_TILE, m.nWall - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+            int f = m.wallFaces[base + t];
+            int o = m.fnOff[f], n = m.fnOff[f + 1] - o;
+            sn[t] = n;
+            for (int j = 0; j < 4; ++j) {
+                int pj = m.fnNodes[o + (j < n ? j : 0)];
+                sx[t * 4 + j] = m.xn[pj]; sy[t * 4 + j] = m.yn[pj]; sz[t * 4 + j] = m.zn[pj];
+            }
+        }
+        __syncthreads();
+        if (!live) continue;
+        for (int t = 0; t < nt; ++t) {
+            int n = sn[t];
+            double w;
+            double a[3] = {sx[t * 4], sy[t * 4], sz[t * 4]};
+            double b[3] = {sx[t * 4 + 1], sy[t * 4 + 1], sz[t * 4 + 1]};
+            if (n <= 2) {
+                w = seg_dist2(p, a, b);
+            } else if (n == 3) {
+                double c3[3] = {sx[t * 4 + 2], sy[t * 4 + 2], sz[t * 4 + 2]};
+                w = tri_dist2(p, a, b, c3);
+            } else {
+                double ctr[3] = {0, 0, 0};
+                for (int k = 0; k < n; ++k) { ctr[0] += sx[t * 4 + k]; ctr[1] += sy[t * 4 + k]; ctr[2] += sz[t * 4 + k]; }
+                double coef = 1.0 / n;
+                ctr[0] *= coef; ctr[1] *= coef; ctr[2] *= coef;
+                w = LARGE;
+                for (int k = 0; k < n; ++k) {
+                    int k1 = (k + 1) % n;
+                    double u[3] = {sx[t * 4 + k], sy[t * 4 + k], sz[t * 4 + k]};
+                    double v[3] = {sx[t * 4 + k1], sy[t * 4 + k1], sz[t * 4 + k1]};
+                    double dd = tri_dist2(p, u, v, ctr);
+                    w = (w < dd) ? w : dd;
+                }
+            }
+            if (best > w) best = w;
+        }
+    }
+    if (live) m.dist[i] = sqrt(best);
+}
+
+// ---------------------------------------------------------------- host side
+
+int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, const int* fnc,
+               const int* fnn, const int* lcIn, const int* rcIn, const int* bct) {
+    DevMesh& m = c->m;
+    const int C = m.C, F = m.F, B = m.B, T = m.T;
+    // face -> node CSR, orientation fix (codes/geometry/src/UnsGrid.cpp:160-170), NormalizeBc
+    std::vector<int> fnOff(F + 1, 0);
+    for (int f = 0; f < F; ++f) {
+        if (fnc[f] < 2 || (m.dim == 2 && fnc[f] != 2) || (m.dim == 3 && fnc[f] > 4)) {
+            set_error("ofx_mesh_upload: unsupported face node count"); return OFX_EINVAL;
+        }
+        fnOff[f + 1] = fnOff[f] + fnc[f];
+    }
+    std::vector<int> nodes(fnn, fnn + fnOff[F]);
+    std::vector<int> lc(lcIn, lcIn + F), rc(rcIn, rcIn + F);
+    for (int f = 0; f < F; ++f) {
+        if (lc[f] < 0) {
+            std::reverse(nodes.begin() + fnOff[f], nodes.begin() + fnOff[f + 1]);
+            std::swap(lc[f], rc[f]);
+        }
+    }
+    for (int k = 0; k < fnOff[F]; ++k)
+        if (nodes[k] < 0 || nodes[k] >= m.nNodes) { set_error("ofx_mesh_upload: face node out of range"); return OFX_EINVAL; }
+    for (int f = 0; f < B; ++f) {
+        if (lc[f] < 0 || lc[f] >= C) { set_error("ofx_mesh_upload: boundary face owner out of range"); return OFX_EINVAL; }
+        if (bct[f] < 1 || bct[f] > 6) { set_error("ofx_mesh_upload: unsupported boundary type (1..6 supported)"); return OFX_EINVAL; }
+        rc[f] = C + f;
+    }
+    for (int f = B; f < F; ++f)
+        if (lc[f] < 0 || lc[f] >= C || rc[f] < 0 || rc[f] >= C || lc[f] == rc[f]) {
+            set_error("ofx_mesh_upload: interior face cell index out of range"); return OFX_EINVAL;
+        }
+    // CellTopo::CalcC2f (codes/geometry/src/CellTopo.cpp:95-119): ascending face ids
+    std::vector<int> off(T + 1, 0);
+    for (int f = 0; f < B; ++f) off[lc[f] + 1]++;
+    for (int f = B; f < F; ++f) { off[lc[f] + 1]++; off[rc[f] + 1]++; }
+    for (int i = 0; i < T; ++i) off[i + 1] += off[i];
+    std::vector<int> c2f(off[T]);
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int f = 0; f < B; ++f) c2f[pos[lc[f]]++] = f;
+    for (int f = B; f < F; ++f) { c2f[pos[lc[f]]++] = f; c2f[pos[rc[f]]++] = ~f; }
+    int maxf = 0;
+    for (int i = 0; i < C; ++i) maxf = std::max(maxf, off[i + 1] - off[i]);
+    m.maxFacesPerCell = maxf;
+    // LU-SGS level schedule (exact: a cell depends only on lower / upper
+    // neighbours, UNsLusgs::CanNotLowerSolve / CanNotUpperSolve,
+    // codes/uns/src/UNsLusgs.cpp:174-196)
+    std::vector<int> lev(C, 0), levU(C, 0);
+    int nlo = 0, nup = 0;
+    for (int i = 0; i < C; ++i) {
+        int mx = -1;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int e = c2f[k], f = e >= 0 ? e : ~e;
+            if (f < B) continue;
+            int j = (e >= 0) ? rc[f] : lc[f];
+            if (j < i) mx = std::max(mx, lev[j]);
+        }
+        lev[i] = mx + 1;
+        nlo = std::max(nlo, lev[i] + 1);
+    }
+    for (int i = C - 1; i >= 0; --i) {
+        int mx = -1;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int e = c2f[k], f = e >= 0 ? e : ~e;
+            if (f < B) continue;
+            int j = (e >= 0) ? rc[f] : lc[f];
+            if (j > i) mx = std::max(mx, levU[j]);
+        }
+        levU[i] = mx + 1;
+        nup = std::max(nup, levU[i] + 1);
+    }
+    auto order_by = [&](const std::vector<int>& L, int nl, std::vector<int>& ord, std::vector<int>& lo) {
+        lo.assign(nl + 1, 0);
+        for (int i = 0; i < C; ++i) lo[L[i] + 1]++;
+        for (int l = 0; l < nl; ++l) lo[l + 1] += lo[l];
+        ord.assign(C, 0);
+        std::vector<int> p(lo.begin(), lo.end() - 1);
+        for (int i = 0; i < C; ++i) ord[p[L[i]]++] = i;
+    };
+    std::vector<int> loOrd, upOrd;
+    order_by(lev, nlo, loOrd, c->sched.loOff);
+    order_by(levU, nup, upOrd, c->sched.upOff);
+    m.nLevLo = nlo; m.nLevUp = nup;
+    std::vector<int> wall;
+    for (int f = 0; f < B; ++f) if (bct[f] == BC_SOLID_SURFACE) wall.push_back(f);
+    m.nWall = (int)wall.size();
+
+    // ---- device allocations
+    auto A = [&](void** p, size_t bytes) { return alloc_bytes(c, p, bytes); };
+    size_t dF = sizeof(double) * F, dT = sizeof(double) * T;
+    OFX_TRY(A((void**)&m.xn, sizeof(double) * m.nNodes));
+    OFX_TRY(A((void**)&m.yn, sizeof(double) * m.nNodes));
+    OFX_TRY(A((void**)&m.zn, sizeof(double) * m.nNodes));
+    OFX_TRY(A((void**)&m.fnOff, sizeof(int) * (F + 1)));
+    OFX_TRY(A((void**)&m.fnNodes, sizeof(int) * std::max(1, fnOff[F])));
+    double** fa[] = {&m.xfn, &m.yfn, &m.zfn, &m.area, &m.xfc, &m.yfc, &m.zfc};
+    for (double** p : fa) OFX_TRY(A((void**)p, dF));
+    double** ca[] = {&m.xcc, &m.ycc, &m.zcc, &m.vol};
+    for (double** p : ca) OFX_TRY(A((void**)p, dT));
+    OFX_TRY(A((void**)&m.dist, sizeof(double) * C));
+    OFX_TRY(A((void**)&m.lc, sizeof(int) * F));
+    OFX_TRY(A((void**)&m.rc, sizeof(int) * F));
+    OFX_TRY(A((void**)&m.bct, sizeof(int) * std::max(1, B)));
+    OFX_TRY(A((void**)&m.c2fOff, sizeof(int) * (T + 1)));
+    OFX_TRY(A((void**)&m.c2f, sizeof(int) * std::max(1, off[T])));
+    OFX_TRY(A((void**)&m.loOrder, sizeof(int) * C));
+    OFX_TRY(A((void**)&m.upOrder, sizeof(int) * C));
+    OFX_TRY(A((void**)&m.loOff, sizeof(int) * (nlo + 1)));
+    OFX_TRY(A((void**)&m.upOff, sizeof(int) * (nup + 1)));
+    OFX_TRY(A((void**)&m.wallFaces, sizeof(int) * std::max(1, m.nWall)));
+    OFX_TRY(upload(c, m.xn, x, sizeof(double) * m.nNodes));
+    OFX_TRY(upload(c, m.yn, y, sizeof(double) * m.nNodes));
+    OFX_TRY(upload(c, m.zn, z, sizeof(double) * m.nNodes));
+    OFX_TRY(upload(c, m.fnOff, fnOff.data(), sizeof(int) * (F + 1)));
+    OFX_TRY(upload(c, m.fnNodes, nodes.data(), sizeof(int) * fnOff[F]));
+    OFX_TRY(upload(c, m.lc, lc.data(), sizeof(int) * F));
+    OFX_TRY(upload(c, m.rc, rc.data(), sizeof(int) * F));
+    if (B) OFX_TRY(upload(c, m.bct, bct, sizeof(int) * B));
+    OFX_TRY(upload(c, m.c2fOff, off.data(), sizeof(int) * (T + 1)));
+    OFX_TRY(upload(c, m.c2f, c2f.data(), sizeof(int) * off[T]));
+    OFX_TRY(upload(c, m.loOrder, loOrd.data(), sizeof(int) * C));
+    OFX_TRY(upload(c, m.upOrder, upOrd.data(), sizeof(int) * C));
+    OFX_TRY(upload(c, m.loOff, c->sched.loOff.data(), sizeof(int) * (nlo + 1)));
+    OFX_TRY(upload(c, m.upOff, c->sched.upOff.data(), sizeof(int) * (nup + 1)));
+    if (m.nWall) OFX_TRY(upload(c, m.wallFaces, wall.data(), sizeof(int) * m.nWall));
+
+    // ---- geometry kernels
+    const int TB = 256;
+    if (m.dim == 2) k_face_metrics_2d<<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    else k_face_metrics_3d<<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    OFX_HIP(hipGetLastError());
+    if (m.dim == 2) k_cell_metrics_2d<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    else k_cell_metrics_3d<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    OFX_HIP(hipGetLastError());
+    if (B) k_ghost_centres<<<(B + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    OFX_HIP(hipGetLastError());
+    k_wall_distance<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    OFX_HIP(hipGetLastError());
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace ofx

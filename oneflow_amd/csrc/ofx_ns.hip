@@ -1,0 +1,878 @@
+// ofx_ns.hip -- Navier-Stokes kernels: boundary conditions, local time step,
+// residual (Green-Gauss gradients, limiter, MUSCL + Roe/SLAU2, viscous flux,
+// ordered atomic-free gather), level-scheduled LU-SGS and the update.
+//
+// Data layout: SoA doubles, cell arrays [eq][T] (T = C + B, ghosts last),
+// face arrays [eq][F], faces boundary-first.  Gathers walk each cell's face
+// list in ascending face id, which is the order the reference's serial
+// scatter AddF2CField (codes/usolver/src/UCom.cpp:144-171) adds into a cell,
+// so the FP64 residual is the reference's bit for bit and no atomics are used.
+#include "ofx_physics.h"
+
+namespace ofx {
+
+constexpr int TB = 256;
+
+static inline int nblk(int n) { return (n + TB - 1) / TB; }
+
+// ============================================================ boundary
+
+// NsCalcGamaT + CalcLaminarViscosity on interior cells (codes/ns/src/NsSolverImp.cpp:72-79)
+__global__ void __launch_bounds__(TB) k_ns_cellstate(DevMesh m, DevFields d, Consts k) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double t = temperature(d.q[i], d.q[4 * m.T + i], k.statecoef);
+    d.tempr[i] = t;
+    d.visl[i] = sutherland(t, k.ts_suth);
+}
+
+// UNsBcSolver::CalcBc + NsBcSolver (codes/uns/src/UNsBcSolver.cpp:55-168,
+// codes/ns/src/NsBcSolver.cpp:51-410), then ghost gamma/T/viscosity.
+__global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.B) return;
+    const int T = m.T;
+    int bt = m.bct[f];
+    int l = m.lc[f], r = m.C + f;
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f];
+    const double vfn = 0.0;
+    double s[5], t[5];
+    for (int e = 0; e < 5; ++e) s[e] = d.q[e * T + l];
+    double gama = 0.5 * (k.gama + k.gama);
+    if (bt == BC_EXTRAPOLATION || bt == BC_OUTFLOW) {
+        for (int e = 0; e < 5; ++e) t[e] = s[e];
+    } else if (bt == BC_INFLOW) {
+        for (int e = 0; e < 5; ++e) t[e] = k.inflow[e];
+    } else if (bt == BC_SOLID_SURFACE && k.vismodel != 0) {
+        for (int e = 0; e < 5; ++e) t[e] = s[e];
+        t[1] = -t[1] + 2.0 * 0.0; t[2] = -t[2] + 2.0 * 0.0; t[3] = -t[3] + 2.0 * 0.0;
+        if (k.isowallbc != 0) {
+            // IsothermalVisWallBc (codes/ns/src/NsBcSolver.cpp:271-324)
+            double tlim = 0.1 * k.twall;
+            double rm = s[0], pm = s[4];
+            double temp = d.tempr[l];
+            double pg1 = pm;
+            double tg1 = 2.0 * k.twall - temp;
+            if (tg1 < tlim) tg1 = tlim;
+            double rw = pg1 / (k.statecoef * k.twall * 1.0);
+            double rg1 = 2.0 * rw - rm;
+            rg1 = dmax(0.5 * rw, rg1);
+            t[0] = rg1; t[4] = pg1;
+        }
+    } else if (bt == BC_SYMMETRY || bt == BC_SOLID_SURFACE) {
+        double vn = xfn * s[1] + yfn * s[2] + zfn * s[3] - vfn;
+        for (int e = 0; e < 5; ++e) t[e] = s[e];
+        t[1] = s[1] - 2.0 * xfn * vn;
+        t[2] = s[2] - 2.0 * yfn * vn;
+        t[3] = s[3] - 2.0 * zfn * vn;
+    } else {  // BC_FARFIELD (codes/ns/src/NsBcSolver.cpp:140-236)
+        double rin = s[0], uin = s[1], vin = s[2], win = s[3], pin = s[4];
+        double rref = k.inflow[0], uref = k.inflow[1], vref = k.inflow[2], wref = k.inflow[3], pref = k.inflow[4];
+        double vnref = xfn * uref + yfn * vref + zfn * wref - vfn;
+        double vnin = xfn * uin + yfn * vin + zfn * win - vfn;
+        double cref = sqrt(dabs(k.gama * pref / rref));
+        double cin = sqrt(dabs(gama * pin / rin));
+        double gamm1 = gama - 1.0;
+        double velin = sqrt(sqr3(uin, vin, win));
+        if (velin > cin) {
+            if (vnin >= 0.0) { for (int e = 0; e < 5; ++e) t[e] = s[e]; }
+            else { for (int e = 0; e < 5; ++e) t[e] = k.inflow[e]; }
+        } else {
+            double riemp = vnin + 2.0 * cin / gamm1;
+            double riemm = vnref - 2.0 * cref / gamm1;
+            double vnb = 0.5 * (riemp + riemm);
+            double cb = 0.25 * (riemp - riemm) * gamm1;
+            double vtx, vty, vtz, entr;
+            if (vnb >= 0.0) {
+                entr = pin / pow(rin, gama);
+                vtx = uin - xfn * vnin; vty = vin - yfn * vnin; vtz = win - zfn * vnin;
+            } else {
+                entr = pref / pow(rref, gama);
+                vtx = uref - xfn * vnref; vty = vref - yfn * vnref; vtz = wref - zfn * vnref;
+            }
+            double rb = pow((cb * cb / (entr * gama)), 1.0 / gamm1);
+            t[0] = rb; t[1] = vtx + xfn * vnb; t[2] = vty + yfn * vnb; t[3] = vtz + zfn * vnb;
+            t[4] = cb * cb * rb / gama;
+        }
+    }
+    for (int e = 0; e < 5; ++e) d.q[e * T + r] = t[e];
+    double tt = temperature(t[0], t[4], k.statecoef);
+    d.tempr[r] = tt;
+    d.visl[r] = sutherland(tt, k.ts_suth);
+}
+
+int ns_boundary(ofx_ctx* c) {
+    k_ns_cellstate<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k);
+    OFX_HIP(hipGetLastError());
+    if (c->m.B) k_ns_bc<<<nblk(c->m.B), TB, 0, c->stream>>>(c->m, c->d, c->k);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// ============================================================ time step
+
+__device__ __forceinline__ void atomic_max_nonneg(double* slot, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ void atomic_min_pos(double* slot, double v) {
+    atomicMin(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)__double_as_longlong(v));
+}
+
+// Block reduction of a positive double to its minimum, then one atomic
+// (positive doubles order like their bit patterns, so the result is exact).
+__device__ __forceinline__ void block_min_pos(double v, double* slot) {
+    __shared__ double sm[TB / 64];
+    for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o));
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sm[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = sm[0];
+        for (int j = 1; j < (int)(blockDim.x >> 6); ++j) x = dmin(x, sm[j]);
+        atomic_min_pos(slot, x);
+    }
+}
+
+// Block reduction of a non-negative double to its maximum, then one atomic.
+__device__ __forceinline__ void block_max_nonneg(double v, double* slot) {
+    __shared__ double sm[TB / 64];
+    for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o));
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sm[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = sm[0];
+        for (int j = 1; j < (int)(blockDim.x >> 6); ++j) x = dmax(x, sm[j]);
+        atomic_max_nonneg(slot, x);
+    }
+}
+
+// Reduction slots (doubles updated with integer atomics on their bit patterns):
+//   red[0]      : min time step (min-reduced, starts at LARGE)
+//   red[1..5]   : -qmin per primitive (Venkat, max-reduced, starts at +0)
+//   red[6..10]  : qmax per primitive (max-reduced, starts at +0)
+//   red[14]     : count of cells needing SolutionFix (int)
+constexpr int RED_DTMIN = 0, RED_QMIN = 1, RED_QMAX = 6, RED_NFIX = 14;
+
+__global__ void k_fill(double* p, int n, double v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// UTimeStep::CalcInvSpectrumField / CalcVisSpectrumField / CalcLocalTimeStep
+// (codes/uns/src/UTimeStep.cpp:190-416, codes/ns/src/TimeStep.cpp:43-133):
+// per cell, the face spectral radii are summed in ascending face order.
+__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int T = m.T;
+    double dtv = LARGE;
+    if (i < m.C) {
+        double invsr = 0.0, vissr = 0.0;
+        for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+            int e = m.c2f[kk];
+            int f = e >= 0 ? e : ~e;
+            int l = m.lc[f], r = m.rc[f];
+            double rl = d.q[l], ul = d.q[T + l], vl = d.q[2 * T + l], wl = d.q[3 * T + l], pl = d.q[4 * T + l];
+            double rr = d.q[r], ur = d.q[T + r], vr = d.q[2 * T + r], wr = d.q[3 * T + r], pr = d.q[4 * T + r];
+            double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], area = m.area[f];
+            double vnl = xfn * ul + yfn * vl + zfn * wl - 0.0;
+            double vnr = xfn * ur + yfn * vr + zfn * wr - 0.0;
+            double gama = 0.5 * (k.gama + k.gama);
+            double pm = 0.5 * (pl + pr), rm = 0.5 * (rl + rr);
+            double cm = sqrt(gama * pm / rm);
+            double vn = 0.5 * (vnl + vnr);
+            invsr += 0.5 * area * (dabs(vn) + cm);
+            if (k.vismodel > 0) {
+                double visl = 0.5 * (d.visl[l] + d.visl[r]);
+                double vist = 0.5 * (d.vist[l] + d.vist[r]);
+                double sr = 0.0;
+                if (k.visSRModel == 1) {
+                    double density = 0.5 * (rl + rr);
+                    double c1 = 4.0 / 3.0 * (visl + vist);
+                    double c2 = k.gama * (visl * k.oprl + vist * k.oprt);
+                    double c3 = 2.0 * dmax(c1, c2) / (k.reynolds * density);
+                    sr = (area * area) * c3;
+                } else if (k.visSRModel == 2) {
+                    double dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) +
+                                       zfn * (m.zcc[r] - m.zcc[l]));
+                    double viscosity = visl + vist;
+                    double density = 0.5 * (rl + rr);
+                    double c1 = 2.0 * viscosity / (density * dist * k.reynolds + SMALL);
+                    sr = 0.5 * c1 * area;
+                } else if (k.visSRModel == 3) {
+                    double density = 0.5 * (rl + rr);
+                    sr = (area * area) / (k.reynolds * density + SMALL);
+                }
+                vissr += sr;
+            }
+        }
+        double vol = m.vol[i];
+        double dt = k.cfl * vol / invsr;
+        if (k.vismodel > 0 && k.visTimeStepModel > 0) {
+            double vts = k.cfl * vol / vissr;
+            dt *= vts / (dt + vts);
+        }
+        d.invsr[i] = invsr;
+        d.vissr[i] = vissr;
+        d.timestep[i] = dt;
+        dtv = dt;
+    }
+    block_min_pos(dtv, d.red + RED_DTMIN);
+}
+
+// UTimeStep::ModifyTimeStep clamp and UNsSpectrum::CalcImplicitSpectrum
+// (codes/uns/src/UTimeStep.cpp:368-381, codes/uns/src/UNsSpectrum.cpp:52-94)
+__global__ void __launch_bounds__(TB) k_ns_dtclamp(DevMesh m, DevFields d, Consts k, const double* mindt) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double dt = d.timestep[i];
+    if (k.max_time_ratio > 0) {
+        double mp = k.max_time_ratio * (*mindt);
+        dt = (dt < mp) ? dt : mp;
+        d.timestep[i] = dt;
+    }
+    double imp = (1.0 / dt) * m.vol[i];
+    imp += d.invsr[i];
+    imp += d.vissr[i];
+    d.impsr[i] = imp;
+}
+
+int ns_timestep(ofx_ctx* c, int iter) {
+    const ofx_config& g = c->cfg;
+    double cfl = g.cfled;
+    if (iter < g.ncfl) {
+        double delta = (g.cfled - g.cflst) / g.ncfl;
+        cfl = g.cflst + delta * iter;
+    }
+    c->k.cfl = cfl;
+    k_fill<<<1, 64, 0, c->stream>>>(c->d.red + RED_DTMIN, 1, LARGE);
+    k_ns_timestep<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k);
+    k_ns_dtclamp<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->d.red + RED_DTMIN);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// ============================================================ residual
+
+// Green-Gauss gradient with inverse-distance face weights of rho,u,v,w,p and T
+// (CalcGradGGCellWeight, codes/usolver/src/UGrad.cpp:88-149) as a per-cell
+// gather; optionally the neighbour min/max differences of the limiter
+// (Limiter::CalcMinMaxDiff, codes/usolver/src/ULimiter.cpp:597-644).
+template <bool MINMAX>
+__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int T = m.T;
+    double mn[5], mx[5];
+    bool live = i < m.C;
+    if (live) {
+        double acc[6][3];
+        for (int s = 0; s < 6; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.0;
+        double qi[5];
+        for (int e = 0; e < 5; ++e) { qi[e] = d.q[e * T + i]; mn[e] = qi[e]; mx[e] = qi[e]; }
+        for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+            int e = m.c2f[kk];
+            int f = e >= 0 ? e : ~e;
+            int l = m.lc[f], r = m.rc[f];
+            double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
+            double delt1 = sqrt(sqr3(xfc - m.xcc[l], yfc - m.ycc[l], zfc - m.zcc[l]));
+            double delt2 = sqrt(sqr3(xfc - m.xcc[r], yfc - m.ycc[r], zfc - m.zcc[r]));
+            double delta = 1.0 / (delt1 + delt2 + SMALL);
+            double cl = delt2 * delta, cr = delt1 * delta;
+            double area = m.area[f];
+            double fnxa = m.xfn[f] * area, fnya = m.yfn[f] * area, fnza = m.zfn[f] * area;
+            double ql[5], qr[5];
+            for (int s = 0; s < 5; ++s) { ql[s] = d.q[s * T + l]; qr[s] = d.q[s * T + r]; }
+            double tl = temperature(ql[0], ql[4], k.statecoef);
+            double tr = temperature(qr[0], qr[4], k.statecoef);
+            for (int s = 0; s < 6; ++s) {
+                double vl = s < 5 ? ql[s] : tl, vr = s < 5 ? qr[s] : tr;
+                double value = cl * vl + cr * vr;
+                if (e >= 0) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
+                else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
+            }
+            if (MINMAX && f >= m.B) {
+                const double* qn = (e >= 0) ? qr : ql;
+                for (int s = 0; s < 5; ++s) { mn[s] = dmin(mn[s], qn[s]); mx[s] = dmax(mx[s], qn[s]); }
+            }
+        }
+        double ovol = 1.0 / m.vol[i];
+        for (int s = 0; s < 6; ++s)
+            for (int a = 0; a < 3; ++a) d.grad[(a * 6 + s) * T + i] = acc[s][a] * ovol;
+        if (MINMAX) {
+            for (int s = 0; s < 5; ++s) {
+                mn[s] -= qi[s]; mx[s] -= qi[s];
+                d.minmax[s * m.C + i] = mn[s];
+                d.minmax[(5 + s) * m.C + i] = mx[s];
+            }
+        }
+    }
+    if (MINMAX) {
+        for (int s = 0; s < 5; ++s) {
+            block_max_nonneg(live ? 0.0 - mn[s] : 0.0, d.red + RED_QMIN + s);
+            block_max_nonneg(live ? mx[s] : 0.0, d.red + RED_QMAX + s);
+        }
+    }
+}
+
+// Limiter::CalcVencatLimiter / CalcBarthLimiter per cell: minimum over the
+// cell's faces (codes/usolver/src/ULimiter.cpp:464-595).
+template <int LIM>
+__global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Consts k) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int T = m.T;
+    double xi = m.xcc[i], yi = m.ycc[i], zi = m.zcc[i];
+    double gx[5], gy[5], gz[5], mn[5], mx[5], lim[5], eps1[5];
+    for (int s = 0; s < 5; ++s) {
+        gx[s] = d.grad[(0 * 6 + s) * T + i];
+        gy[s] = d.grad[(1 * 6 + s) * T + i];
+        gz[s] = d.grad[(2 * 6 + s) * T + i];
+        mn[s] = d.minmax[s * m.C + i];
+        mx[s] = d.minmax[(5 + s) * m.C + i];
+        lim[s] = 1.0;
+        double qmin = -d.red[RED_QMIN + s];
+        double qmax = d.red[RED_QMAX + s];
+        double eps = k.vencat_coef * dmax(qmax - qmin, 1.0);
+        eps1[s] = eps * eps + SMALL;
+    }
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int e = m.c2f[kk];
+        int f = e >= 0 ? e : ~e;
+        double dx = m.xfc[f] - xi, dy = m.yfc[f] - yi, dz = m.zfc[f] - zi;
+        if (LIM == 2) {
+            for (int s = 0; s < 5; ++s) {
+                double dq = gx[s] * dx + gy[s] * dy + gz[s] * dz;
+                lim[s] = dmin(lim[s], vencat_fn(dq, mn[s], mx[s], eps1[s]));
+            }
+        } else {
+            double ds = sqrt(sqr3(dx, dy, dz));
+            double dot = (m.xfn[f] * dx + m.yfn[f] * dy + m.zfn[f] * dz) / (ds + SMALL);
+            if (e < 0) dot = -dot;
+            for (int s = 0; s < 5; ++s) {
+                double dq = gx[s] * dx + gy[s] * dy + gz[s] * dz;
+                lim[s] = dmin(lim[s], barth_fn(dq, mn[s], mx[s], dot));
+            }
+        }
+    }
+    for (int s = 0; s < 5; ++s) d.limiter[s * T + i] = lim[s];
+}
+
+// Per-face inviscid (MUSCL + Roe/SLAU2) and viscous fluxes
+// (UNsInvFlux::CalcInvFlux, codes/uns/src/UNsInvFlux.cpp:141-192;
+//  LimField::CalcFaceValue + NsLimField::BcQlQrFix, codes/usolver/src/ULimiter.cpp:181-256,
+//  codes/uns/src/UNsLimiter.cpp:67-101; UNsVisFlux::CalcVisFlux,
+//  codes/uns/src/UNsVisFlux.cpp:127-440).  LIM: -1 first order, 0 unlimited,
+// 1 limiter array.  Ghost gradients equal their owner's (Grad::CalcGrad tail).
+template <int LIM, bool SLAU2, bool VISC>
+__global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k) {
+    int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    const int T = m.T;
+    int l = m.lc[f], r = m.rc[f];
+    bool bnd = f < m.B;
+    int rg = bnd ? l : r;  // cell holding r's gradient
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], area = m.area[f];
+    double ql[5], qr[5], p1[5], p2[5];
+    for (int e = 0; e < 5; ++e) { ql[e] = d.q[e * T + l]; qr[e] = d.q[e * T + r]; }
+    if (bnd) {
+        for (int e = 0; e < 5; ++e) { double tmp = 0.5 * (ql[e] + qr[e]); p1[e] = tmp; p2[e] = tmp; }
+        if (m.bct[f] == BC_SOLID_SURFACE && k.vismodel != 0) {
+            // bc_q of AdiabaticVisWallBc / IsothermalVisWallBc: owner rho (or wall density), zero velocity, owner p
+            double rw = (k.isowallbc == 0) ? ql[0] : ql[4] / (k.statecoef * k.twall * 1.0);
+            p1[0] = rw; p1[1] = 0.0; p1[2] = 0.0; p1[3] = 0.0; p1[4] = ql[4];
+            for (int e = 0; e < 5; ++e) p2[e] = p1[e];
+        } else if (m.bct[f] == BC_SOLID_SURFACE || m.bct[f] == BC_SYMMETRY) {
+            // bc_q of SymmetryBc is only read on walls; symmetry faces use the average above
+        }
+    } else if (LIM == -1) {
+        for (int e = 0; e < 5; ++e) { p1[e] = ql[e]; p2[e] = qr[e]; }
+    } else {
+        double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
+        double dxl = xfc - m.xcc[l], dyl = yfc - m.ycc[l], dzl = zfc - m.zcc[l];
+        double dxr = xfc - m.xcc[r], dyr = yfc - m.ycc[r], dzr = zfc - m.zcc[r];
+        for (int e = 0; e < 5; ++e) {
+            double gl = d.grad[(0 * 6 + e) * T + l] * dxl + d.grad[(1 * 6 + e) * T + l] * dyl + d.grad[(2 * 6 + e) * T + l] * dzl;
+            double gr = d.grad[(0 * 6 + e) * T + r] * dxr + d.grad[(1 * 6 + e) * T + r] * dyr + d.grad[(2 * 6 + e) * T + r] * dzr;
+            if (LIM == 0) {
+                p1[e] = ql[e] + gl;
+                p2[e] = qr[e] + gr;
+            } else {
+                double phil = d.limiter[e * T + l], phir = d.limiter[e * T + r];
+                p1[e] = ql[e] + dmin(phil, phir) * gl;
+                p2[e] = qr[e] + dmin(phir, phil) * gr;
+            }
+        }
+        if (p1[0] <= 0.0 || p1[4] <= 0.0) for (int e = 0; e < 5; ++e) p1[e] = ql[e];
+        if (p2[0] <= 0.0 || p2[4] <= 0.0) for (int e = 0; e < 5; ++e) p2[e] = qr[e];
+    }
+    double fl[5];
+    if (SLAU2) slau2_flux(p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
+    else roe_flux(k, p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
+    for (int e = 0; e < 5; ++e) d.invflux[e * m.F + f] = area * fl[e];
+
+    if (!VISC) return;
+    VisGeom G;
+    vis_geom(m, f, l, r, k.ivischeme, G);
+    double tl = temperature(ql[0], ql[4], k.statecoef);
+    double tr = temperature(qr[0], qr[4], k.statecoef);
+    double gq[6][3], qfv[6], dqdn[6];
+    for (int s = 1; s < 6; ++s) {
+        if (s == 4) continue;  // pressure does not enter the viscous flux
+        double g1[3] = {d.grad[(0 * 6 + s) * T + l], d.grad[(1 * 6 + s) * T + l], d.grad[(2 * 6 + s) * T + l]};
+        double g2[3] = {d.grad[(0 * 6 + s) * T + rg], d.grad[(1 * 6 + s) * T + rg], d.grad[(2 * 6 + s) * T + rg]};
+        double v1 = s < 5 ? ql[s] : tl, v2 = s < 5 ? qr[s] : tr;
+        vis_grad(k.ivischeme, G, v1, v2, g1, g2, qfv[s], gq[s], dqdn[s]);
+    }
+    double visl = 0.5 * (sutherland(tl, k.ts_suth) + sutherland(tr, k.ts_suth));
+    double vist = 0.5 * (d.vist[l] + d.vist[r]);
+    double vis = visl + vist;
+    double dudx = gq[1][0], dudy = gq[1][1], dudz = gq[1][2];
+    double dvdx = gq[2][0], dvdy = gq[2][1], dvdz = gq[2][2];
+    double dwdx = gq[3][0], dwdy = gq[3][1], dwdz = gq[3][2];
+    double qNormal = 0.0;
+    double kcp = (visl * k.oprl + vist * k.oprt) * k.const_cp;
+    qNormal += xfn * 0.0 + yfn * 0.0 + zfn * 0.0;
+    qNormal += kcp * dqdn[5];
+    double divv2p3 = (2.0 / 3.0) * (dudx + dvdy + dwdz);
+    double txx = vis * (2.0 * dudx - divv2p3);
+    double tyy = vis * (2.0 * dvdy - divv2p3);
+    double tzz = vis * (2.0 * dwdz - divv2p3);
+    double txy = vis * (dudy + dvdx);
+    double txz = vis * (dudz + dwdx);
+    double tyz = vis * (dvdz + dwdy);
+    double fv1 = xfn * txx + yfn * txy + zfn * txz;
+    double fv2 = xfn * txy + yfn * tyy + zfn * tyz;
+    double fv3 = xfn * txz + yfn * tyz + zfn * tzz;
+    double fv4 = qfv[1] * fv1 + qfv[2] * fv2 + qfv[3] * fv3 + qNormal;
+    double coeff = -k.oreynolds * area;
+    d.visflux[0 * m.F + f] = coeff * 0.0;
+    d.visflux[1 * m.F + f] = coeff * fv1;
+    d.visflux[2 * m.F + f] = coeff * fv2;
+    d.visflux[3 * m.F + f] = coeff * fv3;
+    d.visflux[4 * m.F + f] = coeff * fv4;
+}
+
+// Residual gather: res = -sum(inviscid) then -sum(viscous), each over the
+// cell's faces in ascending id, sign by side (AddF2CField called twice).
+template <bool VISC>
+__global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int T = m.T, F = m.F;
+    double r[5] = {0, 0, 0, 0, 0};
+    int b = m.c2fOff[i], n = m.c2fOff[i + 1];
+    for (int kk = b; kk < n; ++kk) {
+        int e = m.c2f[kk];
+        int f = e >= 0 ? e : ~e;
+        if (e >= 0) for (int s = 0; s < 5; ++s) r[s] -= d.invflux[s * F + f];
+        else for (int s = 0; s < 5; ++s) r[s] += d.invflux[s * F + f];
+    }
+    if (VISC) {
+        for (int kk = b; kk < n; ++kk) {
+            int e = m.c2f[kk];
+            int f = e >= 0 ? e : ~e;
+            if (e >= 0) for (int s = 0; s < 5; ++s) r[s] -= d.visflux[s * F + f];
+            else for (int s = 0; s < 5; ++s) r[s] += d.visflux[s * F + f];
+        }
+    }
+    for (int s = 0; s < 5; ++s) d.res[s * T + i] = r[s];
+}
+
+template <int LIM, bool SLAU2>
+static void launch_face(ofx_ctx* c, bool visc) {
+    int nb = nblk(c->m.F);
+    if (visc) k_ns_face<LIM, SLAU2, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+    else k_ns_face<LIM, SLAU2, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+}
+
+int ns_residual(ofx_ctx* c) {
+    const Consts& k = c->k;
+    const int C = c->m.C;
+    bool visc = k.vismodel != 0;
+    bool lim = k.ilim == 1 || k.ilim == 2;
+    if (lim) {
+        k_fill<<<1, 64, 0, c->stream>>>(c->d.red + RED_QMIN, 10, 0.0);
+        k_ns_grad<true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
+    } else {
+        k_ns_grad<false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
+    }
+    OFX_HIP(hipGetLastError());
+    if (k.ilim == 2) k_ns_limiter<2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
+    if (k.ilim == 1) k_ns_limiter<1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
+    OFX_HIP(hipGetLastError());
+    OFX_HIP(hipEventRecord(c->ev[0], c->stream));
+    bool slau2 = k.ischeme == 12;
+    if (k.ilim == -1) { if (slau2) launch_face<-1, true>(c, visc); else launch_face<-1, false>(c, visc); }
+    else if (k.ilim == 0) { if (slau2) launch_face<0, true>(c, visc); else launch_face<0, false>(c, visc); }
+    else { if (slau2) launch_face<1, true>(c, visc); else launch_face<1, false>(c, visc); }
+    OFX_HIP(hipGetLastError());
+    OFX_HIP(hipEventRecord(c->ev[1], c->stream));
+    if (visc) k_ns_gather<true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+    else k_ns_gather<false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// Ghost-cell gradients / limiter for field download (Grad::CalcGrad tail,
+// codes/usolver/src/UGrad.cpp:142-148)
+__global__ void k_ghost_grad(DevMesh m, DevFields d) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.B) return;
+    int l = m.lc[f], r = m.C + f;
+    for (int a = 0; a < 18; ++a) d.grad[a * m.T + r] = d.grad[a * m.T + l];
+    for (int a = 0; a < 3; ++a) d.tgrad[a * m.T + r] = d.tgrad[a * m.T + l];
+}
+
+int fill_ghost_grad(ofx_ctx* c) {
+    if (c->m.B) k_ghost_grad<<<nblk(c->m.B), TB, 0, c->stream>>>(c->m, c->d);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// ============================================================ LU-SGS
+
+// UNsLusgs::Solve for face f seen from cell i: GCom::SetGeometry orientation,
+// PrepareData/PrepareDataFacePrim, NsLusgs::GetStandardFluxIncrement(-1) with
+// CalcFaceEigenValue and CalcDH, then CalcViscousTerm and AddFluxIncrement
+// (codes/uns/src/UNsLusgs.cpp:189-335, codes/ns/src/NsLusgs.cpp:121-228,340-368).
+__device__ __forceinline__ void ns_lusgs_face(const DevMesh& m, const DevFields& d, const Consts& k, int i, int f,
+                                              double* rhs0) {
+    const int T = m.T;
+    int l = m.lc[f], r = m.rc[f];
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], vfn = 0.0, farea = m.area[f];
+    if (r == i) { int t = l; l = r; r = t; xfn = -xfn; yfn = -yfn; zfn = -zfn; vfn = -vfn; }
+    double pj[5], dqj[5];
+    for (int e = 0; e < 5; ++e) { pj[e] = d.q[e * T + r]; dqj[e] = d.dq[e * T + r]; }
+    double rl = d.q[l], ul = d.q[T + l], vl = d.q[2 * T + l], wl = d.q[3 * T + l], pl = d.q[4 * T + l];
+    double rr = pj[0], ur = pj[1], vr = pj[2], wr = pj[3], pr = pj[4];
+    double gl = k.gama, gr = k.gama;
+    double hl = gl / (gl - 1.0) * pl / rl + 0.5 * sqr3(ul, vl, wl);
+    double hr = gr / (gr - 1.0) * pr / rr + 0.5 * sqr3(ur, vr, wr);
+    double tmp0 = sqrt(rr / rl);
+    double tmp1 = 1.0 / (1.0 + tmp0);
+    double rm = sqrt(rr * rl);
+    double um = (ul + ur * tmp0) * tmp1;
+    double vm = (vl + vr * tmp0) * tmp1;
+    double wm = (wl + wr * tmp0) * tmp1;
+    double hm = (hl + hr * tmp0) * tmp1;
+    double pm = rm * (hm - 0.5 * sqr3(um, vm, wm)) * (gl - 1.0) / gl;
+    double gama = k.gama;
+    double c2f = dabs(gama * pm / rm);
+    double cmf = sqrt(c2f);
+    double vnf = xfn * um + yfn * vm + zfn * wm;
+    double lmdF = dabs(vnf - vfn) + cmf;
+    double c2 = dabs(gama * pj[4] / pj[0]);
+    double cm = sqrt(c2);
+    double vn_fluid = xfn * pj[1] + yfn * pj[2] + zfn * pj[3];
+    double vn_rel = vn_fluid - vfn;
+    double lmd1 = 0.5 * (vn_rel + -1.0 * lmdF);
+    double lmd2 = 0.5 * ((vn_rel + cm) + -1.0 * lmdF);
+    double lmd3 = 0.5 * ((vn_rel - cm) + -1.0 * lmdF);
+    double x1 = (lmd1 + lmd1 - lmd2 - lmd3) / (c2 + c2);
+    double x2 = (lmd2 - lmd3) / (cm + cm);
+    double dc = vn_fluid * dqj[0] - xfn * dqj[1] - yfn * dqj[2] - zfn * dqj[3];
+    double c2dc = c2 * dc;
+    double v2 = sqr3(pj[1], pj[2], pj[3]);
+    double ae = gama - 1.0;
+    double af = 0.5 * ae * v2;
+    double dh = -ae * (pj[1] * dqj[1] + pj[2] * dqj[2] + pj[3] * dqj[3] - dqj[4]);
+    double enth = (gama / (gama - 1.0)) * (pj[4] / pj[0]);
+    dh += af * dqj[0];
+    double hmj = enth + 0.5 * v2;
+    double term1 = dh * x1 + dc * x2;
+    double term2 = c2dc * x1 + dh * x2;
+    double dfj[5];
+    dfj[0] = lmd1 * dqj[0] - term1;
+    dfj[1] = lmd1 * dqj[1] - pj[1] * term1 + xfn * term2;
+    dfj[2] = lmd1 * dqj[2] - pj[2] * term1 + yfn * term2;
+    dfj[3] = lmd1 * dqj[3] - pj[3] * term1 + zfn * term2;
+    dfj[4] = lmd1 * dqj[4] - hmj * term1 + vn_fluid * term2;
+    for (int e = 0; e < 5; ++e) dfj[e] *= farea;
+    if (k.vismodel != 0) {
+        double visl = d.visl[r], vist = d.vist[r];
+        double visrad;
+        if (k.visSRModel == 1) {
+            double density = 0.5 * (rl + rr);
+            double c1 = 4.0 / 3.0 * (visl + vist);
+            double cc2 = k.gama * (visl * k.oprl + vist * k.oprt);
+            double c3 = 2.0 * dmax(c1, cc2) / (k.reynolds * density);
+            visrad = (farea * farea) * c3 / m.vol[r];
+        } else {
+            double dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) + zfn * (m.zcc[r] - m.zcc[l]));
+            double viscosity = visl + vist;
+            double density = 0.5 * (rl + rr);
+            double c1 = 2.0 * viscosity / (density * dist * k.reynolds + SMALL);
+            visrad = 0.5 * c1 * farea;
+        }
+        for (int e = 0; e < 5; ++e) rhs0[e] -= visrad * dqj[e];
+    }
+    for (int e = 0; e < 5; ++e) rhs0[e] += dfj[e];
+}
+
+// One cell of the lower (UPPER=false) or upper sweep.  Boundary faces are
+// skipped in both sweeps: their ghost increment is zero, so the reference's
+// upper-sweep contribution through them is exactly +0.
+template <bool UPPER>
+__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k, int i) {
+    const int T = m.T;
+    double rhs0[5] = {0, 0, 0, 0, 0};
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int e = m.c2f[kk];
+        int f = e >= 0 ? e : ~e;
+        if (f < m.B) continue;
+        int j = e >= 0 ? m.rc[f] : m.lc[f];
+        if (UPPER ? (j < i) : (j > i)) continue;
+        ns_lusgs_face(m, d, k, i, f, rhs0);
+    }
+    double rad = d.impsr[i];
+    if (!UPPER) {
+        for (int e = 0; e < 5; ++e) { double t = d.res[e * T + i] - rhs0[e]; t /= rad; d.dq[e * T + i] = t; }
+    } else {
+        for (int e = 0; e < 5; ++e) { double t = -rhs0[e]; t /= rad; d.dq[e * T + i] += t; }
+    }
+}
+
+// Whole sweep in one workgroup: levels in order, a workgroup barrier between
+// levels (every cell of a level depends only on earlier levels).
+template <bool UPPER>
+__global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k) {
+    const int* off = UPPER ? m.upOff : m.loOff;
+    const int* ord = UPPER ? m.upOrder : m.loOrder;
+    int nl = UPPER ? m.nLevUp : m.nLevLo;
+    for (int L = 0; L < nl; ++L) {
+        for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
+            ns_lusgs_cell<UPPER>(m, d, k, ord[idx]);
+        __syncthreads();
+    }
+}
+
+// One level per launch (wide levels on large meshes).
+template <bool UPPER>
+__global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e) {
+    int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= e) return;
+    ns_lusgs_cell<UPPER>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx]);
+}
+
+constexpr int WIDE_LEVEL = 8192;
+
+static int max_width(const std::vector<int>& off) {
+    int w = 0;
+    for (size_t l = 0; l + 1 < off.size(); ++l) w = std::max(w, off[l + 1] - off[l]);
+    return w;
+}
+
+int ns_lusgs(ofx_ctx* c) {
+    const DevMesh& m = c->m;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool up = pass == 1;
+        const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
+        if (max_width(off) <= WIDE_LEVEL) {
+            if (up) k_ns_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d, c->k);
+            else k_ns_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d, c->k);
+        } else {
+            for (size_t L = 0; L + 1 < off.size(); ++L) {
+                int b = off[L], e = off[L + 1];
+                if (up) k_ns_sweep_level<true><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e);
+                else k_ns_sweep_level<false><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e);
+            }
+        }
+        OFX_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// ============================================================ update
+
+// UNsUpdate::UpdateFlowField + NsUpdate::CalcFlowField / WeekSolutionFix /
+// SolutionFix (codes/uns/src/UNsUpdate.cpp:46-146, codes/ns/src/NsUpdate.cpp:40-67,192-217).
+__device__ __forceinline__ bool ns_update_cell(const DevFields& d, const Consts& k, int T, int i, double* prim) {
+    double prim0[5], q0[5], q[5], dq[5];
+    for (int e = 0; e < 5; ++e) { prim0[e] = d.q[e * T + i]; dq[e] = d.dq[e * T + i]; }
+    double gama = k.gama;
+    prim_to_q(prim0, gama, q0);
+    for (int e = 0; e < 5; ++e) q[e] = q0[e] + dq[e];
+    double density = dabs(q[0]);
+    double rd = 1.0 / density;
+    double um = q[1] * rd, vm = q[2] * rd, wm = q[3] * rd, rem = q[4];
+    double v2 = sqr3(um, vm, wm);
+    double reint = rem - 0.5 * density * v2;
+    prim[0] = density; prim[1] = um; prim[2] = vm; prim[3] = wm; prim[4] = (gama - 1.0) * reint;
+    if (prim[0] <= 0.0 || prim[4] <= 0.0) {
+        const double coefs[3] = {0.5, 0.1, 0.01};
+        for (int kk = 0; kk < 3; ++kk) {
+            for (int e = 0; e < 5; ++e) prim[e] += coefs[kk] * dq[e];
+            if (!(prim[0] < 0.0 || prim[4] < 0.0)) return true;
+        }
+        return false;  // needs SolutionFix
+    }
+    return true;
+}
+
+// Pass 1: count cells whose weak fix fails (the neighbour average of
+// SolutionFix reads neighbours in the serial loop's half-updated state).
+__global__ void __launch_bounds__(TB) k_ns_update_check(DevMesh m, DevFields d, Consts k, int* nfix) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double prim[5];
+    if (!ns_update_cell(d, k, m.T, i, prim)) atomicAdd(nfix, 1);
+}
+
+// Pass 2: the common case updates every cell in parallel; if any cell needs
+// SolutionFix the whole update runs in the reference's serial cell order.
+__global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, Consts k, const int* nfix) {
+    const int T = m.T;
+    if (*nfix == 0) {
+        int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+        if (i >= m.C) return;
+        double prim[5];
+        ns_update_cell(d, k, T, i, prim);
+        for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
+        return;
+    }
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int i = 0; i < m.C; ++i) {
+        double prim[5];
+        if (!ns_update_cell(d, k, T, i, prim)) {
+            for (int e = 0; e < 5; ++e) prim[e] = 0.0;
+            double sumV = 0.0;
+            for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+                int e = m.c2f[kk];
+                int f = e >= 0 ? e : ~e;
+                int nei = e >= 0 ? m.rc[f] : m.lc[f];
+                sumV += 1.0;
+                for (int s = 0; s < 5; ++s) {
+                    double v = d.q[s * T + nei];
+                    if (s == 0 || s == 4) v = dabs(v);
+                    prim[s] += v * 1.0;
+                }
+            }
+            double rVol = 1.0 / sumV;
+            for (int s = 0; s < 5; ++s) prim[s] *= rVol;
+        }
+        for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
+    }
+}
+
+int ns_update(ofx_ctx* c) {
+    int* nfix = reinterpret_cast<int*>(c->d.red + RED_NFIX);
+    k_fill<<<1, 64, 0, c->stream>>>(c->d.red + RED_NFIX, 1, 0.0);
+    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, nfix);
+    k_ns_update_apply<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, nfix);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// ============================================================ norms / forces
+
+// DUMP_RESIDUAL norm sqrt(sum res^2 / nCells) (codes/residual/src/Residual.cpp:52-72):
+// fixed-shape two-level sum (block partials, then one block in order), so the
+// result is identical run to run.
+constexpr int NORM_BLOCKS = 512;
+__global__ void __launch_bounds__(TB) k_norm_partial(DevMesh m, DevFields d, double* part) {
+    __shared__ double sm[6][TB];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.C; i += gridDim.x * blockDim.x) {
+        for (int e = 0; e < 5; ++e) { double v = d.res[e * m.T + i]; acc[e] += v * v; }
+        double v = d.turbres[i];
+        acc[5] += v * v;
+    }
+    for (int e = 0; e < 6; ++e) sm[e][threadIdx.x] = acc[e];
+    __syncthreads();
+    for (int s = TB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int e = 0; e < 6; ++e) sm[e][threadIdx.x] += sm[e][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[threadIdx.x * NORM_BLOCKS + blockIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void k_norm_final(const double* part, double* out, int nCells) {
+    int e = threadIdx.x;
+    if (e >= 6) return;
+    double s = 0.0;
+    for (int b = 0; b < NORM_BLOCKS; ++b) s += part[e * NORM_BLOCKS + b];
+    out[e] = sqrt(s / nCells);
+}
+
+int norms(ofx_ctx* c, ofx_norms* out) {
+    double* part = c->d.norms + 8;
+    k_norm_partial<<<NORM_BLOCKS, TB, 0, c->stream>>>(c->m, c->d, part);
+    k_norm_final<<<1, 64, 0, c->stream>>>(part, c->d.norms, c->m.C);
+    OFX_HIP(hipGetLastError());
+    double h[6];
+    OFX_HIP(hipMemcpyAsync(h, c->d.norms, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    for (int e = 0; e < 5; ++e) out->ns[e] = h[e];
+    out->sa = (c->k.vismodel == 3) ? h[5] : 0.0;
+    return 0;
+}
+
+// CalcAeroForce per wall face (codes/force/src/AeroForceTask.cpp:186-440), the
+// sum in face order on one thread as the reference loops.
+__global__ void k_aero(DevMesh m, DevFields d, Consts k, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int T = m.T;
+    double tot[3] = {0, 0, 0}, pres[3] = {0, 0, 0};
+    for (int f = 0; f < m.B; ++f) {
+        if (m.bct[f] != BC_SOLID_SURFACE) continue;
+        int l = m.lc[f], r = m.C + f;
+        double area = m.area[f], fnx = m.xfn[f], fny = m.yfn[f], fnz = m.zfn[f];
+        double fanx = fnx * area, fany = fny * area, fanz = fnz * area;
+        double dx = m.xfc[f] - m.xcc[l], dy = m.yfc[f] - m.ycc[l], dz = m.zfc[f] - m.zcc[l];
+        double gp[3] = {d.grad[(0 * 6 + 4) * T + l], d.grad[(1 * 6 + 4) * T + l], d.grad[(2 * 6 + 4) * T + l]};
+        double wp = d.q[4 * T + l] + gp[0] * dx + gp[1] * dy + gp[2] * dz;
+        if (wp < 0.0) wp = d.q[4 * T + l];
+        double cp = 2.0 * (wp - k.inflow[4]);
+        double pf[3] = {fanx * cp, fany * cp, fanz * cp};
+        double vf[3] = {0, 0, 0};
+        if (k.vismodel > 0) {
+            double g[3][3];
+            for (int s = 0; s < 3; ++s)
+                for (int a = 0; a < 3; ++a) g[s][a] = d.grad[(a * 6 + 1 + s) * T + l];
+            double ex = m.xcc[r] - m.xcc[l], ey = m.ycc[r] - m.ycc[l], ez = m.zcc[r] - m.zcc[l];
+            double ods = 1.0 / sqrt(sqr3(ex, ey, ez));
+            ex *= ods; ey *= ods; ez *= ods;
+            for (int s = 0; s < 3; ++s) {
+                double fOld = g[s][0] * ex + g[s][1] * ey + g[s][2] * ez;
+                double fnew = (d.q[(1 + s) * T + r] - d.q[(1 + s) * T + l]) * ods;
+                double df = fnew - fOld;
+                g[s][0] += df * ex; g[s][1] += df * ey; g[s][2] += df * ez;
+            }
+            double vis = d.visl[l];
+            double dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
+            double dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
+            double dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
+            double divv2p3 = (2.0 / 3.0) * (dudx + dvdy + dwdz);
+            double txx = vis * (2.0 * dudx - divv2p3), tyy = vis * (2.0 * dvdy - divv2p3), tzz = vis * (2.0 * dwdz - divv2p3);
+            double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
+            double coef = -2.0 * area * k.oreynolds;
+            vf[0] = coef * (fnx * txx + fny * txy + fnz * txz);
+            vf[1] = coef * (fnx * txy + fny * tyy + fnz * tyz);
+            vf[2] = coef * (fnx * txz + fny * tyz + fnz * tzz);
+        }
+        for (int a = 0; a < 3; ++a) { tot[a] += pf[a] + vf[a]; pres[a] += pf[a]; }
+    }
+    for (int a = 0; a < 3; ++a) { out[a] = tot[a]; out[3 + a] = pres[a]; }
+}
+
+int aero(ofx_ctx* c, double* out4) {
+    double* buf = c->d.norms + 8;
+    k_aero<<<1, 64, 0, c->stream>>>(c->m, c->d, c->k, buf);
+    OFX_HIP(hipGetLastError());
+    double h[6];
+    OFX_HIP(hipMemcpyAsync(h, buf, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    double aoa = c->cfg.aoa_degree * PI / 180.0, aos = c->cfg.sideslip_degree * PI / 180.0;
+    double sina = std::sin(aoa), cosa = std::cos(aoa), sinb = std::sin(aos), cosb = std::cos(aos);
+    double cl = -sina * h[0] + cosa * h[1];
+    double cd = cosa * cosb * h[0] + sina * cosb * h[1] + sinb * h[2];
+    double cdp = cosa * cosb * h[3] + sina * cosb * h[4] + sinb * h[5];
+    out4[0] = cl; out4[1] = cd; out4[2] = cdp; out4[3] = cd - cdp;
+    return 0;
+}
+
+}  // namespace ofx

@@ -1,0 +1,256 @@
+// ofx_physics.h -- per-face / per-cell physics of the NS + SA path as device
+// functions.  Each keeps the operation order of the reference function it is
+// named after, so that with -ffp-contract=off the FP64 results are the
+// reference's bit for bit wherever only + - * / sqrt are involved.
+#pragma once
+
+#include "ofx_common.h"
+
+namespace ofx {
+
+// Sutherland::CalcViscosity (codes/model/src/Sutherland.cpp), CalcLaminarViscosity clamp
+__device__ __forceinline__ double sutherland(double t, double ts) {
+    double t3 = t * t * t;
+    double v = sqrt(t3) * (1 + ts) / (t + ts);
+    return dmax(0.0, v);
+}
+
+// NsCalcGamaT: T = p / (statecoef * rho * oamw), oamw = 1 (codes/ns/src/NsRhs.cpp:95-102)
+__device__ __forceinline__ double temperature(double rho, double p, double statecoef) {
+    return p / (statecoef * rho * 1.0);
+}
+
+// PrimToQ + CalcInternalEnergy (codes/ns/src/NsInvFlux.cpp:1349-1370,1392-1406)
+__device__ __forceinline__ void prim_to_q(const double* p, double gama, double* q) {
+    double v2 = sqr3(p[1], p[2], p[3]);
+    double em = (p[4] / p[0]) / (gama - 1.0) + 0.5 * v2;
+    q[0] = p[0]; q[1] = p[0] * p[1]; q[2] = p[0] * p[2]; q[3] = p[0] * p[3]; q[4] = p[0] * em;
+}
+
+// NsInvFlux::Roe with ModifyAbsoluteEigenvalue and CalcTotalEnthalpyChange
+// (codes/ns/src/NsInvFlux.cpp:167-308,1329-1347); static faces (vfn = 0);
+// returns the flux per unit area.
+__device__ __forceinline__ void roe_flux(const Consts& k, const double* p1, const double* p2, double gama1,
+                                         double gama2, double xfn, double yfn, double zfn, double* flux) {
+    const double vfn = 0.0;
+    double rl = p1[0], ul = p1[1], vl = p1[2], wl = p1[3], pl = p1[4];
+    double rr = p2[0], ur = p2[1], vr = p2[2], wr = p2[3], pr = p2[4];
+    double v2l = sqr3(ul, vl, wl), v2r = sqr3(ur, vr, wr);
+    double hint_l = (gama1 / (gama1 - 1.0)) * (pl / rl);
+    double hint_r = (gama2 / (gama2 - 1.0)) * (pr / rr);
+    double hl = hint_l + 0.5 * v2l, hr = hint_r + 0.5 * v2r;
+    double vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
+    double vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
+    double rvnl = rl * vnl, rvnr = rr * vnr;
+    double ratio = sqrt(rr / rl);
+    double coef = 1.0 / (1.0 + ratio);
+    double um = (ul + ur * ratio) * coef;
+    double vm = (vl + vr * ratio) * coef;
+    double wm = (wl + wr * ratio) * coef;
+    double hm = (hl + hr * ratio) * coef;
+    double gama = (gama1 + gama2 * ratio) * coef;
+    double v2 = sqr3(um, vm, wm);
+    double vnflow = xfn * um + yfn * vm + zfn * wm;
+    double vnrel = vnflow - vfn;
+    double gamm1 = gama - 1.0;
+    double c2 = gamm1 * (hm - 0.5 * v2);
+    double cm = sqrt(dabs(c2));
+    double aeig1 = dabs(vnrel), aeig2 = dabs(vnrel + cm), aeig3 = dabs(vnrel - cm);
+    flux[0] = rvnl + rvnr;
+    flux[1] = (rvnl * ul + xfn * pl) + (rvnr * ur + xfn * pr);
+    flux[2] = (rvnl * vl + yfn * pl) + (rvnr * vr + yfn * pr);
+    flux[3] = (rvnl * wl + zfn * pl) + (rvnr * wr + zfn * pr);
+    flux[4] = (rvnl * hl + vfn * pl) + (rvnr * hr + vfn * pr);
+    double q1[5], q2[5], dq[5];
+    prim_to_q(p1, gama1, q1);
+    prim_to_q(p2, gama2, q2);
+    for (int e = 0; e < 5; ++e) dq[e] = q2[e] - q1[e];
+    double meig1 = aeig1, meig2 = aeig2, meig3 = aeig3;
+    if (k.ieigenfix == 1) {
+        double maxE = dmax(aeig2, aeig3);
+        double m1 = maxE * k.centropy1, m2 = maxE * k.centropy2;
+        meig1 = dmax(m1, aeig1); meig2 = dmax(m2, aeig2); meig3 = dmax(m2, aeig3);
+    }
+    double xi1 = (2.0 * meig1 - meig2 - meig3) / (2.0 * c2);
+    double xi2 = (meig2 - meig3) / (2.0 * cm);
+    double dc = vnflow * dq[0] - xfn * dq[1] - yfn * dq[2] - zfn * dq[3];
+    double c2dc = c2 * dc;
+    double ae = gama - 1.0;
+    double af = 0.5 * ae * v2;
+    double dh = -ae * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
+    dh += af * dq[0];
+    double term1 = dh * xi1 + dc * xi2;
+    double term2 = c2dc * xi1 + dh * xi2;
+    flux[0] -= (meig1 * dq[0] - term1);
+    flux[1] -= (meig1 * dq[1] - um * term1 + xfn * term2);
+    flux[2] -= (meig1 * dq[2] - vm * term1 + yfn * term2);
+    flux[3] -= (meig1 * dq[3] - wm * term1 + zfn * term2);
+    flux[4] -= (meig1 * dq[4] - hm * term1 + vnflow * term2);
+    for (int e = 0; e < 5; ++e) flux[e] *= 0.5;
+}
+
+// FMSplit2 / FPSplit5 (codes/ns/src/NsInvFlux.cpp:42-65)
+__device__ __forceinline__ double fm_split2(double mach, double ipn) { return ipn * 0.25 * ((mach + ipn) * (mach + ipn)); }
+__device__ __forceinline__ double fp_split5(double mach, double alpha, double ipn) {
+    double macha = dabs(mach);
+    if (macha >= 1.0) return 0.5 * (1.0 + ipn * mach / (macha + SMALL));
+    return fm_split2(mach, ipn) * ((2.0 * ipn - mach) - ipn * 16.0 * alpha * mach * fm_split2(mach, -ipn));
+}
+
+// NsInvFlux::Slau2 (codes/ns/src/NsInvFlux.cpp:1246-1314), static faces
+__device__ __forceinline__ void slau2_flux(const double* p1, const double* p2, double gama1, double gama2,
+                                           double xfn, double yfn, double zfn, double* flux) {
+    const double vfn = 0.0;
+    double rl = p1[0], ul = p1[1], vl = p1[2], wl = p1[3], pl = p1[4];
+    double rr = p2[0], ur = p2[1], vr = p2[2], wr = p2[3], pr = p2[4];
+    double v2l = sqr3(ul, vl, wl), v2r = sqr3(ur, vr, wr);
+    double hint_l = (gama1 / (gama1 - 1.0)) * (pl / rl);
+    double hint_r = (gama2 / (gama2 - 1.0)) * (pr / rr);
+    double hl = hint_l + 0.5 * v2l, hr = hint_r + 0.5 * v2r;
+    double vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
+    double vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
+    double orl = 1.0 / (rl + SMALL), orr = 1.0 / (rr + SMALL);
+    double c2l = gama1 * pl * orl, c2r = gama2 * pr * orr;
+    double cl = sqrt(c2l), cr = sqrt(c2r);
+    double cm = 0.5 * (cl + cr);
+    double t = rr / rl;
+    double vna = (dabs(vnl) + t * dabs(vnr)) / (1 + t);
+    double ml = vnl / cm, mr = vnr / cm;
+    double g = -dmax(dmin(ml, 0.0), -1.0) * dmin(dmax(mr, 0.0), 1.0);
+    double vnp = (1 - g) * vna + g * dabs(vnl);
+    double vnn = (1 - g) * vna + g * dabs(vnr);
+    double va = sqrt(0.5 * (v2l + v2r));
+    double m12 = dmin(1.0, va / cm);
+    double ka = (1 - m12) * (1 - m12);
+    double ms = 0.5 * (rl * (vnl + vnp) + rr * (vnr - vnn) - ka * (pr - pl) / cm);
+    double fp5ml = fp_split5(ml, 0.0, 1.0);
+    double fp5mr = fp_split5(mr, 0.0, -1.0);
+    double ps = 0;
+    ps += 0.5 * (pl + pr);
+    ps += 0.5 * (fp5ml - fp5mr) * (pl - pr);
+    ps += va * (fp5ml + fp5mr - 1) * sqrt(rl * rr) * cm;
+    double mp = 0.5 * (ms + dabs(ms)), mn = 0.5 * (ms - dabs(ms));
+    flux[0] = (mp + mn);
+    flux[1] = (mp * ul + mn * ur) + xfn * ps;
+    flux[2] = (mp * vl + mn * vr) + yfn * ps;
+    flux[3] = (mp * wl + mn * wr) + zfn * ps;
+    flux[4] = (mp * hl + mn * hr) + vfn * ps;
+}
+
+// GCom::CalcTangent (codes/global/src/Com.cpp:76-121)
+__device__ __forceinline__ void tangents(double xfn, double yfn, double zfn, double* t1, double* t2) {
+    if (xfn != 0.0) { t1[0] = yfn; t1[1] = -xfn; t1[2] = 0.0; }
+    else if (yfn != 0.0) { t1[0] = -yfn; t1[1] = xfn; t1[2] = 0.0; }
+    else if (zfn != 0.0) { t1[0] = 0.0; t1[1] = -zfn; t1[2] = yfn; }
+    else { t1[0] = t1[1] = t1[2] = 0; t2[0] = t2[1] = t2[2] = 0; return; }
+    double dtmp = 1.0 / sqrt(sqr3(t1[0], t1[1], t1[2]));
+    t1[0] *= dtmp; t1[1] *= dtmp; t1[2] *= dtmp;
+    t2[0] = yfn * t1[2] - zfn * t1[1];
+    t2[1] = zfn * t1[0] - xfn * t1[2];
+    t2[2] = xfn * t1[1] - yfn * t1[0];
+}
+
+// Geometry of one face for the VisGrad face-gradient methods
+// (VisGradGeom::CalcFaceWeight / CalcGradCoef / PrepareCellGeom,
+//  codes/usolver/src/VisGrad.cpp:44-103)
+struct VisGeom {
+    double xfn, yfn, zfn, t1[3], t2[3];
+    double fw1, fw2;
+    double dx, dy, dz, ods;
+    double d1, d2, dxnl, dynl, dznl, dxnr, dynr, dznr, angle1, angle2;
+    bool bc_nonint;  // boundary face that is not an interface (VisGrad::CalcC1C2)
+};
+
+__device__ __forceinline__ double vis_angle(double dx, double dy, double dz, double dist) {
+    double res = dist / (sqrt(sqr3(dx, dy, dz)) + SMALL);
+    if (res > 1.0) res = 1.0;
+    if (res < -1.0) res = -1.0;
+    return asin(res) * 180.0 / PI;
+}
+
+__device__ __forceinline__ void vis_geom(const DevMesh& m, int f, int l, int r, int method, VisGeom& G) {
+    G.xfn = m.xfn[f]; G.yfn = m.yfn[f]; G.zfn = m.zfn[f];
+    G.bc_nonint = f < m.B;
+    tangents(G.xfn, G.yfn, G.zfn, G.t1, G.t2);
+    double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
+    double xl = m.xcc[l], yl = m.ycc[l], zl = m.zcc[l];
+    double xr = m.xcc[r], yr = m.ycc[r], zr = m.zcc[r];
+    double delt1 = sqrt(sqr3(xfc - xl, yfc - yl, zfc - zl));
+    double delt2 = sqrt(sqr3(xfc - xr, yfc - yr, zfc - zr));
+    double delta = 1.0 / (delt1 + delt2 + SMALL);
+    G.fw1 = delt2 * delta; G.fw2 = delt1 * delta;
+    if (method == 0) {
+        G.dx = xr - xl; G.dy = yr - yl; G.dz = zr - zl;
+        G.ods = 1.0 / sqrt(sqr3(G.dx, G.dy, G.dz));
+        G.dx *= G.ods; G.dy *= G.ods; G.dz *= G.ods;
+    } else if (method == 2) {
+        double ax = xl - xfc, ay = yl - yfc, az = zl - zfc;
+        double bx = xr - xfc, by = yr - yfc, bz = zr - zfc;
+        G.d1 = G.xfn * ax + G.yfn * ay + G.zfn * az;
+        G.d2 = G.xfn * bx + G.yfn * by + G.zfn * bz;
+        G.dxnl = G.xfn * G.d1 - ax; G.dynl = G.yfn * G.d1 - ay; G.dznl = G.zfn * G.d1 - az;
+        G.dxnr = G.xfn * G.d2 - bx; G.dynr = G.yfn * G.d2 - by; G.dznr = G.zfn * G.d2 - bz;
+        G.angle1 = vis_angle(ax, ay, az, -G.d1);
+        G.angle2 = vis_angle(bx, by, bz, G.d2);
+    }
+}
+
+// One scalar through VisGrad (AverGrad, then STD / AVER / TEST method;
+// codes/usolver/src/VisGrad.cpp:130-365).  gf = face gradient, dqdn = normal derivative.
+__device__ __forceinline__ void vis_grad(int method, const VisGeom& G, double q1, double q2, const double* g1,
+                                         const double* g2, double& qf, double* gf, double& dqdn) {
+    gf[0] = 0.5 * (g1[0] + g2[0]); gf[1] = 0.5 * (g1[1] + g2[1]); gf[2] = 0.5 * (g1[2] + g2[2]);
+    dqdn = 0.0;
+    qf = 0.5 * (q1 + q2);
+    if (method == 0) {
+        double fOld = gf[0] * G.dx + gf[1] * G.dy + gf[2] * G.dz;
+        double fnew = (q2 - q1) * G.ods;
+        double df = fnew - fOld;
+        gf[0] += df * G.dx; gf[1] += df * G.dy; gf[2] += df * G.dz;
+        dqdn = G.xfn * gf[0] + G.yfn * gf[1] + G.zfn * gf[2];
+    } else if (method == 1) {
+        dqdn = G.xfn * gf[0] + G.yfn * gf[1] + G.zfn * gf[2];
+    } else {
+        double q11 = q1, q22 = q2;
+        if (G.angle1 > 20.0 && G.angle2 > 20.0) {
+            q11 += g1[0] * G.dxnl + g1[1] * G.dynl + g1[2] * G.dznl;
+            q22 += g2[0] * G.dxnr + g2[1] * G.dynr + g2[2] * G.dznr;
+        }
+        bool ok = G.angle1 > 0.0 && G.angle2 > 0.0 && dabs(G.d1) > SMALL && dabs(G.d2) > SMALL;
+        if (ok) {
+            double dtmp = G.d1 * G.d1 + G.d2 * G.d2;
+            double c1 = (G.d1 * G.d1) / dtmp, c2 = 1.0 - c1;
+            if (G.bc_nonint) { c1 = 1.0; c2 = 0.0; }
+            double dqdn1 = (q11 - qf) / G.d1;
+            double dqdn2 = (q22 - qf) / G.d2;
+            dqdn = c1 * dqdn1 + c2 * dqdn2;
+        }
+        gf[0] = G.fw1 * g1[0] + G.fw2 * g2[0];
+        gf[1] = G.fw1 * g1[1] + G.fw2 * g2[1];
+        gf[2] = G.fw1 * g1[2] + G.fw2 * g2[2];
+        double dt1 = G.t1[0] * gf[0] + G.t1[1] * gf[1] + G.t1[2] * gf[2];
+        double dt2 = G.t2[0] * gf[0] + G.t2[1] * gf[1] + G.t2[2] * gf[2];
+        gf[0] = G.xfn * dqdn + G.t1[0] * dt1 + G.t2[0] * dt2;
+        gf[1] = G.yfn * dqdn + G.t1[1] * dt1 + G.t2[1] * dt2;
+        gf[2] = G.zfn * dqdn + G.t1[2] * dt1 + G.t2[2] * dt2;
+    }
+}
+
+// Venkatakrishnan / Barth limiter functions (codes/usolver/src/ULimiter.cpp:35-97)
+__device__ __forceinline__ double vencat(double x, double y, double eps) {
+    double x2 = x * x, xy = x * y, y2 = y * y, cc = 2.0;
+    return ((x2 + eps + cc * xy) / (x2 + cc * y2 + xy + eps));
+}
+__device__ __forceinline__ double vencat_fn(double dq, double minv, double maxv, double eps) {
+    if (dq > SMALL) return vencat(maxv, dq, eps);
+    else if (dq < -SMALL) return vencat(minv, dq, eps);
+    return 1.0;
+}
+__device__ __forceinline__ double barth_fn(double dq, double minv, double maxv, double dot) {
+    if (dot < 0.0) return 0.0;
+    else if (dq > maxv) return maxv / dq;
+    else if (dq < minv) return minv / dq;
+    return 1.0;
+}
+
+}  // namespace ofx

@@ -1,0 +1,386 @@
+// ofx_sa.hip -- Spalart-Allmaras kernels: eddy viscosity and ghost cells,
+// residual (spectrum + face matrices, source, first-order convection,
+// diffusion, ordered gather), level-scheduled scalar LU-SGS and update.
+#include "ofx_physics.h"
+
+namespace ofx {
+
+constexpr int TBS = 256;
+static inline int nblks(int n) { return (n + TBS - 1) / TBS; }
+
+// TurbSrcFlux::CalcCellVist1Equ (codes/turb/src/TurbSrcFlux.cpp:279-290)
+__device__ __forceinline__ double sa_vist_cell(const DevFields& d, const Consts& k, int T, int i) {
+    double rho = d.q[i], nuet = d.turbq[i], visl = d.visl[i];
+    double olam = rho / (visl + SMALL);
+    double xsi = nuet * olam;
+    double xsi3 = xsi * xsi * xsi;
+    double fv1 = xsi3 / (xsi3 + k.cv13);
+    double vist = fv1 * rho * nuet;
+    double cap = visl * k.max_vis_ratio;
+    return (cap < vist) ? cap : vist;
+}
+
+// UTurbSrcFlux::CalcVist1Equ + SetGhostCellVist (codes/uturb/src/UTurbSrcFlux.cpp:101-125,217-241)
+__global__ void __launch_bounds__(TBS) k_sa_vist(DevMesh m, DevFields d, Consts k) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.T) return;
+    int src = i < m.C ? i : m.lc[i - m.C];
+    d.vist[i] = sa_vist_cell(d, k, m.T, src);
+}
+
+// UTurbBcSolver + TurbBcSolver (codes/uturb/src/UTurbBcSolver.cpp:59-160,
+// codes/turb/src/TurbBcSolver.cpp:142-253)
+__global__ void __launch_bounds__(TBS) k_sa_bc(DevMesh m, DevFields d, Consts k) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.B) return;
+    const int T = m.T;
+    int bt = m.bct[f];
+    int l = m.lc[f], r = m.C + f;
+    double s = d.turbq[l], t;
+    if (bt == BC_INFLOW) t = k.sa_inflow;
+    else if (bt == BC_SOLID_SURFACE) t = -s;
+    else if (bt == BC_FARFIELD) {
+        double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f];
+        const double vfn = 0.0;
+        double rin = d.q[l], uin = d.q[T + l], vin = d.q[2 * T + l], win = d.q[3 * T + l], pin = d.q[4 * T + l];
+        double rref = k.inflow[0], uref = k.inflow[1], vref = k.inflow[2], wref = k.inflow[3], pref = k.inflow[4];
+        double vnref = xfn * uref + yfn * vref + zfn * wref - vfn;
+        double vnin = xfn * uin + yfn * vin + zfn * win - vfn;
+        double gama = k.gama;
+        double cref = sqrt(dabs(k.gama * pref / rref));
+        double cin = sqrt(dabs(gama * pin / rin));
+        double velin = sqrt(sqr3(uin, vin, win));
+        if (velin > cin) {
+            t = (vnin >= 0.0) ? s : k.sa_inflow;
+        } else {
+            double gamm1 = gama - 1.0;
+            double riemp = vnin + 2.0 * cin / gamm1;
+            double riemm = vnref - 2.0 * cref / gamm1;
+            double vnb = 0.5 * (riemp + riemm);
+            t = (vnb >= 0.0) ? s : k.sa_inflow;
+        }
+    } else t = s;  // extrapolation, outflow, symmetry
+    d.turbq[r] = t;
+}
+
+int sa_boundary(ofx_ctx* c) {
+    if (c->k.vismodel != 3) return 0;
+    k_sa_vist<<<nblks(c->m.T), TBS, 0, c->stream>>>(c->m, c->d, c->k);
+    if (c->m.B) k_sa_bc<<<nblks(c->m.B), TBS, 0, c->stream>>>(c->m, c->d, c->k);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// Green-Gauss gradient of nu_tilde (UTurbGrad, codes/uturb/src/UTurbGrad.cpp:51-74)
+__global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int T = m.T;
+    double ax = 0, ay = 0, az = 0;
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int e = m.c2f[kk];
+        int f = e >= 0 ? e : ~e;
+        int l = m.lc[f], r = m.rc[f];
+        double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
+        double delt1 = sqrt(sqr3(xfc - m.xcc[l], yfc - m.ycc[l], zfc - m.zcc[l]));
+        double delt2 = sqrt(sqr3(xfc - m.xcc[r], yfc - m.ycc[r], zfc - m.zcc[r]));
+        double delta = 1.0 / (delt1 + delt2 + SMALL);
+        double cl = delt2 * delta, cr = delt1 * delta;
+        double value = cl * d.turbq[l] + cr * d.turbq[r];
+        double area = m.area[f];
+        double fnxa = m.xfn[f] * area, fnya = m.yfn[f] * area, fnza = m.zfn[f] * area;
+        if (e >= 0) { ax += fnxa * value; ay += fnya * value; az += fnza * value; }
+        else { ax -= fnxa * value; ay -= fnya * value; az -= fnza * value; }
+    }
+    double ovol = 1.0 / m.vol[i];
+    d.tgrad[i] = ax * ovol;
+    d.tgrad[T + i] = ay * ovol;
+    d.tgrad[2 * T + i] = az * ovol;
+}
+
+// Per face: SA spectral radius and off-diagonal face matrices
+// (TurbSpectrum::CalcFaceSpectrum1Equ, codes/turb/src/TurbSpectrum.cpp:75-134),
+// first-order upwind convection (UTurbInvFlux + TurbInvFlux::RoeFlux with
+// turb_ilim = tns_ilim = -1, codes/uturb/src/UTurbInvFlux.cpp:130-202,
+// codes/turb/src/TurbInvFlux.cpp:78-98) and diffusion
+// (UTurbVisFlux::CalcFaceVisFlux1Equ, codes/uturb/src/UTurbVisFlux.cpp:247-369).
+// saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
+__global__ void __launch_bounds__(TBS) k_sa_face(DevMesh m, DevFields d, Consts k) {
+    int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    const int T = m.T, F = m.F;
+    int l = m.lc[f], r = m.rc[f];
+    bool bnd = f < m.B;
+    int rg = bnd ? l : r;
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], farea = m.area[f];
+    const double vfn = 0.0;
+    double rl = d.q[l], rr = d.q[r];
+    double ul = d.q[T + l], vl = d.q[2 * T + l], wl = d.q[3 * T + l];
+    double ur = d.q[T + r], vr = d.q[2 * T + r], wr = d.q[3 * T + r];
+    double nl = d.turbq[l], nr = d.turbq[r];
+    double visll = d.visl[l], vislr = d.visl[r];
+    // spectrum
+    double vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
+    double vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
+    double absVn = dabs(0.5 * (vnl + vnr));
+    double rad = 0.0, m1 = 0.0, m2 = 0.0;
+    rad += farea * 0.5 * absVn;
+    m1 += farea * 0.5 * (-vnl - absVn);
+    m2 += farea * 0.5 * (vnr - absVn);
+    double orl = 1.0 / (rl + SMALL), orr = 1.0 / (rr + SMALL);
+    double nuet1 = visll * orl + nl;
+    double nuet2 = vislr * orr + nr;
+    double nueff = dabs(0.5 * (nuet1 + nuet2));
+    double cvol = 0.5 * (m.vol[l] + m.vol[r]);
+    double ovol = 1.0 / cvol;
+    double areaS2 = farea * farea;
+    double dif = k.oreynolds * ovol * (k.osigma * nueff * areaS2);
+    rad += dif;
+    m1 += -dif;
+    m2 += -dif;
+    // convection on first-order face states (boundary faces: averaged / wall values)
+    double cnl = nl, cnr = nr;
+    double cul = ul, cvl = vl, cwl = wl, cur = ur, cvr = vr, cwr = wr;
+    if (bnd) {
+        double tn = 0.5 * (nl + nr);
+        cnl = tn; cnr = tn;
+        if (m.bct[f] == BC_SOLID_SURFACE && k.vismodel != 0) {
+            cul = cur = 0.0; cvl = cvr = 0.0; cwl = cwr = 0.0;
+        } else {
+            double tu = 0.5 * (ul + ur), tv = 0.5 * (vl + vr), tw = 0.5 * (wl + wr);
+            cul = cur = tu; cvl = cvr = tv; cwl = cwr = tw;
+        }
+    }
+    double cvnl = xfn * cul + yfn * cvl + zfn * cwl - vfn;
+    double cvnr = xfn * cur + yfn * cvr + zfn * cwr - vfn;
+    double vnl_p = 0.5 * (cvnl + dabs(cvnl));
+    double vnr_n = 0.5 * (cvnr - dabs(cvnr));
+    double conv = farea * (vnl_p * 1.0 * cnl + vnr_n * 1.0 * cnr);
+    // diffusion
+    VisGeom G;
+    vis_geom(m, f, l, r, k.iturb_visflux, G);
+    double g1[3] = {d.tgrad[l], d.tgrad[T + l], d.tgrad[2 * T + l]};
+    double g2[3] = {d.tgrad[rg], d.tgrad[T + rg], d.tgrad[2 * T + rg]};
+    double qf, gf[3], dqdn;
+    vis_grad(k.iturb_visflux, G, nl, nr, g1, g2, qf, gf, dqdn);
+    double visl_nv = 0.5 * (visll * orl + vislr * orr);
+    double vist_nv = 0.5 * (nl + nr);
+    double comVis = (visl_nv + vist_nv) * k.osigma;
+    double diff = -k.oreynolds * comVis * dqdn * farea;
+    d.saface[0 * F + f] = rad;
+    d.saface[1 * F + f] = m1;
+    d.saface[2 * F + f] = m2;
+    d.saface[3 * F + f] = conv;
+    d.saface[4 * F + f] = diff;
+}
+
+// TurbCom::CalcSrcSa with CalcWorkVar / CalcSaProd (codes/turb/src/TurbCom.cpp:429-465,771-872)
+__device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, const Consts& k, int i,
+                                          double& res_sa, double& spec_sa) {
+    const int T = m.T;
+    const double one = 1.0, two = 2.0, three = 3.0, six = 6.0, ten = 10.0, zero = 0.0;
+    double dudx = d.grad[(0 * 6 + 1) * T + i], dudy = d.grad[(1 * 6 + 1) * T + i], dudz = d.grad[(2 * 6 + 1) * T + i];
+    double dvdx = d.grad[(0 * 6 + 2) * T + i], dvdy = d.grad[(1 * 6 + 2) * T + i], dvdz = d.grad[(2 * 6 + 2) * T + i];
+    double dwdx = d.grad[(0 * 6 + 3) * T + i], dwdy = d.grad[(1 * 6 + 3) * T + i], dwdz = d.grad[(2 * 6 + 3) * T + i];
+    double rho = d.q[i], nuet = d.turbq[i], visl = d.visl[i];
+    double len_scale = m.dist[i];
+    double olam = rho / (visl + SMALL);
+    double d2 = len_scale * len_scale;
+    double od2 = one / d2;
+    double work1 = dudx + dvdy + dwdz;
+    double work2 = sqr3(dudx, dudy, dudz);
+    double work3 = sqr3(dudy + dvdx, dvdz + dwdy, dwdx + dudz);
+    double work4 = sqrt(sqr3(dwdy - dvdz, dudz - dwdx, dvdx - dudy));
+    if (k.iprod_sa == 1) {
+        double stp = two * work2 + work3 - (2.0 / 3.0) * work1 * work1;
+        work4 = sqrt(dabs(stp));
+    } else if (k.iprod_sa == 4) {
+        double stp = two * work2 + work3 - (2.0 / 3.0) * work1 * work1;
+        double str = work4;
+        work4 = str - two * dmax(zero, str - sqrt(dabs(stp)));
+    } else if (k.iprod_sa == 5) {
+        double stp = two * work2 + work3 - (2.0 / 3.0) * work1 * work1;
+        double str = work4;
+        stp = dabs(stp);
+        work4 = str * (one - dmax(zero, (str * str - stp) / (str * str + stp + SMALL)));
+    }
+    double str = work4;
+    double xsi = nuet * olam + SMALL;
+    double xsi3 = xsi * xsi * xsi;
+    double fv1 = xsi3 / (xsi3 + k.cv13);
+    double fv2 = one - xsi / (one + xsi * fv1);
+    double rs = nuet * k.okarm2 * od2 * k.oreynolds;
+    double nuetRs = nuet * rs;
+    double sBar = rs * fv2;
+    double omega = str;
+    double stdv;
+    if (sBar >= -k.sac2 * omega) stdv = omega + sBar;
+    else {
+        double term1 = (k.sac2 * k.sac2 * omega + k.sac3 * sBar);
+        double term2 = (k.sac3 - 2 * k.sac2) * omega - sBar;
+        stdv = omega + omega * term1 / term2;
+    }
+    double ostd = one / (stdv + SMALL);
+    double r = rs * ostd;
+    r = dmin(r, ten);
+    double r5 = pow(r, 5.0);
+    double gg = r + k.cw2 * (r5 * r - r);
+    double g6 = pow(gg, 6.0);
+    double or6 = one / six;
+    double fw = gg * pow((one + k.cw36) / (g6 + k.cw36), or6);
+    const double ft2 = 0.0;
+    double grd2 = sqr3(d.tgrad[i], d.tgrad[T + i], d.tgrad[2 * T + i]);
+    double prod = k.cb1 * (one - ft2) * stdv * nuet;
+    double diff = k.cb2s * grd2 * k.oreynolds;
+    double dest = (k.cw1k * fw - k.cb1 * ft2) * nuetRs;
+    diff = dmin(k.rprod * prod, diff);
+    double srcTerm = prod - dest;
+    srcTerm += diff;
+    double p2 = fv1 / (xsi + SMALL);
+    double p3 = one + xsi * fv1;
+    double dfv2dk = (three * k.cv13 * pow(p2, 2.0) - one) / pow(p3, 2.0);
+    double dsdnu = k.oreynolds * k.okarm2 * od2 * (fv2 + xsi * dfv2dk);
+    double dfwdg = fw / (gg + SMALL) * (one - g6 / (g6 + k.cw36));
+    double dgdr = one + k.cw2 * (six * r5 - one);
+    double drdnu = k.oreynolds * k.okarm2 * od2 * ostd * (one - nuet * ostd * dsdnu);
+    double dfwdnu = dfwdg * dgdr * drdnu;
+    double prod0 = k.cb1 * (one - ft2) * stdv;
+    double dest0 = (k.cw1 * fw - k.cb1 * ft2 * k.okarm2) * nuet * od2 * k.oreynolds;
+    double prodp = k.cb1 * ((one - ft2) * dsdnu);
+    double destp = (k.cw1 * (fw + dfwdnu * nuet) - k.cb1 * ft2 * k.okarm2) * od2 * k.oreynolds;
+    double prde0 = prod0 - dest0;
+    double prdep = prodp - destp;
+    double part1 = -0.5 * (prde0 - dabs(prde0));
+    double part2 = -0.5 * (prdep - dabs(prdep)) * nuet;
+    spec_sa = part1 + part2;
+    res_sa = srcTerm;
+}
+
+// Per cell: implicit diagonal (unsteady term, face radii in face order, then
+// source Jacobian) and residual (source, then convection, then diffusion, each
+// gathered in ascending face order) -- TurbCalcRHS order
+// (codes/turb/src/TurbRhs.cpp:57-71).
+__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int F = m.F;
+    double vol = m.vol[i];
+    double ts = d.timestep[i] * k.turb_cfl_ratio;
+    double imp = 0.0;
+    imp += (1.0 / ts) * vol;
+    int b = m.c2fOff[i], n = m.c2fOff[i + 1];
+    for (int kk = b; kk < n; ++kk) {
+        int e = m.c2f[kk];
+        imp += d.saface[e >= 0 ? e : ~e];
+    }
+    double res_sa, spec_sa;
+    sa_source(m, d, k, i, res_sa, spec_sa);
+    imp += spec_sa * vol;
+    double res = 0.0;
+    res += res_sa * vol;
+    for (int kk = b; kk < n; ++kk) {
+        int e = m.c2f[kk];
+        if (e >= 0) res -= d.saface[3 * F + e];
+        else res += d.saface[3 * F + ~e];
+    }
+    for (int kk = b; kk < n; ++kk) {
+        int e = m.c2f[kk];
+        if (e >= 0) res -= d.saface[4 * F + e];
+        else res += d.saface[4 * F + ~e];
+    }
+    d.turbimpsr[i] = imp;
+    d.turbres[i] = res;
+}
+
+int sa_residual(ofx_ctx* c) {
+    if (c->k.vismodel != 3) return 0;
+    const DevMesh& m = c->m;
+    if (m.B) k_sa_bc<<<nblks(m.B), TBS, 0, c->stream>>>(m, c->d, c->k);
+    k_sa_grad<<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d);
+    k_sa_face<<<nblks(m.F), TBS, 0, c->stream>>>(m, c->d, c->k);
+    k_sa_cell<<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// UTurbLusgs lower / upper cell (codes/uturb/src/UTurbLusgs.cpp:90-304,
+// codes/turb/src/TurbLusgs.cpp:123-254): matrix_l when the cell is the face's
+// right cell (GCom::SetGeometry swap), matrix_r otherwise.
+template <bool UPPER>
+__device__ __forceinline__ void sa_lusgs_cell(const DevMesh& m, const DevFields& d, int i) {
+    const int F = m.F;
+    double rhs0 = 0.0;
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int e = m.c2f[kk];
+        int f = e >= 0 ? e : ~e;
+        if (f < m.B) continue;
+        int j = e >= 0 ? m.rc[f] : m.lc[f];
+        if (UPPER ? (j < i) : (j > i)) continue;
+        double mat = (e >= 0) ? d.saface[2 * F + f] : d.saface[1 * F + f];
+        rhs0 += mat * d.turbdq[j];
+    }
+    double rad = d.turbimpsr[i];
+    if (!UPPER) { double t = d.turbres[i] - rhs0; t /= rad; d.turbdq[i] = t; }
+    else { double t = -rhs0; t /= rad; d.turbdq[i] += t; }
+}
+
+template <bool UPPER>
+__global__ void __launch_bounds__(1024) k_sa_sweep_1wg(DevMesh m, DevFields d) {
+    const int* off = UPPER ? m.upOff : m.loOff;
+    const int* ord = UPPER ? m.upOrder : m.loOrder;
+    int nl = UPPER ? m.nLevUp : m.nLevLo;
+    for (int L = 0; L < nl; ++L) {
+        for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
+            sa_lusgs_cell<UPPER>(m, d, ord[idx]);
+        __syncthreads();
+    }
+}
+
+template <bool UPPER>
+__global__ void __launch_bounds__(TBS) k_sa_sweep_level(DevMesh m, DevFields d, int b, int e) {
+    int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= e) return;
+    sa_lusgs_cell<UPPER>(m, d, (UPPER ? m.upOrder : m.loOrder)[idx]);
+}
+
+int sa_lusgs(ofx_ctx* c) {
+    if (c->k.vismodel != 3) return 0;
+    const DevMesh& m = c->m;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool up = pass == 1;
+        const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
+        int w = 0;
+        for (size_t L = 0; L + 1 < off.size(); ++L) w = std::max(w, off[L + 1] - off[L]);
+        if (w <= 8192) {
+            if (up) k_sa_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d);
+            else k_sa_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d);
+        } else {
+            for (size_t L = 0; L + 1 < off.size(); ++L) {
+                int b = off[L], e = off[L + 1];
+                if (up) k_sa_sweep_level<true><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
+                else k_sa_sweep_level<false><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
+            }
+        }
+        OFX_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// TurbUpdate::CalcFlowField1Equ, crelax = 1 (codes/turb/src/TurbUpdate.cpp:58-69)
+__global__ void __launch_bounds__(TBS) k_sa_update(DevMesh m, DevFields d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double qOld = d.turbq[i], dqNew = d.turbdq[i];
+    double qNew = qOld + dqNew;
+    qNew = (1.0 - 1.0) * qOld + 1.0 * qNew;
+    d.turbq[i] = qNew;
+}
+
+int sa_update(ofx_ctx* c) {
+    if (c->k.vismodel != 3) return 0;
+    k_sa_update<<<nblks(c->m.C), TBS, 0, c->stream>>>(c->m, c->d);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ofx

@@ -1,0 +1,320 @@
+"""Deterministic synthetic grids for the benchmark and parity workloads.
+
+Every generator returns a `Grid` in the reference's single-zone layout
+(boundary faces first, face nodes ordered so that the 2-D face normal
+(y2-y1, x1-x2) points from the left cell to the right cell, as
+FaceMesh::CalcFaceNormal2D computes it, codes/geometry/src/FaceMesh.cpp:112-136).
+
+* flat_plate      -- laminar flat plate, quads, geometric wall stretching
+* naca_hybrid     -- NACA 4-digit airfoil O-grid: quad wall layers plus
+                     triangles out to a far-field circle
+
+Cell numbering matters for LU-SGS (it defines the lower/upper split).  The
+generators number cells in their natural structured order; `renumber()`
+applies any permutation (e.g. `multicolor_order`) to a grid, and both the GPU
+library and the CPU oracle see the same numbering.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .mesh import BC_FARFIELD, BC_OUTFLOW, BC_SOLID_SURFACE, BC_SYMMETRY, Grid
+
+
+# ---------------------------------------------------------------- assembly
+
+def grid_from_cells(x, y, cells, bc_of_edge, dim=2):
+    """Assemble a 2-D face list from polygon cells given CCW.
+
+    cells: list/array of node index rows (-1 padded) in counter-clockwise order.
+    bc_of_edge(a, b) -> bc type for a boundary edge (a, b) in the owner's CCW order.
+    """
+    cells = np.asarray(cells)
+    nc = cells.shape[0]
+    nvert = (cells >= 0).sum(axis=1)
+    # all cell edges in CCW order
+    e_cell, e_a, e_b = [], [], []
+    for k in range(cells.shape[1]):
+        k1 = (np.arange(nc), )
+        a = cells[:, k]
+        nxt = np.where(k + 1 < nvert, (k + 1), 0)
+        b = cells[np.arange(nc), nxt]
+        valid = k < nvert
+        e_cell.append(np.nonzero(valid)[0])
+        e_a.append(a[valid])
+        e_b.append(b[valid])
+    e_cell = np.concatenate(e_cell)
+    e_a = np.concatenate(e_a)
+    e_b = np.concatenate(e_b)
+    key_lo = np.minimum(e_a, e_b).astype(np.int64)
+    key_hi = np.maximum(e_a, e_b).astype(np.int64)
+    key = key_lo * (len(x) + 1) + key_hi
+    order = np.argsort(key, kind="stable")
+    key_s = key[order]
+    # group edges
+    starts = np.concatenate([[0], np.nonzero(np.diff(key_s))[0] + 1])
+    counts = np.diff(np.concatenate([starts, [len(key_s)]]))
+    if np.any(counts > 2):
+        raise ValueError("non-manifold edge")
+    bnd_idx = order[starts[counts == 1]]
+    pair0 = order[starts[counts == 2]]
+    pair1 = order[starts[counts == 2] + 1]
+    # interior: lc = lower cell id, nodes in lc's CCW order
+    c0, c1 = e_cell[pair0], e_cell[pair1]
+    swap = c1 < c0
+    lc_i = np.where(swap, c1, c0)
+    rc_i = np.where(swap, c0, c1)
+    a_i = np.where(swap, e_a[pair1], e_a[pair0])
+    b_i = np.where(swap, e_b[pair1], e_b[pair0])
+    srt = np.lexsort((np.maximum(lc_i, rc_i), lc_i))
+    lc_i, rc_i, a_i, b_i = lc_i[srt], rc_i[srt], a_i[srt], b_i[srt]
+    # boundary faces: owner CCW order; group by bc type then owner
+    lc_b = e_cell[bnd_idx]
+    a_b = e_a[bnd_idx]
+    b_b = e_b[bnd_idx]
+    bct = np.array([bc_of_edge(int(a), int(b)) for a, b in zip(a_b, b_b)], dtype=np.int32)
+    srt = np.lexsort((lc_b, bct))
+    lc_b, a_b, b_b, bct = lc_b[srt], a_b[srt], b_b[srt], bct[srt]
+    nb = len(lc_b)
+    lc = np.concatenate([lc_b, lc_i]).astype(np.int32)
+    rc = np.concatenate([-np.ones(nb, np.int64), rc_i]).astype(np.int32)
+    fn = np.stack([np.concatenate([a_b, a_i]), np.concatenate([b_b, b_i])], axis=1).astype(np.int32).ravel()
+    nf = len(lc)
+    g = Grid(dim=dim, x=np.asarray(x, np.float64), y=np.asarray(y, np.float64), z=np.zeros(len(x)),
+             face_node_count=np.full(nf, 2, np.int32), face_nodes=fn, lc=lc, rc=rc, bc_type=bct,
+             bc_name_id=bct.copy(), n_cells=nc)
+    g.check()
+    return g
+
+
+def cell_areas(x, y, cells):
+    cells = np.asarray(cells)
+    nv = (cells >= 0).sum(axis=1)
+    a = np.zeros(cells.shape[0])
+    for k in range(cells.shape[1]):
+        i = cells[:, k]
+        j = cells[np.arange(len(cells)), np.where(k + 1 < nv, k + 1, 0)]
+        valid = k < nv
+        a += np.where(valid, x[i] * y[j] - x[j] * y[i], 0.0)
+    return 0.5 * a
+
+
+# ---------------------------------------------------------------- numbering
+
+def cell_adjacency(g: Grid):
+    nb = g.n_bfaces
+    return g.lc[nb:], g.rc[nb:]
+
+
+def multicolor_order(g: Grid):
+    """Greedy colouring of the cell graph; cells sorted by (colour, old id).
+
+    With this numbering every LU-SGS sweep has as many dependency levels as
+    colours (a cell's lower neighbours all carry smaller colours)."""
+    a, b = cell_adjacency(g)
+    n = g.n_cells
+    nbr = [[] for _ in range(n)]
+    for u, v in zip(a.tolist(), b.tolist()):
+        nbr[u].append(v)
+        nbr[v].append(u)
+    color = np.full(n, -1, np.int64)
+    for i in range(n):
+        used = {color[j] for j in nbr[i] if color[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        color[i] = c
+    return np.lexsort((np.arange(n), color))
+
+
+def renumber(g: Grid, order) -> Grid:
+    """Return a copy with cells renumbered: new cell k is old cell order[k]."""
+    order = np.asarray(order)
+    new_of_old = np.empty(g.n_cells, np.int64)
+    new_of_old[order] = np.arange(g.n_cells)
+    nb = g.n_bfaces
+    lc = new_of_old[g.lc]
+    rc = g.rc.astype(np.int64).copy()
+    rc[nb:] = new_of_old[g.rc[nb:]]
+    fn = g.face_nodes.reshape(-1, 2).copy() if g.dim == 2 else None
+    # interior faces: keep lc < rc (flip node order when swapping) and sort
+    li, ri = lc[nb:].copy(), rc[nb:].copy()
+    sw = li > ri
+    li[sw], ri[sw] = ri[sw].copy(), li[sw].copy()
+    if fn is None:
+        raise NotImplementedError("renumber is implemented for 2-D grids")
+    fi = fn[nb:].copy()
+    fi[sw] = fi[sw][:, ::-1]
+    srt = np.lexsort((ri, li))
+    fb = fn[:nb]
+    lb = lc[:nb]
+    srtb = np.lexsort((lb, g.bc_type))
+    out = Grid(dim=g.dim, x=g.x, y=g.y, z=g.z, face_node_count=g.face_node_count.copy(),
+               face_nodes=np.concatenate([fb[srtb], fi[srt]]).astype(np.int32).ravel(),
+               lc=np.concatenate([lb[srtb], li[srt]]).astype(np.int32),
+               rc=np.concatenate([-np.ones(nb, np.int64), ri[srt]]).astype(np.int32),
+               bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_type[srtb].copy(), n_cells=g.n_cells)
+    out.check()
+    return out
+
+
+# ---------------------------------------------------------------- flat plate
+
+def geometric(n, h0, ratio):
+    """n+1 coordinates 0, h0, h0(1+r), ... (geometric spacing)."""
+    d = h0 * ratio ** np.arange(n)
+    return np.concatenate([[0.0], np.cumsum(d)])
+
+
+def flat_plate(ni: int = 161, nj: int = 97, x0: float = -0.5, x1: float = 1.0, height: float = 0.5,
+               ratio: float = 1.1) -> Grid:
+    """Laminar flat plate on [x0,x1]x[0,height]: ni x nj nodes, wall-normal spacing
+    growing by `ratio` from the wall (y=0), x clustered at the leading edge x=0.
+    BCs: y=0, x<0 symmetry; y=0, x>=0 adiabatic wall; x=x0 and y=height far field;
+    x=x1 outflow (the region layout of test/plateuns2dslau2/grid/bcRegionMap.txt)."""
+    nj_c = nj - 1
+    ys = geometric(nj_c, 1.0, ratio)
+    ys = ys / ys[-1] * height
+    # x: cells upstream of the leading edge and along the plate, both clustered at x = 0
+    n_up = max(2, (ni - 1) // 5)
+    n_pl = ni - 1 - n_up
+    xp = geometric(n_pl, 1.0, 1.03)
+    xp = xp / xp[-1] * x1
+    xu = geometric(n_up, 1.0, 1.08)
+    xu = -(xu / xu[-1] * (-x0))[::-1]
+    xs = np.concatenate([xu[:-1], xp])
+    assert len(xs) == ni
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    x, y = X.ravel(), Y.ravel()
+    nid = lambda i, j: i * nj + j
+    I, J = np.meshgrid(np.arange(ni - 1), np.arange(nj - 1), indexing="ij")
+    I, J = I.ravel(), J.ravel()
+    cells = np.stack([nid(I, J), nid(I + 1, J), nid(I + 1, J + 1), nid(I, J + 1)], axis=1)
+
+    def bc(a, b):
+        xa, ya, xb, yb = x[a], y[a], x[b], y[b]
+        if ya == 0.0 and yb == 0.0:
+            return BC_SOLID_SURFACE if min(xa, xb) >= 0.0 else BC_SYMMETRY
+        if xa == x1 and xb == x1:
+            return BC_OUTFLOW
+        return BC_FARFIELD
+
+    return grid_from_cells(x, y, cells, bc)
+
+
+# ---------------------------------------------------------------- airfoil
+
+def naca4(m: float, p: float, t: float, n: int, closed_te: bool = True):
+    """NACA 4-digit surface, n points, clockwise from the trailing edge along
+    the lower surface to the leading edge and back along the upper surface
+    (the trailing-edge point appears once).  m, p, t as fractions of chord."""
+    a4 = -0.1036 if closed_te else -0.1015
+    npts = n // 2 + 1
+    beta = np.linspace(0.0, math.pi, npts)
+    xc = 0.5 * (1.0 - np.cos(beta))  # 0 .. 1, cosine clustered
+    yt = 5.0 * t * (0.2969 * np.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2 + 0.2843 * xc ** 3 + a4 * xc ** 4)
+    if m > 0.0 and p > 0.0:
+        yc = np.where(xc < p, m / p ** 2 * (2 * p * xc - xc ** 2), m / (1 - p) ** 2 * ((1 - 2 * p) + 2 * p * xc - xc ** 2))
+        dyc = np.where(xc < p, 2 * m / p ** 2 * (p - xc), 2 * m / (1 - p) ** 2 * (p - xc))
+    else:
+        yc = np.zeros_like(xc)
+        dyc = np.zeros_like(xc)
+    th = np.arctan(dyc)
+    xu, yu = xc - yt * np.sin(th), yc + yt * np.cos(th)
+    xl, yl = xc + yt * np.sin(th), yc - yt * np.cos(th)
+    # lower surface from TE to LE, then upper from LE to TE (excluding duplicates)
+    xs = np.concatenate([xl[::-1], xu[1:-1]])
+    ys = np.concatenate([yl[::-1], yu[1:-1]])
+    if closed_te:
+        ys[0] = 0.5 * (yl[-1] + yu[-1])
+        xs[0] = 1.0
+    return xs, ys
+
+
+def _smooth_normals(xs, ys, passes=20):
+    n = len(xs)
+    tx = np.roll(xs, -1) - np.roll(xs, 1)
+    ty = np.roll(ys, -1) - np.roll(ys, 1)
+    nx, ny = -ty, tx          # surface traversed clockwise -> outward normal
+    L = np.hypot(nx, ny)
+    nx, ny = nx / L, ny / L
+    for _ in range(passes):
+        nx = 0.5 * nx + 0.25 * (np.roll(nx, 1) + np.roll(nx, -1))
+        ny = 0.5 * ny + 0.25 * (np.roll(ny, 1) + np.roll(ny, -1))
+        L = np.hypot(nx, ny)
+        nx, ny = nx / L, ny / L
+    return nx, ny
+
+
+def naca_hybrid(n_surf: int = 512, n_quad: int = 40, ratio: float = 1.15, reynolds: float = 6.5e6,
+                yplus: float = 1.0, n_outer: int = 98, radius: float = 30.0,
+                m: float = 0.0, p: float = 0.4, t: float = 0.12) -> Grid:
+    """Airfoil O-grid: n_quad quad layers (first height for y+ ~ yplus at
+    `reynolds`, growth `ratio`), then n_outer layers of quads each split into
+    two triangles, reaching a far-field circle of `radius` chords.
+    Cells: n_surf * (n_quad + 2 n_outer)."""
+    xs, ys = naca4(m, p, t, n_surf)
+    ns = len(xs)
+    # first cell height for y+ (flat-plate skin friction estimate)
+    cf = 0.026 / reynolds ** (1.0 / 7.0)
+    h0 = yplus / (reynolds * math.sqrt(0.5 * cf))
+    d_in = geometric(n_quad, h0, ratio)
+    nx, ny = _smooth_normals(xs, ys)
+    # outer layers: distance grows geometrically from the last wall layer to `radius`
+    d_last = d_in[-1]
+    step0 = d_in[-1] - d_in[-2]
+    # solve growth g: step0 * (g^n_outer - 1)/(g-1) = radius - d_last (approximately, by bisection)
+    lo, hi = 1.0001, 2.0
+    for _ in range(100):
+        g = 0.5 * (lo + hi)
+        s = step0 * (g ** n_outer - 1.0) / (g - 1.0)
+        if s > radius:
+            hi = g
+        else:
+            lo = g
+    d_out = d_last + step0 * (g ** np.arange(1, n_outer + 1) - 1.0) / (g - 1.0)
+    dists = np.concatenate([d_in, d_out])
+    nl = len(dists)  # layers of nodes (including the surface)
+    # march outward: each layer moves along the smoothed outward normal of the
+    # previous layer curve; smoothing grows with distance so the layers relax
+    # from body-fitted to circular
+    X = np.zeros((nl, ns))
+    Y = np.zeros((nl, ns))
+    X[0], Y[0] = xs, ys
+    for j in range(1, nl):
+        passes = int(min(400, 4 + 2 * j + (40 * max(0.0, dists[j] - d_last) / radius) * 10))
+        nx, ny = _smooth_normals(X[j - 1], Y[j - 1], passes=passes)
+        step = dists[j] - dists[j - 1]
+        X[j] = X[j - 1] + step * nx
+        Y[j] = Y[j - 1] + step * ny
+    x = X.ravel()
+    y = Y.ravel()
+    nid = lambda j, i: j * ns + (i % ns)
+    cells = []
+    for j in range(nl - 1):
+        i = np.arange(ns)
+        a, b, c, d = nid(j, i), nid(j, i + 1), nid(j + 1, i + 1), nid(j + 1, i)
+        # surface is clockwise, layers go outward: (a, b, c, d) is counter-clockwise
+        if j < n_quad:
+            cells.append(np.stack([a, b, c, d], axis=1))
+        else:
+            t1 = np.stack([a, b, c, -np.ones_like(a)], axis=1)
+            t2 = np.stack([a, c, d, -np.ones_like(a)], axis=1)
+            cells.append(np.stack([t1, t2], axis=1).reshape(-1, 4))
+    cells = np.concatenate(cells)
+    ar = cell_areas(x, y, cells)
+    if np.any(ar <= 0):
+        raise ValueError(f"naca_hybrid: {int((ar <= 0).sum())} cells with non-positive area")
+    jmax = nl - 1
+
+    def bc(a, b):
+        ja, jb = a // ns, b // ns
+        if ja == 0 and jb == 0:
+            return BC_SOLID_SURFACE
+        if ja == jmax and jb == jmax:
+            return BC_FARFIELD
+        raise ValueError("unexpected boundary edge")
+
+    return grid_from_cells(x, y, cells, bc)

@@ -1,0 +1,185 @@
+"""Python view of libofx.so: one zone of the unstructured NS + SA solver on one GPU.
+
+`Solver` mirrors the reference's task interface for the path: each method is
+one registered task of the reference solver (UPDATE_RESIDUALS, CALC_TIME_STEP,
+LUSGS_LOWER/UPPER_SWEEP, UPDATE_FLOWFIELD, CALC_BOUNDARY, ...) and `iterate`
+is the whole pseudo-time step.  All compute runs in the HIP library; this
+module only moves arrays across the C ABI.  It raises if the library is not
+built or no GPU is visible -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from .config import CaseConfig
+from .mesh import Grid
+
+LIB_PATH = Path(__file__).resolve().parent / "libofx.so"
+
+FIELD = dict(Q=0, TURBQ=1, RES=2, TURBRES=3, DQDX=4, DTDX=5, LIMITER=6, TIMESTEP=7, INVSR=8, VISSR=9,
+             DQ=10, TURBDQ=11, TEMPR=12, VISL=13, VIST=14, DIST=15, VOL=16, XCC=17, FACE_NORMAL=18,
+             FACE_CENTER=19, TURBDQDX=20, TURBIMPSR=21, IMPSR=22)
+
+
+class OfxNorms(ctypes.Structure):
+    _fields_ = [("ns", ctypes.c_double * 5), ("sa", ctypes.c_double)]
+
+
+class OfxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library():
+    """Load libofx.so (built by oneflow_amd.build); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise OfxError(f"{LIB_PATH} is not built; run __graft_entry__.build() or python oneflow_amd/build.py")
+    L = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.ofx_last_error.restype = ctypes.c_char_p
+    L.ofx_ctx_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+    L.ofx_ctx_destroy.argtypes = [P]
+    L.ofx_mesh_upload.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ip, ip, ip, ip, ip]
+    L.ofx_mesh_info.argtypes = [P, ip]
+    L.ofx_config_set.argtypes = [P, ctypes.c_void_p]
+    for name in ("ofx_init_freestream", "ofx_ns_boundary", "ofx_sa_boundary", "ofx_residual_ns",
+                 "ofx_residual_sa", "ofx_lusgs_ns", "ofx_lusgs_sa", "ofx_update_ns", "ofx_update_sa", "ofx_sync"):
+        getattr(L, name).argtypes = [P]
+    L.ofx_get_field.argtypes = [P, ctypes.c_int, dp]
+    L.ofx_set_field.argtypes = [P, ctypes.c_int, dp]
+    L.ofx_timestep.argtypes = [P, ctypes.c_int]
+    L.ofx_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
+    L.ofx_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
+    L.ofx_aero.argtypes = [P, dp]
+    L.ofx_profile_last.argtypes = [P, dp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Solver:
+    """One zone + case resident on one GPU (device buffers owned by the context)."""
+
+    def __init__(self, grid: Grid, cfg: CaseConfig, device: int = 0):
+        L = load_library()
+        self._L = L
+        h = ctypes.c_void_p()
+        self._check(L.ofx_ctx_create(ctypes.byref(h), device))
+        self.h = h
+        g = grid
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z))
+        fnc, fnn, lc, rc, bct = (np.ascontiguousarray(a, dtype=np.int32) for a in
+                                 (g.face_node_count, g.face_nodes, g.lc, g.rc, g.bc_type))
+        self._check(L.ofx_mesh_upload(h, g.dim, g.n_nodes, _dp(x), _dp(y), _dp(z), g.n_faces, g.n_cells,
+                                      g.n_bfaces, _ip(fnc), _ip(fnn), _ip(lc), _ip(rc), _ip(bct)))
+        info = np.zeros(5, np.int32)
+        self._check(L.ofx_mesh_info(h, _ip(info)))
+        self.n_cells, self.n_faces, self.n_bfaces, self.n_tcell, self.n_levels = (int(v) for v in info)
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        self._check(L.ofx_config_set(h, ctypes.byref(self._c)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OfxError(f"libofx error {rc}: {self._L.ofx_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self._L.ofx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_shape(self, name):
+        T, F = self.n_tcell, self.n_faces
+        return {"Q": (5, T), "RES": (5, T), "LIMITER": (5, T), "DQ": (5, T), "DQDX": (3, 5, T),
+                "DTDX": (3, T), "TURBDQDX": (3, T), "XCC": (3, T), "FACE_NORMAL": (4, F),
+                "FACE_CENTER": (3, F), "DIST": (self.n_cells,)}.get(name, (T,))
+
+    def get(self, name):
+        a = np.zeros(self.field_shape(name), np.float64)
+        self._check(self._L.ofx_get_field(self.h, FIELD[name], _dp(a)))
+        return a
+
+    def set(self, name, value):
+        a = np.ascontiguousarray(value, dtype=np.float64).reshape(self.field_shape(name))
+        self._check(self._L.ofx_set_field(self.h, FIELD[name], _dp(a)))
+
+    # ---- tasks (reference task names in brackets)
+    def init_freestream(self):        # INIT_FLOWFIELD + INIT_FINAL
+        self._check(self._L.ofx_init_freestream(self.h))
+
+    def ns_boundary(self):            # CALC_BOUNDARY (NS)
+        self._check(self._L.ofx_ns_boundary(self.h))
+
+    def sa_boundary(self):            # CALC_BOUNDARY (SA)
+        self._check(self._L.ofx_sa_boundary(self.h))
+
+    def timestep(self, it):           # CALC_TIME_STEP
+        self._check(self._L.ofx_timestep(self.h, it))
+
+    def residual_ns(self):            # UPDATE_RESIDUALS (NS)
+        self._check(self._L.ofx_residual_ns(self.h))
+
+    def residual_sa(self):            # UPDATE_RESIDUALS (SA)
+        self._check(self._L.ofx_residual_sa(self.h))
+
+    def lusgs_ns(self):               # LUSGS_LOWER_SWEEP + LUSGS_UPPER_SWEEP (NS)
+        self._check(self._L.ofx_lusgs_ns(self.h))
+
+    def lusgs_sa(self):               # LUSGS_LOWER_SWEEP + LUSGS_UPPER_SWEEP (SA)
+        self._check(self._L.ofx_lusgs_sa(self.h))
+
+    def update_ns(self):              # UPDATE_FLOWFIELD_LUSGS (NS)
+        self._check(self._L.ofx_update_ns(self.h))
+
+    def update_sa(self):              # UPDATE_FLOWFIELD_LUSGS (SA)
+        self._check(self._L.ofx_update_sa(self.h))
+
+    def sync(self):
+        self._check(self._L.ofx_sync(self.h))
+
+    def norms(self):                  # DUMP_RESIDUAL
+        n = OfxNorms()
+        self._check(self._L.ofx_norms_get(self.h, ctypes.byref(n)))
+        return np.array(list(n.ns) + [n.sa])
+
+    def iterate(self, first_iter, n, with_norms=True):
+        if with_norms:
+            arr = (OfxNorms * n)()
+            self._check(self._L.ofx_iterate(self.h, first_iter, n, arr))
+            return np.array([list(a.ns) + [a.sa] for a in arr])
+        self._check(self._L.ofx_iterate(self.h, first_iter, n, None))
+        return None
+
+    def aero(self):                   # DUMP_AERODYNAMIC: CL, CD, CD_pres, CD_visc
+        a = np.zeros(4)
+        self._check(self._L.ofx_aero(self.h, _dp(a)))
+        return a
+
+    def profile_last(self):
+        a = np.zeros(2)
+        self._check(self._L.ofx_profile_last(self.h, _dp(a)))
+        return a
