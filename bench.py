@@ -1,20 +1,25 @@
 """Benchmark: cell-residual evaluations/s of the NS + SA residual on one MI355X.
 
-Workload (the single-GPU configuration the metric is quoted on): NACA0012 from
-the analytic 4-digit formula, hybrid grid of 40 quad wall layers (first height
-for y+ ~ 1 at Re 6.5e6, growth 1.15) plus triangles out to 30 chords,
-120,832 cells; Mach 0.8, AoA 1.25 deg, Re 6.5e6, Roe + Spalart-Allmaras +
-Venkatakrishnan limiter, FP64.  Synthetic grid, freestream start, then a few
-pseudo-time steps so the limiter and SA terms are active before timing.
+Workload: NACA0012 from the analytic 4-digit formula, hybrid grid of 40 quad
+wall layers (first height for y+ ~ 1 at Re 6.5e6, growth 1.15) plus
+triangles out to 30 chords, 120,832 cells; Mach 0.8, AoA 1.25 deg, Re 6.5e6,
+Roe + Spalart-Allmaras + Venkatakrishnan limiter, FP64.  Synthetic grid,
+freestream start, then a few pseudo-time steps so the limiter and SA terms
+are active before timing.
 
 One step = one full residual evaluation: the NS residual (gradients, limiter,
 MUSCL + Roe, viscous flux, ordered gather) and the SA residual (spectrum,
 source, convection, diffusion, gather) -- UPDATE_RESIDUALS of both solvers.
 value = nTCell * steps / seconds (cell-residual evaluations per second).
 
-Roofline: algorithmic HBM bytes of one residual evaluation,
-B = 552 * nTCell + 64 * F (each array counted once, intermediates written and
-read once, FP64), over the measured time of one evaluation; peak 8 TB/s.
+roofline: the dominant kernel of the step (largest average duration among the
+NS residual kernels, each timed with HIP events on the solver's stream over
+--kernel-iters back-to-back launches right after the timed steps); achieved = its algorithmic bytes per launch (each array it must
+read or write counted once, see kernel_bytes) / its average duration; peak
+8 TB/s.  step_roofline: the whole evaluation's algorithmic bytes
+552 * nTCell + 64 * F over ms_per_step.  traffic: measured HBM bytes per
+launch of that kernel from the committed rocprofv3 counter summary, when one
+for this workload is present.
 
 cpu_baseline: the CPU oracle (single thread, the reference's loops) timed on
 this host on a bounded sample of the same workload.
@@ -32,6 +37,42 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = ROOT / "profiles" / "naca0012_c1_hbm_traffic.txt"
+
+
+def kernel_bytes(C, B, F, dim):
+    """Algorithmic bytes per launch of the NS residual kernels: every array the
+    kernel must read or write counted once (FP64 values, int32 indices).  In 2-D
+    the w component and z coordinates are neither read nor written."""
+    T = C + B
+    nnz = 2 * F - B                  # entries of the cell -> face lists
+    nq = 4 if dim == 2 else 5        # primitive components carried
+    D = dim
+    idx = 4 * (C + 1) + 4 * nnz      # c2f offsets + entries
+    return {
+        # q, centroids, volume, c2f + c2n, face centre/normal/area -> grad of nq+T, min/max
+        "grad": 8 * nq * T + 8 * D * T + 8 * C + idx + 4 * nnz + 8 * (2 * D + 1) * F
+                + 8 * (nq + 1) * D * C + 8 * 2 * nq * C,
+        # grad, min/max, centroids, c2f, face centres -> limiter
+        "limiter": 8 * nq * D * C + 8 * 2 * nq * C + 8 * D * C + idx + 8 * D * F + 8 * nq * C,
+        # lc/rc, face normal/area/centre, q, centroids, grads (nq + T), limiter,
+        # laminar + eddy viscosity -> inviscid and viscous face fluxes
+        "face": 8 * F + 8 * (D + 1) * F + 8 * D * F + 8 * nq * T + 8 * D * T + 8 * (nq + 1) * D * T
+                + 8 * nq * T + 16 * T + 2 * 8 * nq * F,
+        # c2f, both face fluxes -> residual
+        "gather": idx + 2 * 8 * nq * F + 8 * nq * C,
+    }
+
+
+def read_traffic(kernel):
+    """Measured HBM bytes per launch for `kernel` from the counter summary, or None."""
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    for line in TRAFFIC_SUMMARY.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
 
 
 def c1_config():
@@ -40,10 +81,13 @@ def c1_config():
                       reynolds=6.5e6, centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
 
 
-def c1_grid():
+def c1_grid(numbering="multicolor"):
     from oneflow_amd import meshgen
-    return meshgen.naca_hybrid(n_surf=512, n_quad=40, ratio=1.15, reynolds=6.5e6, n_outer=98, radius=30.0,
-                               m=0.0, p=0.4, t=0.12)
+    g = meshgen.naca_hybrid(n_surf=512, n_quad=40, ratio=1.15, reynolds=6.5e6, n_outer=98, radius=30.0,
+                            m=0.0, p=0.4, t=0.12)
+    if numbering == "multicolor":
+        g = meshgen.renumber(g, meshgen.multicolor_order(g))
+    return g
 
 
 def cpu_baseline(grid, cfg, budget_s=10.0):
@@ -73,6 +117,9 @@ def main():
     ap.add_argument("--pre-iters", type=int, default=20, help="pseudo-time steps before timing")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--numbering", default="multicolor", choices=["natural", "multicolor"])
+    ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -81,7 +128,7 @@ def main():
     from oneflow_amd.solver import Solver
 
     cfg = c1_config()
-    grid = c1_grid()
+    grid = c1_grid(args.numbering)
     print(f"[bench] grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces}", file=sys.stderr, flush=True)
     s = Solver(grid, cfg, device=0)
     s.init_freestream()
@@ -103,18 +150,35 @@ def main():
     ms_per_step = 1e3 * sec / args.steps
     value = T * args.steps / sec
 
-    bytes_alg = 552.0 * T + 64.0 * F
-    achieved = bytes_alg / (ms_per_step * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "residual evaluation (NS+SA kernels, one stream)",
-                "bytes_per_launch": bytes_alg}
+    copy_ms = s.copy_bench(args.copy_bytes, iters=20)
+    copy_gbs = 2.0 * args.copy_bytes / (copy_ms * 1e-3) / 1e9
+    kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, 2)
+    step_bytes = 552.0 * T + 64.0 * F
+    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+    # each NS residual kernel timed with HIP events on the solver's stream over
+    # back-to-back launches on the state the timed steps left
+    names = {"grad": "k_ns_grad", "limiter": "k_ns_limiter", "face": "k_ns_face", "gather": "k_ns_gather"}
+    prof = {key: s.time_kernel(key, iters=args.kernel_iters) for key in names}
+    s.residual_ns()  # leave a consistent residual behind
+    s.sync()
+    kernels = {}
+    for key, kname in names.items():
+        ms = prof[key]
+        kernels[kname] = {"avg_us": 1e3 * ms, "bytes_per_launch": kb[key],
+                          "achieved_gbs": kb[key] / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    dom = max(names, key=lambda k: prof[k])
+    ach = kb[dom] / (prof[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(names[dom]),
+                "kernel": names[dom], "avg_us": 1e3 * prof[dom], "bytes_per_launch": kb[dom],
+                "launches_timed": args.kernel_iters}
 
     cpu = None
     if not args.no_cpu:
         n, dt = cpu_baseline(grid, cfg, args.cpu_budget)
         cpu = {"value": T * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
-               "sample": f"{n} NS+SA residual evaluations of the same {grid.n_cells}-cell grid, CPU oracle, 1 thread, {dt:.1f} s"}
+               "sample": f"{n} NS+SA residual evaluations of the same {grid.n_cells}-cell grid, CPU oracle, "
+                         f"1 thread of {os.cpu_count()} host cores, {dt:.1f} s"}
 
     out = {
         "metric": "cell-residual evals/s + % HBM peak; time-to-1e-8 residual, learned vs plain",
@@ -130,9 +194,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "NACA0012 hybrid quad/tri grid, Roe + SA + Venkat, FP64 residual",
+                   "numbering": args.numbering,
                    "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": T,
                    "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
         "roofline": roofline,
+        "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels": kernels,
+        "copy_kernel_gbs": copy_gbs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

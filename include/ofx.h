@@ -25,6 +25,7 @@
 #ifndef OFX_H
 #define OFX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -191,10 +192,22 @@ int ofx_iterate(ofx_ctx* ctx, int first_iter, int n, ofx_norms* norms_out);
  * (codes/force/src/AeroForceTask.cpp:186-380). */
 int ofx_aero(ofx_ctx* ctx, double* out4);
 
-/* Average duration of the kernels launched by the last ofx_residual_ns call
- * on this context, measured with HIP events on the context's stream:
- * out[0] = whole NS residual (ms), out[1] = NS face-flux kernel (ms). */
-int ofx_profile_last(ofx_ctx* ctx, double* out2);
+/* Kernel timing (no counterpart in the reference; measurement only):
+ * launches one part of the residual `iters` times back to back on the
+ * context's stream between two HIP events, after one warm-up launch, and
+ * returns the average ms per launch.  which: 0 NS gradient + limiter min/max,
+ * 1 limiter, 2 NS face fluxes (MUSCL + Roe/SLAU2 + viscous), 3 NS ordered
+ * gather, 4 whole NS residual, 5 whole SA residual.  Every part is
+ * idempotent on the current state, so the fields are left as one
+ * ofx_residual_ns / ofx_residual_sa call leaves them (which 0 also leaves the
+ * limiter stale until the next limiter launch). */
+int ofx_time_kernel(ofx_ctx* ctx, int which, int iters, double* ms_out);
+
+/* Bandwidth reference (no counterpart in the reference): average duration
+ * (ms) of a plain device-to-device copy kernel moving `bytes` bytes
+ * (a multiple of 16), over `iters` launches after one warm-up launch.
+ * Allocates and frees its own two buffers. */
+int ofx_copy_bench(ofx_ctx* ctx, size_t bytes, int iters, double* ms_out);
 
 /* Synchronise the context's stream. */
 int ofx_sync(ofx_ctx* ctx);

@@ -37,6 +37,11 @@ int upload(ofx_ctx* c, void* dst, const void* src, size_t bytes) {
     return 0;
 }
 
+__global__ void k_fill_field(double* p, long n, double v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 static int alloc_fields(ofx_ctx* c) {
     const size_t T = c->m.T, F = c->m.F, C = c->m.C;
     DevFields& d = c->d;
@@ -64,6 +69,11 @@ static int alloc_fields(ofx_ctx* c) {
     OFX_TRY(A(&d.saface, 5 * F));
     OFX_TRY(A(&d.red, 16));
     OFX_TRY(A(&d.norms, 8 + 6 * 512));
+    OFX_TRY(alloc_bytes(c, (void**)&d.slots, 2 * 12 * NSLOT * sizeof(unsigned long long)));
+    // identity values in both parity buffers: min-reduced quantity 0 starts at LARGE, the others at +0
+    for (int p = 0; p < 2; ++p)
+        k_fill_field<<<1, 64, 0, c->stream>>>(reinterpret_cast<double*>(d.slots) + p * 12 * NSLOT, NSLOT, LARGE);
+    OFX_HIP(hipGetLastError());
     return 0;
 }
 
@@ -126,11 +136,6 @@ static void set_consts(ofx_ctx* c) {
     k.flux_fp32 = g.flux_fp32;
 }
 
-__global__ void k_fill_field(double* p, long n, double v) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 int init_freestream(ofx_ctx* c) {
     const long T = c->m.T;
     for (int e = 0; e < 5; ++e)
@@ -174,7 +179,6 @@ int ofx_ctx_create(ofx_ctx** out, int device) {
     c->device = device;
     OFX_HIP(hipSetDevice(device));
     OFX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto& e : c->ev) OFX_HIP(hipEventCreate(&e));
     *out = c;
     return 0;
 }
@@ -184,7 +188,6 @@ int ofx_ctx_destroy(ofx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) (void)hipFree(p);
-    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -298,6 +301,7 @@ int ofx_get_field(ofx_ctx* c, int field, double* host) {
     } else {
         double* p; size_t n;
         if (field_view(c, field, &p, &n)) { set_error("unknown field id"); return OFX_EINVAL; }
+        if (field == OFX_F_TIMESTEP || field == OFX_F_IMPSR) OFX_TRY(materialize_timestep(c));
         OFX_HIP(D2H(host, p, n));
     }
     OFX_HIP(hipStreamSynchronize(c->stream));
@@ -346,13 +350,11 @@ int ofx_iterate(ofx_ctx* c, int first_iter, int n, ofx_norms* norms_out) {
         OFX_TRY(ns_timestep(c, first_iter + s));
         OFX_TRY(ns_residual(c));
         OFX_TRY(ns_lusgs(c));
-        OFX_TRY(ns_update(c));
-        OFX_TRY(ns_boundary(c));
+        OFX_TRY(ns_update_fused(c));
         if (c->k.vismodel == 3) {
             OFX_TRY(sa_residual(c));
             OFX_TRY(sa_lusgs(c));
-            OFX_TRY(sa_update(c));
-            OFX_TRY(sa_boundary(c));
+            OFX_TRY(sa_update_fused(c));
         }
         if (norms_out) OFX_TRY(ofx_norms_get(c, norms_out + s));
     }
@@ -366,14 +368,73 @@ int ofx_aero(ofx_ctx* c, double* out4) {
     return aero(c, out4);
 }
 
-int ofx_profile_last(ofx_ctx* c, double* out2) {
+int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
     CHECK_READY(c);
-    OFX_HIP(hipStreamSynchronize(c->stream));
+    if (!ms_out || iters < 1 || which < 0 || which > 5) {
+        set_error("time_kernel: which must be 0..5 and iters >= 1");
+        return OFX_EINVAL;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    OFX_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); set_error("time_kernel: hipEventCreate"); return OFX_EHIP; }
+    int rc = 0;
+    auto once = [&]() -> int {
+        if (which <= 3) return ns_residual_parts(c, 1u << which);
+        if (which == 4) return ns_residual(c);
+        return sa_residual(c);
+    };
+    rc = once();  // warm-up launch
+    if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = OFX_EHIP;
+    for (int it = 0; it < iters && !rc; ++it) rc = once();
+    if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = OFX_EHIP;
     float ms = 0;
-    OFX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    out2[0] = 0.0;
-    out2[1] = ms;
-    return 0;
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = OFX_EHIP;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = OFX_EHIP;
+    if (!rc) *ms_out = ms / iters;
+    else if (rc == OFX_EHIP) set_error("time_kernel: HIP event error");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Bandwidth reference: one 16-byte load and store per thread.
+__global__ void __launch_bounds__(256) k_copy16(const double2* __restrict__ a, double2* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+int ofx_copy_bench(ofx_ctx* c, size_t bytes, int iters, double* ms_out) {
+    CHECK_CTX(c);
+    if (!ms_out || iters < 1 || bytes < 16 || bytes % 16 != 0 || bytes / 16 > ((size_t)1 << 36)) {
+        set_error("copy_bench: bytes must be a positive multiple of 16, iters >= 1");
+        return OFX_EINVAL;
+    }
+    OFX_HIP(hipSetDevice(c->device));
+    size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    OFX_HIP(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); set_error("copy_bench: out of device memory"); return OFX_ENOMEM; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    do {
+        if (hipMemsetAsync(a, 0, bytes, c->stream) != hipSuccess) { rc = OFX_EHIP; break; }
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { rc = OFX_EHIP; break; }
+        unsigned nb = (unsigned)((n + 255) / 256);
+        k_copy16<<<nb, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);  // warm-up
+        (void)hipEventRecord(e0, c->stream);
+        for (int it = 0; it < iters; ++it) k_copy16<<<nb, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = OFX_EHIP; break; }
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = OFX_EHIP; break; }
+        *ms_out = ms / iters;
+    } while (0);
+    if (rc) set_error("copy_bench: HIP error");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
 }
 
 int ofx_sync(ofx_ctx* c) {

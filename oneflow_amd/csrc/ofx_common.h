@@ -47,6 +47,7 @@ struct DevMesh {
     int *lc = nullptr, *rc = nullptr, *bct = nullptr;
     // cell -> face CSR in ascending face id; entry = f if the cell is lc[f], ~f if rc[f]
     int *c2fOff = nullptr, *c2f = nullptr;
+    int *c2n = nullptr;  // neighbour cell of each c2f entry (ghost C+f on boundary faces)
     // LU-SGS level schedule: cells grouped by dependency depth
     int *loOrder = nullptr, *loOff = nullptr, *upOrder = nullptr, *upOff = nullptr;
     // nodes (mesh build + wall distance only)
@@ -94,8 +95,9 @@ struct DevFields {
     double *invflux = nullptr;  // [5][F]
     double *visflux = nullptr;  // [5][F]
     double *saface = nullptr;   // [5][F]: radius, matrix_l, matrix_r, conv flux, diff flux
-    double *red = nullptr;      // reduction slots (uint64 bit patterns of non-negative doubles)
+    double *red = nullptr;      // reduction results (see RED_* in ofx_ns.hip)
     double *norms = nullptr;    // [6] residual norm accumulators
+    unsigned long long *slots = nullptr;  // [2 parities][RED_NQ][NSLOT] grid-reduction slots
 };
 
 }  // namespace ofx
@@ -110,8 +112,7 @@ struct ofx_ctx {
     ofx::HostSched sched;
     ofx::DevFields d;
     std::vector<void*> allocs;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double last_ms[2] = {0, 0};
+    int par_dt = 0, par_lim = 0, par_fix = 0;  // parity of the reduction buffers in use
 };
 
 #define OFX_HIP(call)                                                              \
@@ -147,6 +148,96 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a 
 __device__ __forceinline__ double dabs(double a) { return (a < 0) ? -a : a; }
 __device__ __forceinline__ double sqr3(double a, double b, double c) { return a * a + b * b + c * c; }
 
+// Grid-wide min/max without fences or extra launches.  Values are non-negative
+// doubles (maxima) or positive doubles (minima), whose IEEE bit patterns order
+// like the values, so integer atomics on the bits are exact.  Each block
+// reduces in LDS and issues one atomic per quantity into one of NSLOT slots
+// (spreading contention); the consumer reduces the NSLOT slots.  Two parity
+// buffers alternate between calls: a reducing kernel writes buffer `par` and
+// re-arms buffer 1-par (identity values) for the next call, whose consumer of
+// this call has completed by stream order.
+constexpr int NSLOT = 64;
+
+template <int NQ, bool IS_MAX>
+__device__ __forceinline__ void slot_reduce(const double* v, unsigned long long* buf, int par, int nq_total, int q0) {
+    __shared__ double sm[NQ][16];
+    const int nw = (int)(blockDim.x >> 6), w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    for (int q = 0; q < NQ; ++q) {
+        double x = v[q];
+        for (int o = 32; o > 0; o >>= 1) {
+            double y = __shfl_xor(x, o);
+            x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+        }
+        if (lane == 0) sm[q][w] = x;
+    }
+    __syncthreads();
+    int slot = (int)(blockIdx.x % NSLOT);
+    unsigned long long* mine = buf + (size_t)par * nq_total * NSLOT;
+    if (threadIdx.x < NQ) {
+        double x = sm[threadIdx.x][0];
+        for (int j = 1; j < nw; ++j) {
+            double y = sm[threadIdx.x][j];
+            x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+        }
+        unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+        unsigned long long* a = mine + (q0 + threadIdx.x) * NSLOT + slot;
+        if (IS_MAX) atomicMax(a, bits); else atomicMin(a, bits);
+    }
+    if (blockIdx.x == 0) {
+        unsigned long long* other = buf + (size_t)(1 - par) * nq_total * NSLOT;
+        unsigned long long ident = IS_MAX ? 0ull : (unsigned long long)__double_as_longlong(LARGE);
+        for (int t = threadIdx.x; t < NQ * NSLOT; t += blockDim.x) other[q0 * NSLOT + t] = ident;
+    }
+}
+
+// Consumer side: reduce the NSLOT slots of quantity q (whole block, result to all threads).
+template <bool IS_MAX>
+__device__ __forceinline__ double slot_result(const unsigned long long* buf, int par, int nq_total, int q) {
+    __shared__ double r[16];
+    const unsigned long long* mine = buf + (size_t)par * nq_total * NSLOT + q * NSLOT;
+    double x = IS_MAX ? 0.0 : LARGE;
+    if (threadIdx.x < NSLOT) x = __longlong_as_double((long long)mine[threadIdx.x]);
+    if (threadIdx.x < 64) {
+        for (int o = 32; o > 0; o >>= 1) {
+            double y = __shfl_xor(x, o);
+            x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+        }
+        if (threadIdx.x == 0) r[0] = x;
+    }
+    __syncthreads();
+    double out = r[0];
+    __syncthreads();
+    return out;
+}
+
+// Lanes-per-cell helpers: a cell's faces are spread over LPC consecutive lanes
+// of a wave (lane j holds face j of the cell's ascending list).  ordered_sum
+// rebuilds the serial accumulation sum = ((0 + c0) + c1) + ... in face order,
+// bit for bit, with in-wave shuffles.
+template <int LPC>
+__device__ __forceinline__ double ordered_sum(double c, int n) {
+    int base = (int)(threadIdx.x & 63) / LPC * LPC;
+    double sum = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < LPC; ++jj) {
+        double y = __shfl(c, base + jj, 64);
+        if (jj < n) sum = sum + y;
+    }
+    return sum;
+}
+template <int LPC>
+__device__ __forceinline__ double group_min(double x) {
+#pragma unroll
+    for (int o = LPC / 2; o > 0; o >>= 1) { double y = __shfl_xor(x, o, 64); x = (x < y) ? x : y; }
+    return x;
+}
+template <int LPC>
+__device__ __forceinline__ double group_max(double x) {
+#pragma unroll
+    for (int o = LPC / 2; o > 0; o >>= 1) { double y = __shfl_xor(x, o, 64); x = (x > y) ? x : y; }
+    return x;
+}
+
 int alloc_bytes(ofx_ctx* c, void** p, size_t bytes);
 int upload(ofx_ctx* c, void* dst, const void* src, size_t bytes);
 
@@ -157,6 +248,7 @@ int ns_boundary(ofx_ctx* c);
 int sa_boundary(ofx_ctx* c);
 int ns_timestep(ofx_ctx* c, int iter);
 int ns_residual(ofx_ctx* c);
+int ns_residual_parts(ofx_ctx* c, unsigned parts);
 int sa_residual(ofx_ctx* c);
 int ns_lusgs(ofx_ctx* c);
 int sa_lusgs(ofx_ctx* c);
@@ -164,6 +256,9 @@ int ns_update(ofx_ctx* c);
 int sa_update(ofx_ctx* c);
 int norms(ofx_ctx* c, ofx_norms* out);
 int fill_ghost_grad(ofx_ctx* c);
+int materialize_timestep(ofx_ctx* c);
+int ns_update_fused(ofx_ctx* c);
+int sa_update_fused(ofx_ctx* c);
 int aero(ofx_ctx* c, double* out4);
 int init_freestream(ofx_ctx* c);
 

@@ -343,6 +343,11 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     std::vector<int> pos(off.begin(), off.end() - 1);
     for (int f = 0; f < B; ++f) c2f[pos[lc[f]]++] = f;
     for (int f = B; f < F; ++f) { c2f[pos[lc[f]]++] = f; c2f[pos[rc[f]]++] = ~f; }
+    std::vector<int> c2n(off[T]);
+    for (int k = 0; k < off[T]; ++k) {
+        int e = c2f[k], f = e >= 0 ? e : ~e;
+        c2n[k] = (e >= 0) ? rc[f] : lc[f];
+    }
     int maxf = 0;
     for (int i = 0; i < C; ++i) maxf = std::max(maxf, off[i + 1] - off[i]);
     m.maxFacesPerCell = maxf;
@@ -407,6 +412,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(A((void**)&m.bct, sizeof(int) * std::max(1, B)));
     OFX_TRY(A((void**)&m.c2fOff, sizeof(int) * (T + 1)));
     OFX_TRY(A((void**)&m.c2f, sizeof(int) * std::max(1, off[T])));
+    OFX_TRY(A((void**)&m.c2n, sizeof(int) * std::max(1, off[T])));
     OFX_TRY(A((void**)&m.loOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.upOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.loOff, sizeof(int) * (nlo + 1)));
@@ -422,6 +428,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     if (B) OFX_TRY(upload(c, m.bct, bct, sizeof(int) * B));
     OFX_TRY(upload(c, m.c2fOff, off.data(), sizeof(int) * (T + 1)));
     OFX_TRY(upload(c, m.c2f, c2f.data(), sizeof(int) * off[T]));
+    OFX_TRY(upload(c, m.c2n, c2n.data(), sizeof(int) * off[T]));
     OFX_TRY(upload(c, m.loOrder, loOrd.data(), sizeof(int) * C));
     OFX_TRY(upload(c, m.upOrder, upOrd.data(), sizeof(int) * C));
     OFX_TRY(upload(c, m.loOff, c->sched.loOff.data(), sizeof(int) * (nlo + 1)));

@@ -26,18 +26,18 @@ __global__ void __launch_bounds__(TB) k_ns_cellstate(DevMesh m, DevFields d, Con
     d.visl[i] = sutherland(t, k.ts_suth);
 }
 
-// UNsBcSolver::CalcBc + NsBcSolver (codes/uns/src/UNsBcSolver.cpp:55-168,
-// codes/ns/src/NsBcSolver.cpp:51-410), then ghost gamma/T/viscosity.
-__global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= m.B) return;
+// UNsBcSolver::CalcBc + NsBcSolver for one boundary face given the owner's
+// primitive state s and temperature towner (codes/uns/src/UNsBcSolver.cpp:55-168,
+// codes/ns/src/NsBcSolver.cpp:51-410); writes the ghost state, then the ghost
+// gamma/T/viscosity (NsCalcGamaT / CalcLaminarViscosity on F_GHOST).
+__device__ __forceinline__ void ns_bc_face(const DevMesh& m, const DevFields& d, const Consts& k, int f,
+                                           const double* s, double towner) {
     const int T = m.T;
     int bt = m.bct[f];
-    int l = m.lc[f], r = m.C + f;
+    int r = m.C + f;
     double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f];
     const double vfn = 0.0;
-    double s[5], t[5];
-    for (int e = 0; e < 5; ++e) s[e] = d.q[e * T + l];
+    double t[5];
     double gama = 0.5 * (k.gama + k.gama);
     if (bt == BC_EXTRAPOLATION || bt == BC_OUTFLOW) {
         for (int e = 0; e < 5; ++e) t[e] = s[e];
@@ -50,9 +50,8 @@ __global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k) 
             // IsothermalVisWallBc (codes/ns/src/NsBcSolver.cpp:271-324)
             double tlim = 0.1 * k.twall;
             double rm = s[0], pm = s[4];
-            double temp = d.tempr[l];
             double pg1 = pm;
-            double tg1 = 2.0 * k.twall - temp;
+            double tg1 = 2.0 * k.twall - towner;
             if (tg1 < tlim) tg1 = tlim;
             double rw = pg1 / (k.statecoef * k.twall * 1.0);
             double rg1 = 2.0 * rw - rm;
@@ -101,6 +100,15 @@ __global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k) 
     d.visl[r] = sutherland(tt, k.ts_suth);
 }
 
+__global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.B) return;
+    int l = m.lc[f];
+    double s[5];
+    for (int e = 0; e < 5; ++e) s[e] = d.q[e * m.T + l];
+    ns_bc_face(m, d, k, f, s, d.tempr[l]);
+}
+
 int ns_boundary(ofx_ctx* c) {
     k_ns_cellstate<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k);
     OFX_HIP(hipGetLastError());
@@ -111,61 +119,17 @@ int ns_boundary(ofx_ctx* c) {
 
 // ============================================================ time step
 
-__device__ __forceinline__ void atomic_max_nonneg(double* slot, double v) {
-    atomicMax(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)__double_as_longlong(v));
-}
-
-__device__ __forceinline__ void atomic_min_pos(double* slot, double v) {
-    atomicMin(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)__double_as_longlong(v));
-}
-
-// Block reduction of a positive double to its minimum, then one atomic
-// (positive doubles order like their bit patterns, so the result is exact).
-__device__ __forceinline__ void block_min_pos(double v, double* slot) {
-    __shared__ double sm[TB / 64];
-    for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o));
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) sm[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = sm[0];
-        for (int j = 1; j < (int)(blockDim.x >> 6); ++j) x = dmin(x, sm[j]);
-        atomic_min_pos(slot, x);
-    }
-}
-
-// Block reduction of a non-negative double to its maximum, then one atomic.
-__device__ __forceinline__ void block_max_nonneg(double v, double* slot) {
-    __shared__ double sm[TB / 64];
-    for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o));
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) sm[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = sm[0];
-        for (int j = 1; j < (int)(blockDim.x >> 6); ++j) x = dmax(x, sm[j]);
-        atomic_max_nonneg(slot, x);
-    }
-}
-
-// Reduction slots (doubles updated with integer atomics on their bit patterns):
-//   red[0]      : min time step (min-reduced, starts at LARGE)
-//   red[1..5]   : -qmin per primitive (Venkat, max-reduced, starts at +0)
-//   red[6..10]  : qmax per primitive (max-reduced, starts at +0)
-//   red[14]     : count of cells needing SolutionFix (int)
-constexpr int RED_DTMIN = 0, RED_QMIN = 1, RED_QMAX = 6, RED_NFIX = 14;
-
-__global__ void k_fill(double* p, int n, double v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
+// Grid-reduction quantities (slot_reduce / slot_result):
+//   0      : min time step (min)
+//   1..5   : -qmin per primitive (Venkat, max)
+//   6..10  : qmax per primitive (Venkat, max)
+//   11     : 1 if some cell needs SolutionFix (max)
+constexpr int RQ_DTMIN = 0, RQ_QMIN = 1, RQ_QMAX = 6, RQ_FIX = 11, RQ_N = 12;
 
 // UTimeStep::CalcInvSpectrumField / CalcVisSpectrumField / CalcLocalTimeStep
 // (codes/uns/src/UTimeStep.cpp:190-416, codes/ns/src/TimeStep.cpp:43-133):
 // per cell, the face spectral radii are summed in ascending face order.
-__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k, int par) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     const int T = m.T;
     double dtv = LARGE;
@@ -220,24 +184,43 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
         d.timestep[i] = dt;
         dtv = dt;
     }
-    block_min_pos(dtv, d.red + RED_DTMIN);
+    slot_reduce<1, false>(&dtv, d.slots, par, RQ_N, RQ_DTMIN);
 }
 
-// UTimeStep::ModifyTimeStep clamp and UNsSpectrum::CalcImplicitSpectrum
-// (codes/uns/src/UTimeStep.cpp:368-381, codes/uns/src/UNsSpectrum.cpp:52-94)
-__global__ void __launch_bounds__(TB) k_ns_dtclamp(DevMesh m, DevFields d, Consts k, const double* mindt) {
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
+// Time step after UTimeStep::ModifyTimeStep (codes/uns/src/UTimeStep.cpp:368-381)
+// and the LU-SGS diagonal of UNsSpectrum::CalcImplicitSpectrum
+// (codes/uns/src/UNsSpectrum.cpp:52-94, sp2 = 1), evaluated where consumed.
+__device__ __forceinline__ double clamped_dt(const DevFields& d, const Consts& k, int i, double mindt) {
     double dt = d.timestep[i];
     if (k.max_time_ratio > 0) {
-        double mp = k.max_time_ratio * (*mindt);
+        double mp = k.max_time_ratio * mindt;
         dt = (dt < mp) ? dt : mp;
-        d.timestep[i] = dt;
     }
-    double imp = (1.0 / dt) * m.vol[i];
+    return dt;
+}
+__device__ __forceinline__ double ns_impsr(const DevMesh& m, const DevFields& d, const Consts& k, int i, double mindt) {
+    double imp = (1.0 / clamped_dt(d, k, i, mindt)) * m.vol[i];
     imp += d.invsr[i];
     imp += d.vissr[i];
+    return imp;
+}
+
+// Materialise the clamped time step and the LU-SGS diagonal as fields
+// (for download; the solver evaluates them where they are consumed).
+__global__ void __launch_bounds__(TB) k_ns_dtclamp(DevMesh m, DevFields d, Consts k, int par) {
+    double mindt = slot_result<false>(d.slots, par, RQ_N, RQ_DTMIN);
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double dt = clamped_dt(d, k, i, mindt);
+    double imp = ns_impsr(m, d, k, i, mindt);
+    d.timestep[i] = dt;
     d.impsr[i] = imp;
+}
+
+int materialize_timestep(ofx_ctx* c) {
+    k_ns_dtclamp<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_dt);
+    OFX_HIP(hipGetLastError());
+    return 0;
 }
 
 int ns_timestep(ofx_ctx* c, int iter) {
@@ -248,9 +231,8 @@ int ns_timestep(ofx_ctx* c, int iter) {
         cfl = g.cflst + delta * iter;
     }
     c->k.cfl = cfl;
-    k_fill<<<1, 64, 0, c->stream>>>(c->d.red + RED_DTMIN, 1, LARGE);
-    k_ns_timestep<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k);
-    k_ns_dtclamp<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->d.red + RED_DTMIN);
+    c->par_dt ^= 1;
+    k_ns_timestep<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_dt);
     OFX_HIP(hipGetLastError());
     return 0;
 }
@@ -261,103 +243,154 @@ int ns_timestep(ofx_ctx* c, int iter) {
 // (CalcGradGGCellWeight, codes/usolver/src/UGrad.cpp:88-149) as a per-cell
 // gather; optionally the neighbour min/max differences of the limiter
 // (Limiter::CalcMinMaxDiff, codes/usolver/src/ULimiter.cpp:597-644).
-template <bool MINMAX>
-__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k) {
+// The face loop is unrolled to MAXF: all loads of the cell's faces are issued
+// before the contributions are added in ascending face order.  DIM == 2 skips
+// the z components and w, which are identically zero in 2-D.
+template <int DIM, int MAXF, bool MINMAX>
+__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k, int par) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     const int T = m.T;
     double mn[5], mx[5];
     bool live = i < m.C;
     if (live) {
-        double acc[6][3];
-        for (int s = 0; s < 6; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.0;
+        int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+        int ent[MAXF], nb[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            ent[j] = (j < n) ? m.c2f[b + j] : 0;
+            nb[j] = (j < n) ? m.c2n[b + j] : i;
+        }
         double qi[5];
-        for (int e = 0; e < 5; ++e) { qi[e] = d.q[e * T + i]; mn[e] = qi[e]; mx[e] = qi[e]; }
-        for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-            int e = m.c2f[kk];
-            int f = e >= 0 ? e : ~e;
-            int l = m.lc[f], r = m.rc[f];
-            double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
-            double delt1 = sqrt(sqr3(xfc - m.xcc[l], yfc - m.ycc[l], zfc - m.zcc[l]));
-            double delt2 = sqrt(sqr3(xfc - m.xcc[r], yfc - m.ycc[r], zfc - m.zcc[r]));
-            double delta = 1.0 / (delt1 + delt2 + SMALL);
-            double cl = delt2 * delta, cr = delt1 * delta;
-            double area = m.area[f];
-            double fnxa = m.xfn[f] * area, fnya = m.yfn[f] * area, fnza = m.zfn[f] * area;
-            double ql[5], qr[5];
-            for (int s = 0; s < 5; ++s) { ql[s] = d.q[s * T + l]; qr[s] = d.q[s * T + r]; }
-            double tl = temperature(ql[0], ql[4], k.statecoef);
-            double tr = temperature(qr[0], qr[4], k.statecoef);
-            for (int s = 0; s < 6; ++s) {
-                double vl = s < 5 ? ql[s] : tl, vr = s < 5 ? qr[s] : tr;
-                double value = cl * vl + cr * vr;
-                if (e >= 0) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
-                else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
-            }
-            if (MINMAX && f >= m.B) {
-                const double* qn = (e >= 0) ? qr : ql;
-                for (int s = 0; s < 5; ++s) { mn[s] = dmin(mn[s], qn[s]); mx[s] = dmax(mx[s], qn[s]); }
+        for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
+        double xi = m.xcc[i], yi = m.ycc[i], zi = (DIM == 3) ? m.zcc[i] : 0.0;
+        double fx[MAXF], fy[MAXF], fz[MAXF], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF];
+        double cx[MAXF], cy[MAXF], cz[MAXF], qn[MAXF][5];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
+                fx[j] = m.xfc[f]; fy[j] = m.yfc[f]; fz[j] = (DIM == 3) ? m.zfc[f] : 0.0;
+                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
+                ar[j] = m.area[f];
+                cx[j] = m.xcc[c]; cy[j] = m.ycc[c]; cz[j] = (DIM == 3) ? m.zcc[c] : 0.0;
+                for (int e2 = 0; e2 < 5; ++e2) qn[j][e2] = (DIM == 2 && e2 == 3) ? 0.0 : d.q[e2 * T + c];
             }
         }
+        for (int e = 0; e < 5; ++e) { mn[e] = qi[e]; mx[e] = qi[e]; }
+        double ti = temperature(qi[0], qi[4], k.statecoef);
+        double acc[6][3];
+        for (int s = 0; s < 6; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                bool own = ent[j] >= 0;  // this cell is the face's left cell
+                double xl = own ? xi : cx[j], yl = own ? yi : cy[j], zl = own ? zi : cz[j];
+                double xr = own ? cx[j] : xi, yr = own ? cy[j] : yi, zr = own ? cz[j] : zi;
+                double delt1 = sqrt(sqr3(fx[j] - xl, fy[j] - yl, fz[j] - zl));
+                double delt2 = sqrt(sqr3(fx[j] - xr, fy[j] - yr, fz[j] - zr));
+                double delta = 1.0 / (delt1 + delt2 + SMALL);
+                double cl = delt2 * delta, cr = delt1 * delta;
+                double fnxa = nx[j] * ar[j], fnya = ny[j] * ar[j], fnza = nz[j] * ar[j];
+                double tn = temperature(qn[j][0], qn[j][4], k.statecoef);
+                for (int s = 0; s < 6; ++s) {
+                    double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
+                    double value = own ? (cl * vi + cr * vn) : (cl * vn + cr * vi);
+                    if (own) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
+                    else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
+                }
+                int f = own ? ent[j] : ~ent[j];
+                if (MINMAX && f >= m.B)
+                    for (int s = 0; s < 5; ++s) { mn[s] = dmin(mn[s], qn[j][s]); mx[s] = dmax(mx[s], qn[j][s]); }
+            }
+        }
+        // In 2-D the z components and w's gradient stay at the zeros the
+        // arrays were allocated with (w = 0 and z-extent 0 make them exactly 0).
         double ovol = 1.0 / m.vol[i];
-        for (int s = 0; s < 6; ++s)
-            for (int a = 0; a < 3; ++a) d.grad[(a * 6 + s) * T + i] = acc[s][a] * ovol;
+        for (int s = 0; s < 6; ++s) {
+            if (DIM == 2 && s == 3) continue;
+            for (int a = 0; a < DIM; ++a) d.grad[(a * 6 + s) * T + i] = acc[s][a] * ovol;
+        }
         if (MINMAX) {
             for (int s = 0; s < 5; ++s) {
                 mn[s] -= qi[s]; mx[s] -= qi[s];
+                if (DIM == 2 && s == 3) continue;
                 d.minmax[s * m.C + i] = mn[s];
                 d.minmax[(5 + s) * m.C + i] = mx[s];
             }
         }
     }
     if (MINMAX) {
-        for (int s = 0; s < 5; ++s) {
-            block_max_nonneg(live ? 0.0 - mn[s] : 0.0, d.red + RED_QMIN + s);
-            block_max_nonneg(live ? mx[s] : 0.0, d.red + RED_QMAX + s);
-        }
+        double v[10];
+        for (int s = 0; s < 5; ++s) { v[s] = live ? 0.0 - mn[s] : 0.0; v[5 + s] = live ? mx[s] : 0.0; }
+        slot_reduce<10, true>(v, d.slots, par, RQ_N, RQ_QMIN);
     }
 }
 
-// Limiter::CalcVencatLimiter / CalcBarthLimiter per cell: minimum over the
-// cell's faces (codes/usolver/src/ULimiter.cpp:464-595).
-template <int LIM>
-__global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Consts k) {
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
-    const int T = m.T;
-    double xi = m.xcc[i], yi = m.ycc[i], zi = m.zcc[i];
-    double gx[5], gy[5], gz[5], mn[5], mx[5], lim[5], eps1[5];
-    for (int s = 0; s < 5; ++s) {
-        gx[s] = d.grad[(0 * 6 + s) * T + i];
-        gy[s] = d.grad[(1 * 6 + s) * T + i];
-        gz[s] = d.grad[(2 * 6 + s) * T + i];
-        mn[s] = d.minmax[s * m.C + i];
-        mx[s] = d.minmax[(5 + s) * m.C + i];
-        lim[s] = 1.0;
-        double qmin = -d.red[RED_QMIN + s];
-        double qmax = d.red[RED_QMAX + s];
-        double eps = k.vencat_coef * dmax(qmax - qmin, 1.0);
-        eps1[s] = eps * eps + SMALL;
+// Venkatakrishnan / Barth limiter per cell: minimum over the cell's faces
+// (Limiter::CalcVencatLimiter / CalcBarthLimiter, codes/usolver/src/ULimiter.cpp:464-595).
+// LPC lanes per cell; the minimum is exact in any order.
+template <int DIM, int LPC, int LIM>
+__global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Consts k, int par) {
+    __shared__ double sext[10];
+    {
+        // the 10 limiter extrema: one wave per quantity group, one latency round
+        const unsigned long long* mine = d.slots + (size_t)par * RQ_N * NSLOT;
+        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        double x[3];
+        for (int r = 0; r < 3; ++r) {
+            int q = w + 4 * r;
+            x[r] = (q < 10) ? __longlong_as_double((long long)mine[(RQ_QMIN + q) * NSLOT + lane]) : 0.0;
+        }
+        for (int r = 0; r < 3; ++r) {
+            int q = w + 4 * r;
+            double y = x[r];
+            for (int o = 32; o > 0; o >>= 1) { double z = __shfl_xor(y, o); y = (y > z) ? y : z; }
+            if (q < 10 && lane == 0) sext[q] = y;
+        }
+        __syncthreads();
     }
-    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-        int e = m.c2f[kk];
+    const int T = m.T;
+    int gid = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int i = gid / LPC, j = gid % LPC;
+    bool live = i < m.C;
+    int b = 0, n = 0;
+    if (live) { b = m.c2fOff[i]; n = m.c2fOff[i + 1] - b; }
+    bool has = live && j < n;
+    double dx = 0, dy = 0, dz = 0, dot = 0;
+    if (has) {
+        int e = m.c2f[b + j];
         int f = e >= 0 ? e : ~e;
-        double dx = m.xfc[f] - xi, dy = m.yfc[f] - yi, dz = m.zfc[f] - zi;
-        if (LIM == 2) {
-            for (int s = 0; s < 5; ++s) {
-                double dq = gx[s] * dx + gy[s] * dy + gz[s] * dz;
-                lim[s] = dmin(lim[s], vencat_fn(dq, mn[s], mx[s], eps1[s]));
-            }
-        } else {
+        dx = m.xfc[f] - m.xcc[i];
+        dy = m.yfc[f] - m.ycc[i];
+        dz = (DIM == 3) ? m.zfc[f] - m.zcc[i] : 0.0;
+        if (LIM == 1) {
             double ds = sqrt(sqr3(dx, dy, dz));
-            double dot = (m.xfn[f] * dx + m.yfn[f] * dy + m.zfn[f] * dz) / (ds + SMALL);
+            double nz = (DIM == 3) ? m.zfn[f] : 0.0;
+            dot = (m.xfn[f] * dx + m.yfn[f] * dy + nz * dz) / (ds + SMALL);
             if (e < 0) dot = -dot;
-            for (int s = 0; s < 5; ++s) {
-                double dq = gx[s] * dx + gy[s] * dy + gz[s] * dz;
-                lim[s] = dmin(lim[s], barth_fn(dq, mn[s], mx[s], dot));
-            }
         }
     }
-    for (int s = 0; s < 5; ++s) d.limiter[s * T + i] = lim[s];
+    for (int s = 0; s < 5; ++s) {
+        double lim = 1.0;
+        bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
+        if (has && !skip) {
+            double gx = d.grad[(0 * 6 + s) * T + i];
+            double gy = d.grad[(1 * 6 + s) * T + i];
+            double gz = (DIM == 3) ? d.grad[(2 * 6 + s) * T + i] : 0.0;
+            double mn = d.minmax[s * m.C + i];
+            double mx = d.minmax[(5 + s) * m.C + i];
+            double dq = gx * dx + gy * dy + gz * dz;
+            if (LIM == 2) {
+                double eps = k.vencat_coef * dmax(sext[5 + s] - (-sext[s]), 1.0);
+                double eps1 = eps * eps + SMALL;
+                lim = vencat_fn(dq, mn, mx, eps1);
+            } else {
+                lim = barth_fn(dq, mn, mx, dot);
+            }
+        }
+        lim = group_min<LPC>(lim);
+        if (live && j == 0) d.limiter[s * T + i] = dmin(1.0, lim);
+    }
 }
 
 // Per-face inviscid (MUSCL + Roe/SLAU2) and viscous fluxes
@@ -366,7 +399,7 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
 //  codes/uns/src/UNsLimiter.cpp:67-101; UNsVisFlux::CalcVisFlux,
 //  codes/uns/src/UNsVisFlux.cpp:127-440).  LIM: -1 first order, 0 unlimited,
 // 1 limiter array.  Ghost gradients equal their owner's (Grad::CalcGrad tail).
-template <int LIM, bool SLAU2, bool VISC>
+template <int DIM, int LIM, bool SLAU2, bool VISC>
 __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
@@ -374,9 +407,12 @@ __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k
     int l = m.lc[f], r = m.rc[f];
     bool bnd = f < m.B;
     int rg = bnd ? l : r;  // cell holding r's gradient
-    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], area = m.area[f];
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = (DIM == 3) ? m.zfn[f] : 0.0, area = m.area[f];
     double ql[5], qr[5], p1[5], p2[5];
-    for (int e = 0; e < 5; ++e) { ql[e] = d.q[e * T + l]; qr[e] = d.q[e * T + r]; }
+    for (int e = 0; e < 5; ++e) {
+        ql[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + l];
+        qr[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + r];
+    }
     if (bnd) {
         for (int e = 0; e < 5; ++e) { double tmp = 0.5 * (ql[e] + qr[e]); p1[e] = tmp; p2[e] = tmp; }
         if (m.bct[f] == BC_SOLID_SURFACE && k.vismodel != 0) {
@@ -390,12 +426,20 @@ __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k
     } else if (LIM == -1) {
         for (int e = 0; e < 5; ++e) { p1[e] = ql[e]; p2[e] = qr[e]; }
     } else {
-        double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
-        double dxl = xfc - m.xcc[l], dyl = yfc - m.ycc[l], dzl = zfc - m.zcc[l];
-        double dxr = xfc - m.xcc[r], dyr = yfc - m.ycc[r], dzr = zfc - m.zcc[r];
+        double xfc = m.xfc[f], yfc = m.yfc[f], zfc = (DIM == 3) ? m.zfc[f] : 0.0;
+        double zl = (DIM == 3) ? m.zcc[l] : 0.0, zr = (DIM == 3) ? m.zcc[r] : 0.0;
+        double dxl = xfc - m.xcc[l], dyl = yfc - m.ycc[l], dzl = zfc - zl;
+        double dxr = xfc - m.xcc[r], dyr = yfc - m.ycc[r], dzr = zfc - zr;
         for (int e = 0; e < 5; ++e) {
-            double gl = d.grad[(0 * 6 + e) * T + l] * dxl + d.grad[(1 * 6 + e) * T + l] * dyl + d.grad[(2 * 6 + e) * T + l] * dzl;
-            double gr = d.grad[(0 * 6 + e) * T + r] * dxr + d.grad[(1 * 6 + e) * T + r] * dyr + d.grad[(2 * 6 + e) * T + r] * dzr;
+            if (DIM == 2 && e == 3) {
+                // w and its gradient are zero in 2-D: the reconstruction leaves w = 0
+                p1[e] = ql[e] + 0.0; p2[e] = qr[e] + 0.0;
+                continue;
+            }
+            double gzl = (DIM == 3) ? d.grad[(2 * 6 + e) * T + l] : 0.0;
+            double gzr = (DIM == 3) ? d.grad[(2 * 6 + e) * T + r] : 0.0;
+            double gl = d.grad[(0 * 6 + e) * T + l] * dxl + d.grad[(1 * 6 + e) * T + l] * dyl + gzl * dzl;
+            double gr = d.grad[(0 * 6 + e) * T + r] * dxr + d.grad[(1 * 6 + e) * T + r] * dyr + gzr * dzr;
             if (LIM == 0) {
                 p1[e] = ql[e] + gl;
                 p2[e] = qr[e] + gr;
@@ -421,8 +465,12 @@ __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k
     double gq[6][3], qfv[6], dqdn[6];
     for (int s = 1; s < 6; ++s) {
         if (s == 4) continue;  // pressure does not enter the viscous flux
-        double g1[3] = {d.grad[(0 * 6 + s) * T + l], d.grad[(1 * 6 + s) * T + l], d.grad[(2 * 6 + s) * T + l]};
-        double g2[3] = {d.grad[(0 * 6 + s) * T + rg], d.grad[(1 * 6 + s) * T + rg], d.grad[(2 * 6 + s) * T + rg]};
+        if (DIM == 2 && s == 3) {
+            gq[3][0] = gq[3][1] = gq[3][2] = 0.0; qfv[3] = 0.0; dqdn[3] = 0.0;
+            continue;
+        }
+        double g1[3] = {d.grad[(0 * 6 + s) * T + l], d.grad[(1 * 6 + s) * T + l], (DIM == 3) ? d.grad[(2 * 6 + s) * T + l] : 0.0};
+        double g2[3] = {d.grad[(0 * 6 + s) * T + rg], d.grad[(1 * 6 + s) * T + rg], (DIM == 3) ? d.grad[(2 * 6 + s) * T + rg] : 0.0};
         double v1 = s < 5 ? ql[s] : tl, v2 = s < 5 ? qr[s] : tr;
         vis_grad(k.ivischeme, G, v1, v2, g1, g2, qfv[s], gq[s], dqdn[s]);
     }
@@ -456,65 +504,98 @@ __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k
 }
 
 // Residual gather: res = -sum(inviscid) then -sum(viscous), each over the
-// cell's faces in ascending id, sign by side (AddF2CField called twice).
-template <bool VISC>
+// cell's faces in ascending id, sign by side (AddF2CField called twice);
+// LPC lanes per cell, ordered shuffle sums.
+template <int LPC, bool VISC>
 __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
     const int T = m.T, F = m.F;
-    double r[5] = {0, 0, 0, 0, 0};
-    int b = m.c2fOff[i], n = m.c2fOff[i + 1];
-    for (int kk = b; kk < n; ++kk) {
-        int e = m.c2f[kk];
+    int gid = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int i = gid / LPC, j = gid % LPC;
+    bool live = i < m.C;
+    int b = 0, n = 0;
+    if (live) { b = m.c2fOff[i]; n = m.c2fOff[i + 1] - b; }
+    bool has = live && j < n;
+    double fi[5] = {0, 0, 0, 0, 0}, fv[5] = {0, 0, 0, 0, 0};
+    if (has) {
+        int e = m.c2f[b + j];
         int f = e >= 0 ? e : ~e;
-        if (e >= 0) for (int s = 0; s < 5; ++s) r[s] -= d.invflux[s * F + f];
-        else for (int s = 0; s < 5; ++s) r[s] += d.invflux[s * F + f];
+        for (int s = 0; s < 5; ++s) { double x = d.invflux[s * F + f]; fi[s] = (e >= 0) ? -x : x; }
+        if (VISC) for (int s = 0; s < 5; ++s) { double x = d.visflux[s * F + f]; fv[s] = (e >= 0) ? -x : x; }
     }
-    if (VISC) {
-        for (int kk = b; kk < n; ++kk) {
-            int e = m.c2f[kk];
-            int f = e >= 0 ? e : ~e;
-            if (e >= 0) for (int s = 0; s < 5; ++s) r[s] -= d.visflux[s * F + f];
-            else for (int s = 0; s < 5; ++s) r[s] += d.visflux[s * F + f];
+    for (int s = 0; s < 5; ++s) {
+        // continue the inviscid chain with the viscous terms: ((0 + i0) + i1 ...) + v0 + v1 ...
+        int base = (int)(threadIdx.x & 63) / LPC * LPC;
+        double sum = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < LPC; ++jj) { double y = __shfl(fi[s], base + jj, 64); if (jj < n) sum = sum + y; }
+        if (VISC) {
+#pragma unroll
+            for (int jj = 0; jj < LPC; ++jj) { double y = __shfl(fv[s], base + jj, 64); if (jj < n) sum = sum + y; }
         }
+        if (live && j == 0) d.res[s * T + i] = sum;
     }
-    for (int s = 0; s < 5; ++s) d.res[s * T + i] = r[s];
 }
 
-template <int LIM, bool SLAU2>
+template <int DIM, int LIM, bool SLAU2>
 static void launch_face(ofx_ctx* c, bool visc) {
     int nb = nblk(c->m.F);
-    if (visc) k_ns_face<LIM, SLAU2, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
-    else k_ns_face<LIM, SLAU2, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+    if (visc) k_ns_face<DIM, LIM, SLAU2, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+    else k_ns_face<DIM, LIM, SLAU2, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
 }
 
-int ns_residual(ofx_ctx* c) {
+template <int DIM>
+static void launch_face_dim(ofx_ctx* c, bool visc) {
     const Consts& k = c->k;
-    const int C = c->m.C;
+    bool slau2 = k.ischeme == 12;
+    if (k.ilim == -1) { if (slau2) launch_face<DIM, -1, true>(c, visc); else launch_face<DIM, -1, false>(c, visc); }
+    else if (k.ilim == 0) { if (slau2) launch_face<DIM, 0, true>(c, visc); else launch_face<DIM, 0, false>(c, visc); }
+    else { if (slau2) launch_face<DIM, 1, true>(c, visc); else launch_face<DIM, 1, false>(c, visc); }
+}
+
+// NS residual kernels; `parts` selects them (bit 0 gradient + min/max, bit 1
+// limiter, bit 2 face fluxes, bit 3 gather) so each can be timed alone.
+template <int DIM, int MAXF>
+static void launch_cells(ofx_ctx* c, unsigned parts) {
+    const Consts& k = c->k;
+    const int C1 = c->m.C;         // threads: one per cell
+    const int C = c->m.C * MAXF;   // threads: MAXF lanes per cell
     bool visc = k.vismodel != 0;
     bool lim = k.ilim == 1 || k.ilim == 2;
-    if (lim) {
-        k_fill<<<1, 64, 0, c->stream>>>(c->d.red + RED_QMIN, 10, 0.0);
-        k_ns_grad<true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
-    } else {
-        k_ns_grad<false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
+    if (parts & 1u) {
+        if (lim) {
+            c->par_lim ^= 1;
+            k_ns_grad<DIM, MAXF, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        } else {
+            k_ns_grad<DIM, MAXF, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
+        }
     }
-    OFX_HIP(hipGetLastError());
-    if (k.ilim == 2) k_ns_limiter<2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
-    if (k.ilim == 1) k_ns_limiter<1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k);
-    OFX_HIP(hipGetLastError());
-    OFX_HIP(hipEventRecord(c->ev[0], c->stream));
-    bool slau2 = k.ischeme == 12;
-    if (k.ilim == -1) { if (slau2) launch_face<-1, true>(c, visc); else launch_face<-1, false>(c, visc); }
-    else if (k.ilim == 0) { if (slau2) launch_face<0, true>(c, visc); else launch_face<0, false>(c, visc); }
-    else { if (slau2) launch_face<1, true>(c, visc); else launch_face<1, false>(c, visc); }
-    OFX_HIP(hipGetLastError());
-    OFX_HIP(hipEventRecord(c->ev[1], c->stream));
-    if (visc) k_ns_gather<true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
-    else k_ns_gather<false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+    if ((parts & 2u) && lim) {
+        if (k.ilim == 2) k_ns_limiter<DIM, MAXF, 2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        else k_ns_limiter<DIM, MAXF, 1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+    }
+    if (parts & 4u) launch_face_dim<DIM>(c, visc);
+    if (parts & 8u) {
+        if (visc) k_ns_gather<MAXF, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+        else k_ns_gather<MAXF, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+    }
+}
+
+int ns_residual_parts(ofx_ctx* c, unsigned parts) {
+    int mf = c->m.maxFacesPerCell;
+    if (c->m.dim == 2) {
+        if (mf <= 4) launch_cells<2, 4>(c, parts);
+        else if (mf <= 8) launch_cells<2, 8>(c, parts);
+        else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
+    } else {
+        if (mf <= 8) launch_cells<3, 8>(c, parts);
+        else if (mf <= 16) launch_cells<3, 16>(c, parts);
+        else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
+    }
     OFX_HIP(hipGetLastError());
     return 0;
 }
+
+int ns_residual(ofx_ctx* c) { return ns_residual_parts(c, 0xFu); }
 
 // Ghost-cell gradients / limiter for field download (Grad::CalcGrad tail,
 // codes/usolver/src/UGrad.cpp:142-148)
@@ -616,7 +697,7 @@ __device__ __forceinline__ void ns_lusgs_face(const DevMesh& m, const DevFields&
 // skipped in both sweeps: their ghost increment is zero, so the reference's
 // upper-sweep contribution through them is exactly +0.
 template <bool UPPER>
-__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k, int i) {
+__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k, int i, double mindt) {
     const int T = m.T;
     double rhs0[5] = {0, 0, 0, 0, 0};
     for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
@@ -627,7 +708,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
         if (UPPER ? (j < i) : (j > i)) continue;
         ns_lusgs_face(m, d, k, i, f, rhs0);
     }
-    double rad = d.impsr[i];
+    double rad = ns_impsr(m, d, k, i, mindt);
     if (!UPPER) {
         for (int e = 0; e < 5; ++e) { double t = d.res[e * T + i] - rhs0[e]; t /= rad; d.dq[e * T + i] = t; }
     } else {
@@ -638,31 +719,33 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
 // Whole sweep in one workgroup: levels in order, a workgroup barrier between
 // levels (every cell of a level depends only on earlier levels).
 template <bool UPPER>
-__global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k, int par) {
+    double mindt = slot_result<false>(d.slots, par, RQ_N, RQ_DTMIN);
     const int* off = UPPER ? m.upOff : m.loOff;
     const int* ord = UPPER ? m.upOrder : m.loOrder;
     int nl = UPPER ? m.nLevUp : m.nLevLo;
     for (int L = 0; L < nl; ++L) {
         for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
-            ns_lusgs_cell<UPPER>(m, d, k, ord[idx]);
+            ns_lusgs_cell<UPPER>(m, d, k, ord[idx], mindt);
         __syncthreads();
     }
 }
 
 // One level per launch (wide levels on large meshes).
 template <bool UPPER>
-__global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e) {
+__global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int par) {
+    double mindt = slot_result<false>(d.slots, par, RQ_N, RQ_DTMIN);
     int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= e) return;
-    ns_lusgs_cell<UPPER>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx]);
+    ns_lusgs_cell<UPPER>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx], mindt);
 }
 
-constexpr int WIDE_LEVEL = 8192;
-
-static int max_width(const std::vector<int>& off) {
-    int w = 0;
-    for (size_t l = 0; l + 1 < off.size(); ++l) w = std::max(w, off[l + 1] - off[l]);
-    return w;
+// Launch policy: few wide levels (multicolour numbering) -> one grid-wide
+// launch per level; many narrow levels (natural numbering) -> one workgroup
+// walking all levels.
+bool use_level_launches(const std::vector<int>& off) {
+    int nl = (int)off.size() - 1;
+    return nl <= 64;
 }
 
 int ns_lusgs(ofx_ctx* c) {
@@ -670,14 +753,15 @@ int ns_lusgs(ofx_ctx* c) {
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
         const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
-        if (max_width(off) <= WIDE_LEVEL) {
-            if (up) k_ns_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d, c->k);
-            else k_ns_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d, c->k);
+        if (!use_level_launches(off)) {
+            if (up) k_ns_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            else k_ns_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
         } else {
             for (size_t L = 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
-                if (up) k_ns_sweep_level<true><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e);
-                else k_ns_sweep_level<false><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e);
+                if (e <= b) continue;
+                if (up) k_ns_sweep_level<true><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
+                else k_ns_sweep_level<false><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
             }
         }
         OFX_HIP(hipGetLastError());
@@ -712,56 +796,97 @@ __device__ __forceinline__ bool ns_update_cell(const DevFields& d, const Consts&
     return true;
 }
 
-// Pass 1: count cells whose weak fix fails (the neighbour average of
-// SolutionFix reads neighbours in the serial loop's half-updated state).
-__global__ void __launch_bounds__(TB) k_ns_update_check(DevMesh m, DevFields d, Consts k, int* nfix) {
+// Pass 1: does any cell need SolutionFix?  (The neighbour average of
+// SolutionFix reads neighbours in the serial loop's half-updated state, so
+// such an update runs serially in cell order.)
+__global__ void __launch_bounds__(TB) k_ns_update_check(DevMesh m, DevFields d, Consts k, int par) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
-    double prim[5];
-    if (!ns_update_cell(d, k, m.T, i, prim)) atomicAdd(nfix, 1);
+    double flag = 0.0;
+    if (i < m.C) {
+        double prim[5];
+        if (!ns_update_cell(d, k, m.T, i, prim)) flag = 1.0;
+    }
+    slot_reduce<1, true>(&flag, d.slots, par, RQ_N, RQ_FIX);
 }
 
-// Pass 2: the common case updates every cell in parallel; if any cell needs
-// SolutionFix the whole update runs in the reference's serial cell order.
-__global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, Consts k, const int* nfix) {
+__device__ void ns_solution_fix(const DevMesh& m, const DevFields& d, int i, double* prim) {
     const int T = m.T;
-    if (*nfix == 0) {
+    for (int e = 0; e < 5; ++e) prim[e] = 0.0;
+    double sumV = 0.0;
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int e = m.c2f[kk];
+        int f = e >= 0 ? e : ~e;
+        int nei = e >= 0 ? m.rc[f] : m.lc[f];
+        sumV += 1.0;
+        for (int s = 0; s < 5; ++s) {
+            double v = d.q[s * T + nei];
+            if (s == 0 || s == 4) v = dabs(v);
+            prim[s] += v * 1.0;
+        }
+    }
+    double rVol = 1.0 / sumV;
+    for (int s = 0; s < 5; ++s) prim[s] *= rVol;
+}
+
+// Pass 2: UPDATE_FLOWFIELD; with BC also the following CALC_BOUNDARY
+// (NsCalcBoundary): T / viscosity of the cell and its boundary faces' ghosts.
+template <bool BC>
+__global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, Consts k, int par) {
+    const int T = m.T;
+    double anyfix = slot_result<true>(d.slots, par, RQ_N, RQ_FIX);
+    if (anyfix == 0.0) {
         int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
         if (i >= m.C) return;
         double prim[5];
         ns_update_cell(d, k, T, i, prim);
         for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
+        if (BC) {
+            double t = temperature(prim[0], prim[4], k.statecoef);
+            d.tempr[i] = t;
+            d.visl[i] = sutherland(t, k.ts_suth);
+            for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+                int f = m.c2f[kk];
+                // boundary faces (owner side, f < B) come first; entries of faces
+                // where the cell is the right cell are stored as ~f < 0
+                if (f < 0 || f >= m.B) break;
+                ns_bc_face(m, d, k, f, prim, t);
+            }
+        }
         return;
     }
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int i = 0; i < m.C; ++i) {
         double prim[5];
-        if (!ns_update_cell(d, k, T, i, prim)) {
-            for (int e = 0; e < 5; ++e) prim[e] = 0.0;
-            double sumV = 0.0;
-            for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-                int e = m.c2f[kk];
-                int f = e >= 0 ? e : ~e;
-                int nei = e >= 0 ? m.rc[f] : m.lc[f];
-                sumV += 1.0;
-                for (int s = 0; s < 5; ++s) {
-                    double v = d.q[s * T + nei];
-                    if (s == 0 || s == 4) v = dabs(v);
-                    prim[s] += v * 1.0;
-                }
-            }
-            double rVol = 1.0 / sumV;
-            for (int s = 0; s < 5; ++s) prim[s] *= rVol;
-        }
+        if (!ns_update_cell(d, k, T, i, prim)) ns_solution_fix(m, d, i, prim);
         for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
+    }
+    if (BC) {
+        for (int i = 0; i < m.C; ++i) {
+            double t = temperature(d.q[i], d.q[4 * T + i], k.statecoef);
+            d.tempr[i] = t;
+            d.visl[i] = sutherland(t, k.ts_suth);
+        }
+        for (int f = 0; f < m.B; ++f) {
+            int l = m.lc[f];
+            double s[5];
+            for (int e = 0; e < 5; ++e) s[e] = d.q[e * T + l];
+            ns_bc_face(m, d, k, f, s, d.tempr[l]);
+        }
     }
 }
 
 int ns_update(ofx_ctx* c) {
-    int* nfix = reinterpret_cast<int*>(c->d.red + RED_NFIX);
-    k_fill<<<1, 64, 0, c->stream>>>(c->d.red + RED_NFIX, 1, 0.0);
-    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, nfix);
-    k_ns_update_apply<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, nfix);
+    c->par_fix ^= 1;
+    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
+    k_ns_update_apply<false><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+int ns_update_fused(ofx_ctx* c) {
+    c->par_fix ^= 1;
+    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
+    k_ns_update_apply<true><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
     OFX_HIP(hipGetLastError());
     return 0;
 }

@@ -5,6 +5,8 @@
 
 namespace ofx {
 
+bool use_level_launches(const std::vector<int>& off);
+
 constexpr int TBS = 256;
 static inline int nblks(int n) { return (n + TBS - 1) / TBS; }
 
@@ -28,39 +30,39 @@ __global__ void __launch_bounds__(TBS) k_sa_vist(DevMesh m, DevFields d, Consts 
     d.vist[i] = sa_vist_cell(d, k, m.T, src);
 }
 
-// UTurbBcSolver + TurbBcSolver (codes/uturb/src/UTurbBcSolver.cpp:59-160,
+// UTurbBcSolver + TurbBcSolver for one boundary face: ghost nu_tilde from the
+// owner's nu_tilde s and NS state (codes/uturb/src/UTurbBcSolver.cpp:59-160,
 // codes/turb/src/TurbBcSolver.cpp:142-253)
+__device__ __forceinline__ double sa_bc_value(const DevMesh& m, const DevFields& d, const Consts& k, int f, int l,
+                                              double s) {
+    const int T = m.T;
+    int bt = m.bct[f];
+    if (bt == BC_INFLOW) return k.sa_inflow;
+    if (bt == BC_SOLID_SURFACE) return -s;
+    if (bt != BC_FARFIELD) return s;  // extrapolation, outflow, symmetry
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f];
+    const double vfn = 0.0;
+    double rin = d.q[l], uin = d.q[T + l], vin = d.q[2 * T + l], win = d.q[3 * T + l], pin = d.q[4 * T + l];
+    double rref = k.inflow[0], uref = k.inflow[1], vref = k.inflow[2], wref = k.inflow[3], pref = k.inflow[4];
+    double vnref = xfn * uref + yfn * vref + zfn * wref - vfn;
+    double vnin = xfn * uin + yfn * vin + zfn * win - vfn;
+    double gama = k.gama;
+    double cref = sqrt(dabs(k.gama * pref / rref));
+    double cin = sqrt(dabs(gama * pin / rin));
+    double velin = sqrt(sqr3(uin, vin, win));
+    if (velin > cin) return (vnin >= 0.0) ? s : k.sa_inflow;
+    double gamm1 = gama - 1.0;
+    double riemp = vnin + 2.0 * cin / gamm1;
+    double riemm = vnref - 2.0 * cref / gamm1;
+    double vnb = 0.5 * (riemp + riemm);
+    return (vnb >= 0.0) ? s : k.sa_inflow;
+}
+
 __global__ void __launch_bounds__(TBS) k_sa_bc(DevMesh m, DevFields d, Consts k) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.B) return;
-    const int T = m.T;
-    int bt = m.bct[f];
-    int l = m.lc[f], r = m.C + f;
-    double s = d.turbq[l], t;
-    if (bt == BC_INFLOW) t = k.sa_inflow;
-    else if (bt == BC_SOLID_SURFACE) t = -s;
-    else if (bt == BC_FARFIELD) {
-        double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f];
-        const double vfn = 0.0;
-        double rin = d.q[l], uin = d.q[T + l], vin = d.q[2 * T + l], win = d.q[3 * T + l], pin = d.q[4 * T + l];
-        double rref = k.inflow[0], uref = k.inflow[1], vref = k.inflow[2], wref = k.inflow[3], pref = k.inflow[4];
-        double vnref = xfn * uref + yfn * vref + zfn * wref - vfn;
-        double vnin = xfn * uin + yfn * vin + zfn * win - vfn;
-        double gama = k.gama;
-        double cref = sqrt(dabs(k.gama * pref / rref));
-        double cin = sqrt(dabs(gama * pin / rin));
-        double velin = sqrt(sqr3(uin, vin, win));
-        if (velin > cin) {
-            t = (vnin >= 0.0) ? s : k.sa_inflow;
-        } else {
-            double gamm1 = gama - 1.0;
-            double riemp = vnin + 2.0 * cin / gamm1;
-            double riemm = vnref - 2.0 * cref / gamm1;
-            double vnb = 0.5 * (riemp + riemm);
-            t = (vnb >= 0.0) ? s : k.sa_inflow;
-        }
-    } else t = s;  // extrapolation, outflow, symmetry
-    d.turbq[r] = t;
+    int l = m.lc[f];
+    d.turbq[m.C + f] = sa_bc_value(m, d, k, f, l, d.turbq[l]);
 }
 
 int sa_boundary(ofx_ctx* c) {
@@ -71,26 +73,57 @@ int sa_boundary(ofx_ctx* c) {
     return 0;
 }
 
-// Green-Gauss gradient of nu_tilde (UTurbGrad, codes/uturb/src/UTurbGrad.cpp:51-74)
-__global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d) {
+// TurbCalcBc (start of TurbCalcRHS, codes/turb/src/TurbRhs.cpp:57-62) fused
+// with the Green-Gauss gradient of nu_tilde (UTurbGrad,
+// codes/uturb/src/UTurbGrad.cpp:51-74): each cell sets the ghosts of its own
+// boundary faces (they depend only on the owner) and uses them.
+template <int DIM, int MAXF>
+__global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts k) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const int T = m.T;
+    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    int ent[MAXF], nb[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        ent[j] = (j < n) ? m.c2f[b + j] : 0;
+        nb[j] = (j < n) ? m.c2n[b + j] : i;
+    }
+    double nui = d.turbq[i];
+    double xi = m.xcc[i], yi = m.ycc[i], zi = (DIM == 3) ? m.zcc[i] : 0.0;
+    double fx[MAXF], fy[MAXF], fz[MAXF], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF], cx[MAXF], cy[MAXF], cz[MAXF], vn[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (j < n) {
+            int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
+            fx[j] = m.xfc[f]; fy[j] = m.yfc[f]; fz[j] = (DIM == 3) ? m.zfc[f] : 0.0;
+            nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
+            ar[j] = m.area[f];
+            cx[j] = m.xcc[c]; cy[j] = m.ycc[c]; cz[j] = (DIM == 3) ? m.zcc[c] : 0.0;
+            if (f < m.B) {
+                vn[j] = sa_bc_value(m, d, k, f, i, nui);
+                d.turbq[c] = vn[j];
+            } else {
+                vn[j] = d.turbq[c];
+            }
+        }
+    }
     double ax = 0, ay = 0, az = 0;
-    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-        int e = m.c2f[kk];
-        int f = e >= 0 ? e : ~e;
-        int l = m.lc[f], r = m.rc[f];
-        double xfc = m.xfc[f], yfc = m.yfc[f], zfc = m.zfc[f];
-        double delt1 = sqrt(sqr3(xfc - m.xcc[l], yfc - m.ycc[l], zfc - m.zcc[l]));
-        double delt2 = sqrt(sqr3(xfc - m.xcc[r], yfc - m.ycc[r], zfc - m.zcc[r]));
-        double delta = 1.0 / (delt1 + delt2 + SMALL);
-        double cl = delt2 * delta, cr = delt1 * delta;
-        double value = cl * d.turbq[l] + cr * d.turbq[r];
-        double area = m.area[f];
-        double fnxa = m.xfn[f] * area, fnya = m.yfn[f] * area, fnza = m.zfn[f] * area;
-        if (e >= 0) { ax += fnxa * value; ay += fnya * value; az += fnza * value; }
-        else { ax -= fnxa * value; ay -= fnya * value; az -= fnza * value; }
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (j < n) {
+            bool own = ent[j] >= 0;
+            double xl = own ? xi : cx[j], yl = own ? yi : cy[j], zl = own ? zi : cz[j];
+            double xr = own ? cx[j] : xi, yr = own ? cy[j] : yi, zr = own ? cz[j] : zi;
+            double delt1 = sqrt(sqr3(fx[j] - xl, fy[j] - yl, fz[j] - zl));
+            double delt2 = sqrt(sqr3(fx[j] - xr, fy[j] - yr, fz[j] - zr));
+            double delta = 1.0 / (delt1 + delt2 + SMALL);
+            double cl = delt2 * delta, cr = delt1 * delta;
+            double value = own ? (cl * nui + cr * vn[j]) : (cl * vn[j] + cr * nui);
+            double fnxa = nx[j] * ar[j], fnya = ny[j] * ar[j], fnza = nz[j] * ar[j];
+            if (own) { ax += fnxa * value; ay += fnya * value; az += fnza * value; }
+            else { ax -= fnxa * value; ay -= fnya * value; az -= fnza * value; }
+        }
     }
     double ovol = 1.0 / m.vol[i];
     d.tgrad[i] = ax * ovol;
@@ -105,6 +138,7 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d) {
 // codes/turb/src/TurbInvFlux.cpp:78-98) and diffusion
 // (UTurbVisFlux::CalcFaceVisFlux1Equ, codes/uturb/src/UTurbVisFlux.cpp:247-369).
 // saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
+template <int DIM>
 __global__ void __launch_bounds__(TBS) k_sa_face(DevMesh m, DevFields d, Consts k) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
@@ -112,11 +146,11 @@ __global__ void __launch_bounds__(TBS) k_sa_face(DevMesh m, DevFields d, Consts 
     int l = m.lc[f], r = m.rc[f];
     bool bnd = f < m.B;
     int rg = bnd ? l : r;
-    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], farea = m.area[f];
+    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = (DIM == 3) ? m.zfn[f] : 0.0, farea = m.area[f];
     const double vfn = 0.0;
     double rl = d.q[l], rr = d.q[r];
-    double ul = d.q[T + l], vl = d.q[2 * T + l], wl = d.q[3 * T + l];
-    double ur = d.q[T + r], vr = d.q[2 * T + r], wr = d.q[3 * T + r];
+    double ul = d.q[T + l], vl = d.q[2 * T + l], wl = (DIM == 3) ? d.q[3 * T + l] : 0.0;
+    double ur = d.q[T + r], vr = d.q[2 * T + r], wr = (DIM == 3) ? d.q[3 * T + r] : 0.0;
     double nl = d.turbq[l], nr = d.turbq[r];
     double visll = d.visl[l], vislr = d.visl[r];
     // spectrum
@@ -159,8 +193,8 @@ __global__ void __launch_bounds__(TBS) k_sa_face(DevMesh m, DevFields d, Consts 
     // diffusion
     VisGeom G;
     vis_geom(m, f, l, r, k.iturb_visflux, G);
-    double g1[3] = {d.tgrad[l], d.tgrad[T + l], d.tgrad[2 * T + l]};
-    double g2[3] = {d.tgrad[rg], d.tgrad[T + rg], d.tgrad[2 * T + rg]};
+    double g1[3] = {d.tgrad[l], d.tgrad[T + l], (DIM == 3) ? d.tgrad[2 * T + l] : 0.0};
+    double g2[3] = {d.tgrad[rg], d.tgrad[T + rg], (DIM == 3) ? d.tgrad[2 * T + rg] : 0.0};
     double qf, gf[3], dqdn;
     vis_grad(k.iturb_visflux, G, nl, nr, g1, g2, qf, gf, dqdn);
     double visl_nv = 0.5 * (visll * orl + vislr * orr);
@@ -260,45 +294,73 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
 // source Jacobian) and residual (source, then convection, then diffusion, each
 // gathered in ascending face order) -- TurbCalcRHS order
 // (codes/turb/src/TurbRhs.cpp:57-71).
-__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k) {
+template <int MAXF>
+__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k, int par) {
+    double mindt = slot_result<false>(d.slots, par, 12, 0);
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const int F = m.F;
+    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    int ent[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) ent[j] = (j < n) ? m.c2f[b + j] : 0;
+    double rad[MAXF], cv[MAXF], df[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (j < n) {
+            int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+            rad[j] = d.saface[f];
+            cv[j] = d.saface[3 * F + f];
+            df[j] = d.saface[4 * F + f];
+        }
+    }
     double vol = m.vol[i];
-    double ts = d.timestep[i] * k.turb_cfl_ratio;
+    double dt = d.timestep[i];
+    if (k.max_time_ratio > 0) {
+        double mp = k.max_time_ratio * mindt;
+        dt = (dt < mp) ? dt : mp;
+    }
+    double ts = dt * k.turb_cfl_ratio;
     double imp = 0.0;
     imp += (1.0 / ts) * vol;
-    int b = m.c2fOff[i], n = m.c2fOff[i + 1];
-    for (int kk = b; kk < n; ++kk) {
-        int e = m.c2f[kk];
-        imp += d.saface[e >= 0 ? e : ~e];
-    }
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j)
+        if (j < n) imp += rad[j];
     double res_sa, spec_sa;
     sa_source(m, d, k, i, res_sa, spec_sa);
     imp += spec_sa * vol;
     double res = 0.0;
     res += res_sa * vol;
-    for (int kk = b; kk < n; ++kk) {
-        int e = m.c2f[kk];
-        if (e >= 0) res -= d.saface[3 * F + e];
-        else res += d.saface[3 * F + ~e];
-    }
-    for (int kk = b; kk < n; ++kk) {
-        int e = m.c2f[kk];
-        if (e >= 0) res -= d.saface[4 * F + e];
-        else res += d.saface[4 * F + ~e];
-    }
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j)
+        if (j < n) { if (ent[j] >= 0) res -= cv[j]; else res += cv[j]; }
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j)
+        if (j < n) { if (ent[j] >= 0) res -= df[j]; else res += df[j]; }
     d.turbimpsr[i] = imp;
     d.turbres[i] = res;
 }
 
+template <int DIM, int MAXF>
+static void launch_sa(ofx_ctx* c) {
+    const DevMesh& m = c->m;
+    k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+    k_sa_face<DIM><<<nblks(m.F), TBS, 0, c->stream>>>(m, c->d, c->k);
+    k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+}
+
 int sa_residual(ofx_ctx* c) {
     if (c->k.vismodel != 3) return 0;
-    const DevMesh& m = c->m;
-    if (m.B) k_sa_bc<<<nblks(m.B), TBS, 0, c->stream>>>(m, c->d, c->k);
-    k_sa_grad<<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d);
-    k_sa_face<<<nblks(m.F), TBS, 0, c->stream>>>(m, c->d, c->k);
-    k_sa_cell<<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+    int mf = c->m.maxFacesPerCell;
+    if (c->m.dim == 2) {
+        if (mf <= 4) launch_sa<2, 4>(c);
+        else if (mf <= 8) launch_sa<2, 8>(c);
+        else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
+    } else {
+        if (mf <= 6) launch_sa<3, 6>(c);
+        else if (mf <= 12) launch_sa<3, 12>(c);
+        else { set_error("cells with more than 12 faces are not supported"); return OFX_EINVAL; }
+    }
     OFX_HIP(hipGetLastError());
     return 0;
 }
@@ -349,14 +411,13 @@ int sa_lusgs(ofx_ctx* c) {
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
         const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
-        int w = 0;
-        for (size_t L = 0; L + 1 < off.size(); ++L) w = std::max(w, off[L + 1] - off[L]);
-        if (w <= 8192) {
+        if (!use_level_launches(off)) {
             if (up) k_sa_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d);
             else k_sa_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d);
         } else {
             for (size_t L = 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
+                if (e <= b) continue;
                 if (up) k_sa_sweep_level<true><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
                 else k_sa_sweep_level<false><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
             }
@@ -374,6 +435,33 @@ __global__ void __launch_bounds__(TBS) k_sa_update(DevMesh m, DevFields d) {
     double qNew = qOld + dqNew;
     qNew = (1.0 - 1.0) * qOld + 1.0 * qNew;
     d.turbq[i] = qNew;
+}
+
+// UPDATE_FLOWFIELD + CALC_BOUNDARY of the SA solver: new nu_tilde, eddy
+// viscosity, and the ghosts of the cell's boundary faces (vist copied from
+// the owner, nu_tilde from the SA boundary conditions).
+__global__ void __launch_bounds__(TBS) k_sa_update_fused(DevMesh m, DevFields d, Consts k) {
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double qOld = d.turbq[i], dqNew = d.turbdq[i];
+    double qNew = qOld + dqNew;
+    qNew = (1.0 - 1.0) * qOld + 1.0 * qNew;
+    d.turbq[i] = qNew;
+    double vist = sa_vist_cell(d, k, m.T, i);
+    d.vist[i] = vist;
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int f = m.c2f[kk];
+        if (f < 0 || f >= m.B) break;  // boundary faces first; right-cell entries are ~f < 0
+        d.vist[m.C + f] = vist;
+        d.turbq[m.C + f] = sa_bc_value(m, d, k, f, i, qNew);
+    }
+}
+
+int sa_update_fused(ofx_ctx* c) {
+    if (c->k.vismodel != 3) return 0;
+    k_sa_update_fused<<<nblks(c->m.C), TBS, 0, c->stream>>>(c->m, c->d, c->k);
+    OFX_HIP(hipGetLastError());
+    return 0;
 }
 
 int sa_update(ofx_ctx* c) {

@@ -62,7 +62,8 @@ def load_library():
     L.ofx_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
     L.ofx_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
     L.ofx_aero.argtypes = [P, dp]
-    L.ofx_profile_last.argtypes = [P, dp]
+    L.ofx_time_kernel.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
+    L.ofx_copy_bench.argtypes = [P, ctypes.c_size_t, ctypes.c_int, dp]
     _lib = L
     return L
 
@@ -179,7 +180,16 @@ class Solver:
         self._check(self._L.ofx_aero(self.h, _dp(a)))
         return a
 
-    def profile_last(self):
-        a = np.zeros(2)
-        self._check(self._L.ofx_profile_last(self.h, _dp(a)))
-        return a
+    KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5}
+
+    def time_kernel(self, which, iters=100):
+        """Average ms of one residual part over `iters` back-to-back launches."""
+        a = np.zeros(1)
+        self._check(self._L.ofx_time_kernel(self.h, self.KERNELS[which], iters, _dp(a)))
+        return float(a[0])
+
+    def copy_bench(self, nbytes, iters=20):
+        """Average ms of a plain device copy of nbytes (bandwidth reference)."""
+        a = np.zeros(1)
+        self._check(self._L.ofx_copy_bench(self.h, ctypes.c_size_t(nbytes), iters, _dp(a)))
+        return float(a[0])
