@@ -399,7 +399,10 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
 //  codes/uns/src/UNsLimiter.cpp:67-101; UNsVisFlux::CalcVisFlux,
 //  codes/uns/src/UNsVisFlux.cpp:127-440).  LIM: -1 first order, 0 unlimited,
 // 1 limiter array.  Ghost gradients equal their owner's (Grad::CalcGrad tail).
-template <int DIM, int LIM, bool SLAU2, bool VISC>
+// F32: the FP32-flux variant -- the inviscid flux is evaluated in FP32 from
+// the FP64 reconstruction and both face fluxes are stored as FP32 (the
+// gather accumulates them in FP64).
+template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32>
 __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
@@ -452,10 +455,21 @@ __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k
         if (p1[0] <= 0.0 || p1[4] <= 0.0) for (int e = 0; e < 5; ++e) p1[e] = ql[e];
         if (p2[0] <= 0.0 || p2[4] <= 0.0) for (int e = 0; e < 5; ++e) p2[e] = qr[e];
     }
-    double fl[5];
-    if (SLAU2) slau2_flux(p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
-    else roe_flux(k, p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
-    for (int e = 0; e < 5; ++e) d.invflux[e * m.F + f] = area * fl[e];
+    if (F32) {
+        float a1[5], a2[5], fl[5];
+        for (int e = 0; e < 5; ++e) { a1[e] = (float)p1[e]; a2[e] = (float)p2[e]; }
+        float gf = (float)k.gama, nx = (float)xfn, ny = (float)yfn, nz = (float)zfn;
+        if (SLAU2) slau2_flux<float>(a1, a2, gf, gf, nx, ny, nz, fl);
+        else roe_flux<float>(k.ieigenfix, (float)k.centropy1, (float)k.centropy2, a1, a2, gf, gf, nx, ny, nz, fl);
+        float* out = reinterpret_cast<float*>(d.invflux);
+        float af = (float)area;
+        for (int e = 0; e < 5; ++e) out[e * m.F + f] = af * fl[e];
+    } else {
+        double fl[5];
+        if (SLAU2) slau2_flux<double>(p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
+        else roe_flux<double>(k.ieigenfix, k.centropy1, k.centropy2, p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
+        for (int e = 0; e < 5; ++e) d.invflux[e * m.F + f] = area * fl[e];
+    }
 
     if (!VISC) return;
     VisGeom G;
@@ -496,17 +510,26 @@ __global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k
     double fv3 = xfn * txz + yfn * tyz + zfn * tzz;
     double fv4 = qfv[1] * fv1 + qfv[2] * fv2 + qfv[3] * fv3 + qNormal;
     double coeff = -k.oreynolds * area;
-    d.visflux[0 * m.F + f] = coeff * 0.0;
-    d.visflux[1 * m.F + f] = coeff * fv1;
-    d.visflux[2 * m.F + f] = coeff * fv2;
-    d.visflux[3 * m.F + f] = coeff * fv3;
-    d.visflux[4 * m.F + f] = coeff * fv4;
+    if (F32) {
+        float* out = reinterpret_cast<float*>(d.visflux);
+        out[0 * m.F + f] = (float)(coeff * 0.0);
+        out[1 * m.F + f] = (float)(coeff * fv1);
+        out[2 * m.F + f] = (float)(coeff * fv2);
+        out[3 * m.F + f] = (float)(coeff * fv3);
+        out[4 * m.F + f] = (float)(coeff * fv4);
+    } else {
+        d.visflux[0 * m.F + f] = coeff * 0.0;
+        d.visflux[1 * m.F + f] = coeff * fv1;
+        d.visflux[2 * m.F + f] = coeff * fv2;
+        d.visflux[3 * m.F + f] = coeff * fv3;
+        d.visflux[4 * m.F + f] = coeff * fv4;
+    }
 }
 
 // Residual gather: res = -sum(inviscid) then -sum(viscous), each over the
 // cell's faces in ascending id, sign by side (AddF2CField called twice);
 // LPC lanes per cell, ordered shuffle sums.
-template <int LPC, bool VISC>
+template <int LPC, bool VISC, bool F32>
 __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
     const int T = m.T, F = m.F;
     int gid = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
@@ -519,8 +542,17 @@ __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
     if (has) {
         int e = m.c2f[b + j];
         int f = e >= 0 ? e : ~e;
-        for (int s = 0; s < 5; ++s) { double x = d.invflux[s * F + f]; fi[s] = (e >= 0) ? -x : x; }
-        if (VISC) for (int s = 0; s < 5; ++s) { double x = d.visflux[s * F + f]; fv[s] = (e >= 0) ? -x : x; }
+        const float* inv32 = reinterpret_cast<const float*>(d.invflux);
+        const float* vis32 = reinterpret_cast<const float*>(d.visflux);
+        for (int s = 0; s < 5; ++s) {
+            double x = F32 ? (double)inv32[s * F + f] : d.invflux[s * F + f];
+            fi[s] = (e >= 0) ? -x : x;
+        }
+        if (VISC)
+            for (int s = 0; s < 5; ++s) {
+                double x = F32 ? (double)vis32[s * F + f] : d.visflux[s * F + f];
+                fv[s] = (e >= 0) ? -x : x;
+            }
     }
     for (int s = 0; s < 5; ++s) {
         // continue the inviscid chain with the viscous terms: ((0 + i0) + i1 ...) + v0 + v1 ...
@@ -539,8 +571,13 @@ __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
 template <int DIM, int LIM, bool SLAU2>
 static void launch_face(ofx_ctx* c, bool visc) {
     int nb = nblk(c->m.F);
-    if (visc) k_ns_face<DIM, LIM, SLAU2, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
-    else k_ns_face<DIM, LIM, SLAU2, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+    if (c->k.flux_fp32) {
+        if (visc) k_ns_face<DIM, LIM, SLAU2, true, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+        else k_ns_face<DIM, LIM, SLAU2, false, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+    } else {
+        if (visc) k_ns_face<DIM, LIM, SLAU2, true, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+        else k_ns_face<DIM, LIM, SLAU2, false, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+    }
 }
 
 template <int DIM>
@@ -575,8 +612,13 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
     }
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
     if (parts & 8u) {
-        if (visc) k_ns_gather<MAXF, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
-        else k_ns_gather<MAXF, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+        if (k.flux_fp32) {
+            if (visc) k_ns_gather<MAXF, true, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather<MAXF, false, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+        } else {
+            if (visc) k_ns_gather<MAXF, true, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather<MAXF, false, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+        }
     }
 }
 

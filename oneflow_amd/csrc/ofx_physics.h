@@ -20,116 +20,128 @@ __device__ __forceinline__ double temperature(double rho, double p, double state
     return p / (statecoef * rho * 1.0);
 }
 
+// Real-type generic helpers for the face fluxes (R = double: the FP64 path,
+// R = float: the FP32-flux variant); same expressions as dmin/dmax/dabs/sqr3.
+template <typename R> __device__ __forceinline__ R tmin(R a, R b) { return (a < b) ? a : b; }
+template <typename R> __device__ __forceinline__ R tmax(R a, R b) { return (a > b) ? a : b; }
+template <typename R> __device__ __forceinline__ R tabs(R a) { return (a < R(0)) ? -a : a; }
+template <typename R> __device__ __forceinline__ R tsq3(R a, R b, R c) { return a * a + b * b + c * c; }
+
 // PrimToQ + CalcInternalEnergy (codes/ns/src/NsInvFlux.cpp:1349-1370,1392-1406)
-__device__ __forceinline__ void prim_to_q(const double* p, double gama, double* q) {
-    double v2 = sqr3(p[1], p[2], p[3]);
-    double em = (p[4] / p[0]) / (gama - 1.0) + 0.5 * v2;
+template <typename R>
+__device__ __forceinline__ void prim_to_q(const R* p, R gama, R* q) {
+    R v2 = tsq3(p[1], p[2], p[3]);
+    R em = (p[4] / p[0]) / (gama - R(1.0)) + R(0.5) * v2;
     q[0] = p[0]; q[1] = p[0] * p[1]; q[2] = p[0] * p[2]; q[3] = p[0] * p[3]; q[4] = p[0] * em;
 }
 
 // NsInvFlux::Roe with ModifyAbsoluteEigenvalue and CalcTotalEnthalpyChange
 // (codes/ns/src/NsInvFlux.cpp:167-308,1329-1347); static faces (vfn = 0);
 // returns the flux per unit area.
-__device__ __forceinline__ void roe_flux(const Consts& k, const double* p1, const double* p2, double gama1,
-                                         double gama2, double xfn, double yfn, double zfn, double* flux) {
-    const double vfn = 0.0;
-    double rl = p1[0], ul = p1[1], vl = p1[2], wl = p1[3], pl = p1[4];
-    double rr = p2[0], ur = p2[1], vr = p2[2], wr = p2[3], pr = p2[4];
-    double v2l = sqr3(ul, vl, wl), v2r = sqr3(ur, vr, wr);
-    double hint_l = (gama1 / (gama1 - 1.0)) * (pl / rl);
-    double hint_r = (gama2 / (gama2 - 1.0)) * (pr / rr);
-    double hl = hint_l + 0.5 * v2l, hr = hint_r + 0.5 * v2r;
-    double vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
-    double vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
-    double rvnl = rl * vnl, rvnr = rr * vnr;
-    double ratio = sqrt(rr / rl);
-    double coef = 1.0 / (1.0 + ratio);
-    double um = (ul + ur * ratio) * coef;
-    double vm = (vl + vr * ratio) * coef;
-    double wm = (wl + wr * ratio) * coef;
-    double hm = (hl + hr * ratio) * coef;
-    double gama = (gama1 + gama2 * ratio) * coef;
-    double v2 = sqr3(um, vm, wm);
-    double vnflow = xfn * um + yfn * vm + zfn * wm;
-    double vnrel = vnflow - vfn;
-    double gamm1 = gama - 1.0;
-    double c2 = gamm1 * (hm - 0.5 * v2);
-    double cm = sqrt(dabs(c2));
-    double aeig1 = dabs(vnrel), aeig2 = dabs(vnrel + cm), aeig3 = dabs(vnrel - cm);
+template <typename R>
+__device__ __forceinline__ void roe_flux(int ieigenfix, R centropy1, R centropy2, const R* p1, const R* p2,
+                                         R gama1, R gama2, R xfn, R yfn, R zfn, R* flux) {
+    const R vfn = R(0.0);
+    R rl = p1[0], ul = p1[1], vl = p1[2], wl = p1[3], pl = p1[4];
+    R rr = p2[0], ur = p2[1], vr = p2[2], wr = p2[3], pr = p2[4];
+    R v2l = tsq3(ul, vl, wl), v2r = tsq3(ur, vr, wr);
+    R hint_l = (gama1 / (gama1 - R(1.0))) * (pl / rl);
+    R hint_r = (gama2 / (gama2 - R(1.0))) * (pr / rr);
+    R hl = hint_l + R(0.5) * v2l, hr = hint_r + R(0.5) * v2r;
+    R vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
+    R vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
+    R rvnl = rl * vnl, rvnr = rr * vnr;
+    R ratio = sqrt(rr / rl);
+    R coef = R(1.0) / (R(1.0) + ratio);
+    R um = (ul + ur * ratio) * coef;
+    R vm = (vl + vr * ratio) * coef;
+    R wm = (wl + wr * ratio) * coef;
+    R hm = (hl + hr * ratio) * coef;
+    R gama = (gama1 + gama2 * ratio) * coef;
+    R v2 = tsq3(um, vm, wm);
+    R vnflow = xfn * um + yfn * vm + zfn * wm;
+    R vnrel = vnflow - vfn;
+    R gamm1 = gama - R(1.0);
+    R c2 = gamm1 * (hm - R(0.5) * v2);
+    R cm = sqrt(tabs(c2));
+    R aeig1 = tabs(vnrel), aeig2 = tabs(vnrel + cm), aeig3 = tabs(vnrel - cm);
     flux[0] = rvnl + rvnr;
     flux[1] = (rvnl * ul + xfn * pl) + (rvnr * ur + xfn * pr);
     flux[2] = (rvnl * vl + yfn * pl) + (rvnr * vr + yfn * pr);
     flux[3] = (rvnl * wl + zfn * pl) + (rvnr * wr + zfn * pr);
     flux[4] = (rvnl * hl + vfn * pl) + (rvnr * hr + vfn * pr);
-    double q1[5], q2[5], dq[5];
+    R q1[5], q2[5], dq[5];
     prim_to_q(p1, gama1, q1);
     prim_to_q(p2, gama2, q2);
     for (int e = 0; e < 5; ++e) dq[e] = q2[e] - q1[e];
-    double meig1 = aeig1, meig2 = aeig2, meig3 = aeig3;
-    if (k.ieigenfix == 1) {
-        double maxE = dmax(aeig2, aeig3);
-        double m1 = maxE * k.centropy1, m2 = maxE * k.centropy2;
-        meig1 = dmax(m1, aeig1); meig2 = dmax(m2, aeig2); meig3 = dmax(m2, aeig3);
+    R meig1 = aeig1, meig2 = aeig2, meig3 = aeig3;
+    if (ieigenfix == 1) {
+        R maxE = tmax(aeig2, aeig3);
+        R m1 = maxE * centropy1, m2 = maxE * centropy2;
+        meig1 = tmax(m1, aeig1); meig2 = tmax(m2, aeig2); meig3 = tmax(m2, aeig3);
     }
-    double xi1 = (2.0 * meig1 - meig2 - meig3) / (2.0 * c2);
-    double xi2 = (meig2 - meig3) / (2.0 * cm);
-    double dc = vnflow * dq[0] - xfn * dq[1] - yfn * dq[2] - zfn * dq[3];
-    double c2dc = c2 * dc;
-    double ae = gama - 1.0;
-    double af = 0.5 * ae * v2;
-    double dh = -ae * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
+    R xi1 = (R(2.0) * meig1 - meig2 - meig3) / (R(2.0) * c2);
+    R xi2 = (meig2 - meig3) / (R(2.0) * cm);
+    R dc = vnflow * dq[0] - xfn * dq[1] - yfn * dq[2] - zfn * dq[3];
+    R c2dc = c2 * dc;
+    R ae = gama - R(1.0);
+    R af = R(0.5) * ae * v2;
+    R dh = -ae * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
     dh += af * dq[0];
-    double term1 = dh * xi1 + dc * xi2;
-    double term2 = c2dc * xi1 + dh * xi2;
+    R term1 = dh * xi1 + dc * xi2;
+    R term2 = c2dc * xi1 + dh * xi2;
     flux[0] -= (meig1 * dq[0] - term1);
     flux[1] -= (meig1 * dq[1] - um * term1 + xfn * term2);
     flux[2] -= (meig1 * dq[2] - vm * term1 + yfn * term2);
     flux[3] -= (meig1 * dq[3] - wm * term1 + zfn * term2);
     flux[4] -= (meig1 * dq[4] - hm * term1 + vnflow * term2);
-    for (int e = 0; e < 5; ++e) flux[e] *= 0.5;
+    for (int e = 0; e < 5; ++e) flux[e] *= R(0.5);
 }
 
 // FMSplit2 / FPSplit5 (codes/ns/src/NsInvFlux.cpp:42-65)
-__device__ __forceinline__ double fm_split2(double mach, double ipn) { return ipn * 0.25 * ((mach + ipn) * (mach + ipn)); }
-__device__ __forceinline__ double fp_split5(double mach, double alpha, double ipn) {
-    double macha = dabs(mach);
-    if (macha >= 1.0) return 0.5 * (1.0 + ipn * mach / (macha + SMALL));
-    return fm_split2(mach, ipn) * ((2.0 * ipn - mach) - ipn * 16.0 * alpha * mach * fm_split2(mach, -ipn));
+template <typename R>
+__device__ __forceinline__ R fm_split2(R mach, R ipn) { return ipn * R(0.25) * ((mach + ipn) * (mach + ipn)); }
+template <typename R>
+__device__ __forceinline__ R fp_split5(R mach, R alpha, R ipn) {
+    R macha = tabs(mach);
+    if (macha >= R(1.0)) return R(0.5) * (R(1.0) + ipn * mach / (macha + R(SMALL)));
+    return fm_split2(mach, ipn) * ((R(2.0) * ipn - mach) - ipn * R(16.0) * alpha * mach * fm_split2(mach, -ipn));
 }
 
 // NsInvFlux::Slau2 (codes/ns/src/NsInvFlux.cpp:1246-1314), static faces
-__device__ __forceinline__ void slau2_flux(const double* p1, const double* p2, double gama1, double gama2,
-                                           double xfn, double yfn, double zfn, double* flux) {
-    const double vfn = 0.0;
-    double rl = p1[0], ul = p1[1], vl = p1[2], wl = p1[3], pl = p1[4];
-    double rr = p2[0], ur = p2[1], vr = p2[2], wr = p2[3], pr = p2[4];
-    double v2l = sqr3(ul, vl, wl), v2r = sqr3(ur, vr, wr);
-    double hint_l = (gama1 / (gama1 - 1.0)) * (pl / rl);
-    double hint_r = (gama2 / (gama2 - 1.0)) * (pr / rr);
-    double hl = hint_l + 0.5 * v2l, hr = hint_r + 0.5 * v2r;
-    double vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
-    double vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
-    double orl = 1.0 / (rl + SMALL), orr = 1.0 / (rr + SMALL);
-    double c2l = gama1 * pl * orl, c2r = gama2 * pr * orr;
-    double cl = sqrt(c2l), cr = sqrt(c2r);
-    double cm = 0.5 * (cl + cr);
-    double t = rr / rl;
-    double vna = (dabs(vnl) + t * dabs(vnr)) / (1 + t);
-    double ml = vnl / cm, mr = vnr / cm;
-    double g = -dmax(dmin(ml, 0.0), -1.0) * dmin(dmax(mr, 0.0), 1.0);
-    double vnp = (1 - g) * vna + g * dabs(vnl);
-    double vnn = (1 - g) * vna + g * dabs(vnr);
-    double va = sqrt(0.5 * (v2l + v2r));
-    double m12 = dmin(1.0, va / cm);
-    double ka = (1 - m12) * (1 - m12);
-    double ms = 0.5 * (rl * (vnl + vnp) + rr * (vnr - vnn) - ka * (pr - pl) / cm);
-    double fp5ml = fp_split5(ml, 0.0, 1.0);
-    double fp5mr = fp_split5(mr, 0.0, -1.0);
-    double ps = 0;
-    ps += 0.5 * (pl + pr);
-    ps += 0.5 * (fp5ml - fp5mr) * (pl - pr);
-    ps += va * (fp5ml + fp5mr - 1) * sqrt(rl * rr) * cm;
-    double mp = 0.5 * (ms + dabs(ms)), mn = 0.5 * (ms - dabs(ms));
+template <typename R>
+__device__ __forceinline__ void slau2_flux(const R* p1, const R* p2, R gama1, R gama2,
+                                           R xfn, R yfn, R zfn, R* flux) {
+    const R vfn = R(0.0);
+    R rl = p1[0], ul = p1[1], vl = p1[2], wl = p1[3], pl = p1[4];
+    R rr = p2[0], ur = p2[1], vr = p2[2], wr = p2[3], pr = p2[4];
+    R v2l = tsq3(ul, vl, wl), v2r = tsq3(ur, vr, wr);
+    R hint_l = (gama1 / (gama1 - R(1.0))) * (pl / rl);
+    R hint_r = (gama2 / (gama2 - R(1.0))) * (pr / rr);
+    R hl = hint_l + R(0.5) * v2l, hr = hint_r + R(0.5) * v2r;
+    R vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
+    R vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
+    R orl = R(1.0) / (rl + R(SMALL)), orr = R(1.0) / (rr + R(SMALL));
+    R c2l = gama1 * pl * orl, c2r = gama2 * pr * orr;
+    R cl = sqrt(c2l), cr = sqrt(c2r);
+    R cm = R(0.5) * (cl + cr);
+    R t = rr / rl;
+    R vna = (tabs(vnl) + t * tabs(vnr)) / (R(1.0) + t);
+    R ml = vnl / cm, mr = vnr / cm;
+    R g = -tmax(tmin(ml, R(0.0)), R(-1.0)) * tmin(tmax(mr, R(0.0)), R(1.0));
+    R vnp = (R(1.0) - g) * vna + g * tabs(vnl);
+    R vnn = (R(1.0) - g) * vna + g * tabs(vnr);
+    R va = sqrt(R(0.5) * (v2l + v2r));
+    R m12 = tmin(R(1.0), va / cm);
+    R ka = (R(1.0) - m12) * (R(1.0) - m12);
+    R ms = R(0.5) * (rl * (vnl + vnp) + rr * (vnr - vnn) - ka * (pr - pl) / cm);
+    R fp5ml = fp_split5(ml, R(0.0), R(1.0));
+    R fp5mr = fp_split5(mr, R(0.0), R(-1.0));
+    R ps = R(0.0);
+    ps += R(0.5) * (pl + pr);
+    ps += R(0.5) * (fp5ml - fp5mr) * (pl - pr);
+    ps += va * (fp5ml + fp5mr - R(1.0)) * sqrt(rl * rr) * cm;
+    R mp = R(0.5) * (ms + tabs(ms)), mn = R(0.5) * (ms - tabs(ms));
     flux[0] = (mp + mn);
     flux[1] = (mp * ul + mn * ur) + xfn * ps;
     flux[2] = (mp * vl + mn * vr) + yfn * ps;

@@ -318,3 +318,159 @@ def naca_hybrid(n_surf: int = 512, n_quad: int = 40, ratio: float = 1.15, reynol
         raise ValueError("unexpected boundary edge")
 
     return grid_from_cells(x, y, cells, bc)
+
+
+# ---------------------------------------------------------------- 2-D box
+
+def box2d(nx: int, ny: int, x0=0.0, x1=1.0, y0=0.0, y1=1.0, tri=False, bc=None, jitter=0.0, seed=0):
+    """Rectangle of nx x ny quads (or 2 triangles each when tri), optional
+    node jitter (interior nodes, fraction of the spacing, seeded).
+    bc(xm, ym) -> bc type for a boundary edge with midpoint (xm, ym);
+    default: far field everywhere."""
+    xs, ys = np.linspace(x0, x1, nx + 1), np.linspace(y0, y1, ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        hx, hy = (x1 - x0) / nx, (y1 - y0) / ny
+        inner = np.zeros_like(X, bool)
+        inner[1:-1, 1:-1] = True
+        X = X + np.where(inner, rng.uniform(-jitter, jitter, X.shape) * hx, 0.0)
+        Y = Y + np.where(inner, rng.uniform(-jitter, jitter, Y.shape) * hy, 0.0)
+    x, y = X.ravel(), Y.ravel()
+    nid = lambda i, j: i * (ny + 1) + j
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            if tri:
+                cells.append([a, b, c, -1])
+                cells.append([a, c, d, -1])
+            else:
+                cells.append([a, b, c, d])
+    bcf = bc if bc is not None else (lambda xm, ym: BC_FARFIELD)
+    return grid_from_cells(x, y, np.array(cells), lambda a, b: bcf(0.5 * (x[a] + x[b]), 0.5 * (y[a] + y[b])))
+
+
+# ---------------------------------------------------------------- 3-D assembly
+
+# Outward-oriented local faces (right-hand rule gives the outward normal) of
+# positively oriented cells.  Hex: bottom 0-3 counter-clockwise seen from +z,
+# top 4-7 above them.  Prism: bottom triangle 0,1,2 counter-clockwise, top 3,4,5.
+HEX_FACES = ((0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (3, 7, 6, 2), (0, 4, 7, 3), (1, 2, 6, 5))
+PRISM_FACES = ((0, 2, 1), (3, 4, 5), (0, 1, 4, 3), (1, 2, 5, 4), (2, 0, 3, 5))
+TET_FACES = ((0, 2, 1), (0, 1, 3), (0, 3, 2), (1, 2, 3))
+
+
+def grid_from_polyhedra(x, y, z, cells, bc_of_face):
+    """Assemble a 3-D face list from cells given as node tuples of length 4
+    (tet), 6 (prism) or 8 (hex), each positively oriented.  Interior faces get
+    lc < rc with nodes ordered so the normal points from lc to rc; boundary
+    faces point out of their owner.  bc_of_face(xc, yc, zc, n) -> bc type
+    (face centroid and outward unit normal)."""
+    x, y, z = (np.asarray(v, np.float64) for v in (x, y, z))
+    table = {4: TET_FACES, 6: PRISM_FACES, 8: HEX_FACES}
+    faces = {}
+    for ci, cell in enumerate(cells):
+        for loc in table[len(cell)]:
+            nodes = tuple(int(cell[k]) for k in loc)
+            key = tuple(sorted(nodes))
+            if key in faces:
+                owner, onodes, other = faces[key]
+                if other is not None:
+                    raise ValueError("non-manifold face")
+                faces[key] = (owner, onodes, ci)
+            else:
+                faces[key] = (ci, nodes, None)
+    bnd, inner = [], []
+    for key, (owner, nodes, other) in faces.items():
+        if other is None:
+            p = np.array([[x[n], y[n], z[n]] for n in nodes])
+            c = p.mean(axis=0)
+            nrm = np.zeros(3)
+            for i in range(len(nodes)):
+                nrm += np.cross(p[i], p[(i + 1) % len(nodes)])
+            nrm /= np.linalg.norm(nrm)
+            bnd.append((bc_of_face(c[0], c[1], c[2], nrm), owner, nodes))
+        else:
+            # nodes are in `owner`'s outward order; lc must be the smaller id
+            if owner < other:
+                inner.append((owner, other, nodes))
+            else:
+                inner.append((other, owner, tuple(reversed(nodes))))
+    bnd.sort(key=lambda t: (t[0], t[1]))
+    inner.sort(key=lambda t: (t[0], t[1]))
+    nb = len(bnd)
+    fnc = [len(t[2]) for t in bnd] + [len(t[2]) for t in inner]
+    fn = [n for t in bnd for n in t[2]] + [n for t in inner for n in t[2]]
+    lc = [t[1] for t in bnd] + [t[0] for t in inner]
+    rc = [-1] * nb + [t[1] for t in inner]
+    bct = np.array([t[0] for t in bnd], np.int32)
+    g = Grid(dim=3, x=x, y=y, z=z, face_node_count=np.array(fnc, np.int32), face_nodes=np.array(fn, np.int32),
+             lc=np.array(lc, np.int32), rc=np.array(rc, np.int32), bc_type=bct, bc_name_id=bct.copy(),
+             n_cells=len(cells))
+    g.check()
+    return g
+
+
+def _tet_volume(p, cell):
+    a, b, c, d = (p[k] for k in cell)
+    return float(np.dot(np.cross(b - a, c - a), d - a)) / 6.0
+
+
+def hex_to_tets(h, p):
+    """Kuhn split of a hex (HEX node order) into 6 positively oriented tets;
+    conforming between neighbouring hexes split the same way."""
+    # corner index by (i, j, k) bits
+    corner = {(0, 0, 0): h[0], (1, 0, 0): h[1], (1, 1, 0): h[2], (0, 1, 0): h[3],
+              (0, 0, 1): h[4], (1, 0, 1): h[5], (1, 1, 1): h[6], (0, 1, 1): h[7]}
+    tets = []
+    for perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        v = [0, 0, 0]
+        path = [corner[tuple(v)]]
+        for ax in perm:
+            v[ax] = 1
+            path.append(corner[tuple(v)])
+        if _tet_volume(p, path) < 0:
+            path[1], path[2] = path[2], path[1]
+        tets.append(tuple(path))
+    return tets
+
+
+def hex_to_prisms(h):
+    """Two prisms per hex, bottom quad split along its 0-2 diagonal."""
+    return [(h[0], h[1], h[2], h[4], h[5], h[6]), (h[0], h[2], h[3], h[4], h[6], h[7])]
+
+
+def box3d(nx: int, ny: int, nz: int, kind: str = "hex", extent=(1.0, 1.0, 1.0), zs=None, prism_layers=0,
+          bc=None, jitter=0.0, seed=0):
+    """Box of nx x ny x nz hexes, kept as hexes ("hex"), split into 6 tets
+    ("tet"), 2 prisms ("prism"), or prisms in the lowest `prism_layers`
+    layers and tets above ("mixed").  zs: optional node z levels.
+    bc(xc, yc, zc, n) -> bc type (default far field)."""
+    xs = np.linspace(0.0, extent[0], nx + 1)
+    ys = np.linspace(0.0, extent[1], ny + 1)
+    zz = np.asarray(zs, np.float64) if zs is not None else np.linspace(0.0, extent[2], nz + 1)
+    X, Y, Z = np.meshgrid(xs, ys, zz, indexing="ij")
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        inner = np.zeros_like(X, bool)
+        inner[1:-1, 1:-1, 1:-1] = True
+        h = min(extent[0] / nx, extent[1] / ny, float(np.min(np.diff(zz))))
+        for A in (X, Y, Z):
+            A += np.where(inner, rng.uniform(-jitter, jitter, A.shape) * h, 0.0)
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    nid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                h = (nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                     nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1))
+                if kind == "hex":
+                    cells.append(h)
+                elif kind == "prism" or (kind == "mixed" and k < prism_layers):
+                    cells.extend(hex_to_prisms(h))
+                else:
+                    cells.extend(hex_to_tets(h, p))
+    bcf = bc if bc is not None else (lambda xc, yc, zc, n: BC_FARFIELD)
+    return grid_from_polyhedra(p[:, 0], p[:, 1], p[:, 2], cells, bcf)

@@ -5,10 +5,11 @@
 // accumulation order; compiled with -ffp-contract=off.  Each function names
 // the reference function (file:line, relative to the OneFLOW tree) it follows.
 //
-// Parity status of each part is recorded in DESIGN.md of the repository; the
-// end-to-end pin is the reference's own golden res.dat / turbres.dat /
+// The end-to-end pin is the reference's own golden res.dat / turbres.dat /
 // aero.dat of test/plateuns2dslau2, test/turbplateuns2droe_sa and
-// test/rae2822_roe_sa (checked by tests/test_oracle_golden.py).
+// test/rae2822_roe_sa (tests/test_oracle_golden.py); the options those cases
+// do not reach are pinned by closed forms and invariants
+// (tests/test_oracle_invariants.py).
 
 #include "oracle.h"
 

@@ -1,0 +1,156 @@
+"""GPU (libofx.so) vs the CPU oracle across the options the reference cases
+do not exercise: Barth and zero-order reconstruction, VIS_AVER / VIS_TEST face
+gradients, isothermal walls, inflow / outflow / extrapolation boundaries, the
+other viscous spectral-radius models and SA diffusion methods, and 3-D grids
+of hexes, tets, prisms and prism layers under tets.
+
+Each case runs on small jittered synthetic grids: one pseudo-time step task by
+task with the oracle's state given to the GPU before every task (tolerance
+1e-12 normwise, FP64), then five whole steps (tolerance 1e-10: the SA source's
+pow() may differ from the host libm by an ulp, which the chained steps carry).
+"""
+import numpy as np
+import pytest
+
+from oneflow_amd import meshgen
+from oneflow_amd.config import CaseConfig
+from oneflow_amd.mesh import BC_EXTRAPOLATION, BC_FARFIELD, BC_INFLOW, BC_OUTFLOW, BC_SOLID_SURFACE, BC_SYMMETRY
+from oracle.pyoracle import OracleCase
+from tests.parity_util import report
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-12
+
+
+def plate_bc(xm, ym, *rest):
+    if abs(xm) < 1e-12:
+        return BC_INFLOW
+    if abs(xm - 1.0) < 1e-12:
+        return BC_OUTFLOW
+    if abs(ym) < 1e-12:
+        return BC_SOLID_SURFACE if xm > 0.25 else BC_SYMMETRY
+    return BC_FARFIELD
+
+
+def box_bc3(xc, yc, zc, n):
+    if abs(zc) < 1e-12:
+        return BC_SOLID_SURFACE if (xc > 0.25 and yc < 0.75) else BC_SYMMETRY
+    if abs(xc - 1.0) < 1e-12:
+        return BC_EXTRAPOLATION
+    return BC_FARFIELD
+
+
+GRIDS = {
+    "quad": lambda: meshgen.box2d(12, 8, jitter=0.2, seed=11, bc=plate_bc),
+    "tri": lambda: meshgen.box2d(10, 6, tri=True, jitter=0.2, seed=12, bc=plate_bc),
+    "hex": lambda: meshgen.box3d(5, 4, 4, "hex", jitter=0.15, seed=13, bc=box_bc3),
+    "tet": lambda: meshgen.box3d(4, 3, 3, "tet", jitter=0.15, seed=14, bc=box_bc3),
+    "prism": lambda: meshgen.box3d(4, 4, 3, "prism", jitter=0.15, seed=15, bc=box_bc3),
+    "mixed": lambda: meshgen.box3d(4, 3, 4, "mixed", prism_layers=2, jitter=0.15, seed=16, bc=box_bc3),
+}
+
+OPTS = {
+    "roe_barth_lam_std": dict(ischeme=1, ilim=1, vismodel=1, ivischeme=0),
+    "slau2_o1_lam_aver_iso": dict(ischeme=12, ilim=-1, vismodel=1, ivischeme=1, isowallbc=1, twall_dim=330.0),
+    "roe_venkat_sa_test": dict(ischeme=1, ilim=2, vismodel=3, ivischeme=2, vencat_coef=0.01),
+    "slau2_barth_euler": dict(ischeme=12, ilim=1, vismodel=0),
+    "roe_none_sa_aver_iso_sr1": dict(ischeme=1, ilim=0, vismodel=3, ivischeme=1, isowallbc=1, twall_dim=280.0,
+                                     visSRModel=1, iturb_visflux=0),
+    "roe_venkat_sa_sr3_tv1": dict(ischeme=1, ilim=2, vismodel=3, ivischeme=0, visSRModel=3, iturb_visflux=1,
+                                  iprod_sa=1, vencat_coef=0.01),
+}
+
+CASES = [(g, o) for g in ("quad", "tri") for o in OPTS] + \
+        [(g, o) for g in ("hex", "tet", "prism", "mixed") for o in ("roe_venkat_sa_test", "slau2_o1_lam_aver_iso",
+                                                                      "roe_barth_lam_std")]
+
+
+def make(grid, opt):
+    from oneflow_amd.solver import Solver
+    g = GRIDS[grid]()
+    kw = dict(dim=g.dim, mach_ref=0.5, aoa_degree=2.0, reynolds=5.0e4, cflst=1.0, cfled=5.0, ncfl=10)
+    kw.update(OPTS[opt])
+    cfg = CaseConfig(**kw)
+    print(f"[options] grid={grid} opt={opt} cells={g.n_cells} faces={g.n_faces}", flush=True)
+    return Solver(g, cfg), OracleCase(g, cfg), g
+
+
+def cmp(gpu, ora, names, tol=TOL):
+    for n in names:
+        a, b = gpu.get(n), ora.get(n)
+        if n in ("RES", "DQ", "LIMITER", "TIMESTEP", "INVSR", "VISSR", "IMPSR", "TURBRES", "TURBDQ", "TURBIMPSR"):
+            C = ora.n_cells
+            a, b = a[..., :C], b[..., :C]
+        e = report(n, a, b)
+        assert e <= tol, (n, e)
+
+
+STATE = ["Q", "TURBQ", "TEMPR", "VISL", "VIST", "TIMESTEP", "INVSR", "VISSR", "RES", "TURBRES", "DQ", "TURBDQ"]
+
+
+def sync(gpu, ora):
+    for n in STATE:
+        gpu.set(n, ora.get(n))
+
+
+@pytest.mark.parametrize("grid,opt", CASES)
+def test_options_task_by_task(grid, opt):
+    gpu, ora, g = make(grid, opt)
+    cmp(gpu, ora, ["FACE_NORMAL", "FACE_CENTER", "XCC", "VOL", "DIST"], tol=0.0)
+    for s in (gpu, ora):
+        s.init_freestream()
+    ora.iterate(1, 3)  # a developed, non-uniform state
+    sync(gpu, ora)
+    for s in (gpu, ora):
+        s.timestep(4)
+    cmp(gpu, ora, ["TIMESTEP", "INVSR", "VISSR"])
+    sync(gpu, ora)
+    for s in (gpu, ora):
+        s.residual_ns()
+    cmp(gpu, ora, ["DQDX", "DTDX"])
+    if ora.cfg.ilim in (1, 2):
+        cmp(gpu, ora, ["LIMITER"])
+    cmp(gpu, ora, ["RES"])
+    sync(gpu, ora)
+    for s in (gpu, ora):
+        s.lusgs_ns()
+    cmp(gpu, ora, ["IMPSR", "DQ"])
+    sync(gpu, ora)
+    for s in (gpu, ora):
+        s.update_ns()
+        s.ns_boundary()
+    cmp(gpu, ora, ["Q", "TEMPR", "VISL"])
+    if ora.cfg.vismodel == 3:
+        sync(gpu, ora)
+        for s in (gpu, ora):
+            s.residual_ns()
+        sync(gpu, ora)
+        for s in (gpu, ora):
+            s.residual_sa()
+        cmp(gpu, ora, ["TURBDQDX", "TURBIMPSR"])
+        cmp(gpu, ora, ["TURBRES"], tol=1e-11)
+        sync(gpu, ora)
+        gpu.set("TURBIMPSR", ora.get("TURBIMPSR"))
+        for s in (gpu, ora):
+            s.lusgs_sa()
+        cmp(gpu, ora, ["TURBDQ"])
+        sync(gpu, ora)
+        for s in (gpu, ora):
+            s.update_sa()
+            s.sa_boundary()
+        cmp(gpu, ora, ["TURBQ", "VIST"])
+
+
+@pytest.mark.parametrize("grid,opt", CASES)
+def test_options_five_steps(grid, opt):
+    gpu, ora, g = make(grid, opt)
+    for s in (gpu, ora):
+        s.init_freestream()
+    ng = gpu.iterate(1, 5)
+    no = ora.iterate(1, 5)
+    e = report("norms", ng, no)
+    assert e <= 1.0e-10, e
+    cmp(gpu, ora, ["Q", "TURBQ"], tol=1.0e-10)
+    ag, ao = gpu.aero(), ora.aero()
+    assert np.allclose(ag, ao, rtol=1e-9, atol=1e-14), (ag, ao)
