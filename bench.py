@@ -33,6 +33,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -109,6 +111,79 @@ def cpu_baseline(grid, cfg, budget_s=10.0):
     return n, dt
 
 
+def _sweep_grid(case):
+    return case.grid("multicolor")
+
+
+def batch_main(args):
+    """The batched airfoil sweep: 64 seeded NACA 4-digit cases (20k-200k cells,
+    Mach 0.3-0.85, AoA -2..6 deg, Re 3e6-1e7, Roe + SA + Venkat, FP64) advanced
+    together.  One step = one full pseudo-time step (NS + SA residual, LU-SGS,
+    update) of every case.  Timed: graph replays of the batch (face tiles
+    autotuned per case) against the same steps launched case after case."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from oneflow_amd.cases import airfoil_sweep
+    from oneflow_amd.solver import Batch, Solver
+
+    cases = airfoil_sweep(args.cases, seed=args.seed)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        grids = list(ex.map(_sweep_grid, cases))
+    ncell = sum(g.n_cells for g in grids)
+    nt = sum(g.n_tcell for g in grids)
+    print(f"[bench] {len(grids)} grids, {ncell} cells, generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    sol = []
+    for g, c in zip(grids, cases):
+        s = Solver(g, c.config(), device=0)
+        s.init_freestream()
+        sol.append(s)
+    b = Batch(sol, steps_per_graph=args.graph_steps)
+    ramp = max(c.config().ncfl for c in cases)
+    b.iterate(1, ramp)  # CFL ramp, launched directly
+    tiles = b.autotune(iters=10)
+    it = ramp + 1
+    b.iterate(it, args.graph_steps)  # records the graph, first replay
+    it += args.graph_steps
+    t0 = time.perf_counter()
+    k = b.iterate(it, args.steps)
+    t1 = time.perf_counter()
+    it += args.steps
+    ms_batch = 1e3 * (t1 - t0) / args.steps
+    # the same steps case after case, eager launches, no graph
+    seq_steps = max(2, args.steps // 4)
+    for s in sol:
+        s.sync()
+    t0 = time.perf_counter()
+    for s in sol:
+        s.iterate(it, seq_steps, with_norms=False)
+    t1 = time.perf_counter()
+    ms_seq = 1e3 * (t1 - t0) / seq_steps
+    out = {
+        "metric": "cell-steps/s of the batched airfoil sweep (full NS+SA pseudo-time steps)",
+        "value": nt * 1e3 / ms_batch,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": ramp + args.graph_steps,
+        "ms_per_step": ms_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(cases)} NACA 4-digit airfoils, hybrid grids, Roe + SA + Venkat, FP64",
+                   "cells": ncell, "nTCell": nt, "seed": args.seed, "graph_steps": args.graph_steps,
+                   "graph_launches": k},
+        "sequential_ms_per_step": ms_seq,
+        "batch_speedup": ms_seq / ms_batch,
+        "face_tiles": {str(t): int(n) for t, n in zip(*np.unique(tiles[:, 0], return_counts=True))},
+    }
+    print(json.dumps(out))
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,10 +195,17 @@ def main():
     ap.add_argument("--numbering", default="multicolor", choices=["natural", "multicolor"])
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
+    ap.add_argument("--workload", default="c1", choices=["c1", "batch"],
+                    help="c1: single-grid residual throughput (default); batch: the 64-case sweep")
+    ap.add_argument("--cases", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--graph-steps", type=int, default=10, help="steps recorded per graph replay (batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "batch":
+        return batch_main(args)
 
     from oneflow_amd.solver import Solver
 

@@ -209,6 +209,64 @@ int ofx_time_kernel(ofx_ctx* ctx, int which, int iters, double* ms_out);
  * Allocates and frees its own two buffers. */
 int ofx_copy_bench(ofx_ctx* ctx, size_t bytes, int iters, double* ms_out);
 
+/* Face-kernel tile, the threads per block of the NS and SA face-flux kernels:
+ * 64, 128, 256 (default) or 512.  Results do not depend on it (no
+ * counterpart in the reference; a launch parameter). */
+int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
+
+/* Autotuner (no counterpart in the reference): times the NS face kernel and
+ * the SA residual on the current state for each face tile over `iters`
+ * launches, keeps the fastest, and reports them in chosen2[0..1] (may be
+ * NULL).  Leaves res / turbres as one evaluation of the current state. */
+int ofx_autotune(ofx_ctx* ctx, int iters, int* chosen2);
+
+/* Batched cases (the reference solves zones one after another,
+ * codes/task/src/SimpleTask.cpp; here many independent cases advance
+ * together).  A batch references existing contexts (mesh, config and state
+ * set, all on one device; the batch does not own them).  Pseudo-time steps
+ * past every case's CFL ramp are recorded once into one HIP graph holding
+ * steps_per_graph steps (rounded up to even) of every case, one independent
+ * branch per case, and replayed; steps inside the ramp are launched directly.
+ * Per case, the results are bit-identical to ofx_iterate on that context. */
+typedef struct ofx_batch ofx_batch;
+int ofx_batch_create(ofx_batch** out, ofx_ctx* const* ctxs, int n, int steps_per_graph);
+int ofx_batch_destroy(ofx_batch* b);
+/* Advance every case n steps from step first_iter (TimeIntegral::Lusgs for NS
+ * then SA, as ofx_iterate); graph_launches (may be NULL) receives the number
+ * of graph replays used. */
+int ofx_batch_iterate(ofx_batch* b, int first_iter, int n, int* graph_launches);
+/* Residual norms of each case's current residual, out[n]. */
+int ofx_batch_norms(ofx_batch* b, ofx_norms* out);
+/* ofx_autotune on every case; chosen (may be NULL) receives 2 tiles per case
+ * (a recorded graph is recorded again at the next ofx_batch_iterate). */
+int ofx_batch_autotune(ofx_batch* b, int iters, int* chosen);
+
+/* ---- warm-start predictor (no counterpart in the reference) ----------
+ * A BF16 multilayer perceptron, 9 inputs -> 5 hidden layers of 256 (SiLU)
+ * -> 5 outputs, evaluated and trained by the library's own MFMA kernels
+ * (FP32 accumulation, FP32 Adam master weights).  Inputs per sample:
+ * [x, y, wall-distance feature, m, p, t, log10 Re, Mach, AoA] (scaled by the
+ * caller); outputs: the normalised rho, u, v, p, nu_tilde targets the caller
+ * trained on.  The parameter vector is, per layer, W[kin][kout] then b[kout]
+ * with padded shapes kin = 16, 256, 256, 256, 256, 256 and
+ * kout = 256, 256, 256, 256, 256, 64 (padding entries are zero). */
+typedef struct ofx_mlp ofx_mlp;
+int ofx_mlp_create(ofx_mlp** out, int device, unsigned long long seed);
+int ofx_mlp_destroy(ofx_mlp* net);
+int ofx_mlp_param_count(ofx_mlp* net, int* count);
+int ofx_mlp_get_params(ofx_mlp* net, float* host);
+int ofx_mlp_set_params(ofx_mlp* net, const float* host);
+/* y[n][5] = net(x[n][9]), n <= 2^20 per call. */
+int ofx_mlp_forward(ofx_mlp* net, const float* x, int n, float* y);
+/* Mean-squared-error loss over the 5 outputs and its gradient with respect to
+ * every parameter (same layout as the parameter vector); no update. */
+int ofx_mlp_gradients(ofx_mlp* net, const float* x, const float* t, int n, float* loss, float* grads);
+/* `steps` Adam steps (learning rate lr) on mini-batches of `batch` samples
+ * drawn without replacement from (x, t), reshuffled each epoch (seeded);
+ * loss_out (may be NULL) receives each step's mini-batch loss. */
+int ofx_mlp_train(ofx_mlp* net, const float* x, const float* t, int n, int batch, int steps, float lr,
+                  unsigned long long seed, float* loss_out);
+
 /* Synchronise the context's stream. */
 int ofx_sync(ofx_ctx* ctx);
 

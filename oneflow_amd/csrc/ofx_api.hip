@@ -209,6 +209,7 @@ int ofx_mesh_upload(ofx_ctx* c, int dim, int nNodes, const double* x, const doub
     OFX_TRY(alloc_fields(c));
     OFX_HIP(hipStreamSynchronize(c->stream));
     c->has_mesh = true;
+    ++c->gen;
     return 0;
 }
 
@@ -235,6 +236,7 @@ int ofx_config_set(ofx_ctx* c, const ofx_config* cfg) {
     c->cfg = *cfg;
     set_consts(c);
     c->has_cfg = true;
+    ++c->gen;
     return 0;
 }
 
@@ -343,19 +345,32 @@ int ofx_norms_get(ofx_ctx* c, ofx_norms* out) {
 // One StrongIter per step: NS then SA, each the TimeIntegral::Lusgs task list
 // ZERO_DQ_FIELD, CALC_TIME_STEP, LOAD_RESIDUALS, UPDATE_RESIDUALS, LU-SGS
 // sweeps, UPDATE_FLOWFIELD_LUSGS, CALC_BOUNDARY.
+}  // extern "C"
+
+namespace ofx {
+// One pseudo-time step, kernels only (no host synchronisation, no memset), so
+// it can also be recorded into a HIP graph.
+int step_launch(ofx_ctx* c, int iter) {
+    OFX_TRY(ns_timestep(c, iter));
+    OFX_TRY(ns_residual(c));
+    OFX_TRY(ns_lusgs(c));
+    OFX_TRY(ns_update_fused(c));
+    if (c->k.vismodel == 3) {
+        OFX_TRY(sa_residual(c));
+        OFX_TRY(sa_lusgs(c));
+        OFX_TRY(sa_update_fused(c));
+    }
+    return 0;
+}
+}  // namespace ofx
+
+extern "C" {
+
 int ofx_iterate(ofx_ctx* c, int first_iter, int n, ofx_norms* norms_out) {
     CHECK_READY(c);
     if (n < 0) { set_error("n must be >= 0"); return OFX_EINVAL; }
     for (int s = 0; s < n; ++s) {
-        OFX_TRY(ns_timestep(c, first_iter + s));
-        OFX_TRY(ns_residual(c));
-        OFX_TRY(ns_lusgs(c));
-        OFX_TRY(ns_update_fused(c));
-        if (c->k.vismodel == 3) {
-            OFX_TRY(sa_residual(c));
-            OFX_TRY(sa_lusgs(c));
-            OFX_TRY(sa_update_fused(c));
-        }
+        OFX_TRY(step_launch(c, first_iter + s));
         if (norms_out) OFX_TRY(ofx_norms_get(c, norms_out + s));
     }
     OFX_HIP(hipStreamSynchronize(c->stream));
@@ -395,6 +410,47 @@ int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
+}
+
+int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
+    CHECK_CTX(c);
+    auto ok = [](int t) { return t == 64 || t == 128 || t == 256 || t == 512; };
+    if (!ok(ns_tile) || !ok(sa_tile)) { set_error("face tile must be 64, 128, 256 or 512"); return OFX_EINVAL; }
+    c->face_tb = ns_tile;
+    c->saface_tb = sa_tile;
+    ++c->gen;
+    return 0;
+}
+
+int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
+    CHECK_READY(c);
+    if (iters < 1) { set_error("autotune: iters >= 1"); return OFX_EINVAL; }
+    static const int cand[4] = {64, 128, 256, 512};
+    double best = 1e300, ms = 0;
+    int pick = c->face_tb;
+    for (int t : cand) {
+        c->face_tb = t;
+        OFX_TRY(ofx_time_kernel(c, 2, iters, &ms));
+        if (ms < best) { best = ms; pick = t; }
+    }
+    c->face_tb = pick;
+    if (c->k.vismodel == 3) {
+        best = 1e300;
+        pick = c->saface_tb;
+        for (int t : cand) {
+            c->saface_tb = t;
+            OFX_TRY(ofx_time_kernel(c, 5, iters, &ms));
+            if (ms < best) { best = ms; pick = t; }
+        }
+        c->saface_tb = pick;
+    }
+    ++c->gen;
+    // leave the residual fields as one evaluation of the current state leaves them
+    OFX_TRY(ns_residual(c));
+    if (c->k.vismodel == 3) OFX_TRY(sa_residual(c));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    if (chosen2) { chosen2[0] = c->face_tb; chosen2[1] = c->saface_tb; }
+    return 0;
 }
 
 // Bandwidth reference: one 16-byte load and store per thread.

@@ -113,6 +113,8 @@ struct ofx_ctx {
     ofx::DevFields d;
     std::vector<void*> allocs;
     int par_dt = 0, par_lim = 0, par_fix = 0;  // parity of the reduction buffers in use
+    int face_tb = 256, saface_tb = 256;        // face-kernel tile (threads per block)
+    int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
 };
 
 #define OFX_HIP(call)                                                              \
@@ -249,6 +251,7 @@ int sa_boundary(ofx_ctx* c);
 int ns_timestep(ofx_ctx* c, int iter);
 int ns_residual(ofx_ctx* c);
 int ns_residual_parts(ofx_ctx* c, unsigned parts);
+int step_launch(ofx_ctx* c, int iter);
 int sa_residual(ofx_ctx* c);
 int ns_lusgs(ofx_ctx* c);
 int sa_lusgs(ofx_ctx* c);

@@ -1,0 +1,584 @@
+// ofx_mlp.hip -- the warm-start predictor: a BF16 multilayer perceptron
+// (inputs 9 -> 5 hidden layers of 256 with SiLU -> 5 outputs) trained and
+// evaluated with hand-written MFMA kernels on the GPU.  No counterpart in the
+// reference: it predicts a starting flow field, and the solver then iterates
+// from it to the same tolerance as from freestream.
+//
+// Numerics: weights are kept in FP32 (Adam master copy) and used in BF16;
+// activations between layers are BF16; every product is accumulated in FP32
+// by v_mfma_f32_16x16x16_bf16.  Pre-activations are kept in FP32 for the
+// backward pass.
+//
+// Layout: activations row-major [samples][features]; weights [in][out]
+// row-major.  Feature counts are padded: input 9 -> 16, output 5 -> 64 (the
+// padding weights are zero and stay zero: their gradients are exactly zero);
+// the sample count is padded to a multiple of 64 with zero rows that carry no
+// loss.
+#include "ofx_common.h"
+
+namespace ofx {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+constexpr int NIN = 9, NIN_P = 16, NHID = 256, NLAYER = 6, NOUT = 5, NOUT_P = 64;
+constexpr int MLP_MAXN = 1 << 20;  // samples per call
+constexpr int SPLITK_CHUNK = 1024;
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+    float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+enum { EPI_PARTIAL = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3 };
+
+// C[M x N] = A[M x K] B[K x N]; A, B bf16 row-major (lda, ldb); M and N
+// multiples of 64; blockIdx.z selects the K range [z*kchunk, (z+1)*kchunk)
+// (kchunk a multiple of 16).  256 threads per 64 x 64 tile; wave w owns rows
+// 32*(w>>1).. and columns 32*(w&1).. as 2 x 2 MFMA 16x16x16 tiles.  MFMA
+// operands: lane l supplies A[l%16][4*(l/16)+0..3] and B[4*(l/16)+0..3][l%16]
+// and receives D[4*(l/16)+r][l%16], r = 0..3.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+                                              int kchunk, int M, float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc,
+                                              const float* __restrict__ bias, const float* __restrict__ zin,
+                                              float* __restrict__ zout) {
+    __shared__ __align__(16) bf16 sA[64][16];
+    __shared__ __align__(16) bf16 sB[64][16];  // transposed: [n][k]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int kb = blockIdx.z * kchunk;
+    v4f acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kb; k0 < kb + kchunk; k0 += 16) {
+        {   // A tile 64 x 16: thread t loads row t/4, k 4*(t%4)..+3
+            int r = t >> 2, kq = t & 3;
+            *(v4s*)&sA[r][4 * kq] = *(const v4s*)&A[(size_t)(m0 + r) * lda + k0 + 4 * kq];
+            // B tile 16 x 64: thread t loads k t/16, n 4*(t%16)..+3, stores transposed
+            int kk = t >> 4, nq = t & 15;
+            v4s v = *(const v4s*)&B[(size_t)(k0 + kk) * ldb + n0 + 4 * nq];
+            sB[4 * nq + 0][kk] = (bf16)v.x;
+            sB[4 * nq + 1][kk] = (bf16)v.y;
+            sB[4 * nq + 2][kk] = (bf16)v.z;
+            sB[4 * nq + 3][kk] = (bf16)v.w;
+        }
+        __syncthreads();
+        v4s a[2], b[2];
+        for (int i = 0; i < 2; ++i) a[i] = *(const v4s*)&sA[32 * (w >> 1) + 16 * i + (lane & 15)][4 * (lane >> 4)];
+        for (int j = 0; j < 2; ++j) b[j] = *(const v4s*)&sB[32 * (w & 1) + 16 * j + (lane & 15)][4 * (lane >> 4)];
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                int row = m0 + 32 * (w >> 1) + 16 * i + 4 * (lane >> 4) + r;
+                int col = n0 + 32 * (w & 1) + 16 * j + (lane & 15);
+                size_t o = (size_t)row * ldc + col;
+                float v = acc[i][j][r];
+                if (EPI == EPI_PARTIAL) {
+                    Cf[(size_t)blockIdx.z * M * ldc + o] = v;
+                } else if (EPI == EPI_BIAS_SILU) {
+                    float z = v + bias[col];
+                    zout[o] = z;
+                    Cb[o] = f2bf(silu(z));
+                } else if (EPI == EPI_BIAS) {
+                    Cf[o] = v + bias[col];
+                } else {  // EPI_DSILU: gradient through the activation
+                    float g = v * dsilu(zin[o]);
+                    Cf[o] = g;
+                    Cb[o] = f2bf(g);
+                }
+            }
+}
+
+// out[c][r] = in[r][c] for an R x Cc bf16 matrix; rows R..Rp-1 of the output's
+// transposed shape (columns of `in` beyond Cc) are zero-filled up to Cp.
+__global__ void k_transpose(const bf16* in, int R, int Cc, bf16* out, int Cp) {
+    __shared__ bf16 tile[32][33];
+    int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        int r = by + k, c = bx + threadIdx.x;
+        tile[k][threadIdx.x] = (r < R && c < Cc) ? in[(size_t)r * Cc + c] : (bf16)0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        int c = bx + k, r = by + threadIdx.x;
+        if (c < Cp && r < R) out[(size_t)c * R + r] = tile[threadIdx.x][k];
+    }
+}
+
+// Sum of the split-K partials in split order.
+__global__ void k_splitk_sum(const float* part, int nsplit, size_t n, float* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * n + i];
+    out[i] = s;
+}
+
+// Column sums of an M x N fp32 matrix (bias gradients), one thread per column.
+__global__ void k_colsum(const float* g, int M, int N, float* out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float s = 0.f;
+    for (int r = 0; r < M; ++r) s += g[(size_t)r * N + c];
+    out[c] = s;
+}
+
+// Inputs fp32 [n][NIN] -> bf16 [Mp][NIN_P] (zero padded).
+__global__ void k_pack_inputs(const float* x, int n, int Mp, bf16* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Mp * NIN_P) return;
+    int r = i / NIN_P, c = i % NIN_P;
+    out[i] = (r < n && c < NIN) ? f2bf(x[(size_t)r * NIN + c]) : (bf16)0;
+}
+
+// MSE over the real outputs: loss partials and the bf16 / fp32 output gradient.
+__global__ void k_mse(const float* y, const float* tgt, int n, int Mp, bf16* dyb, float* dyf, float* lpart) {
+    __shared__ float sm[256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < Mp * NOUT_P) {
+        int r = i / NOUT_P, c = i % NOUT_P;
+        float g = 0.f;
+        if (r < n && c < NOUT) {
+            float d = y[i] - tgt[(size_t)r * NOUT + c];
+            l = d * d;
+            g = 2.0f * d / (float)(n * NOUT);
+        }
+        dyb[i] = f2bf(g);
+        dyf[i] = g;
+    }
+    sm[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = sm[0];
+}
+
+__global__ void k_sum_partials(const float* p, int n, float scale, float* out) {
+    if (threadIdx.x || blockIdx.x) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += p[i];
+    out[0] = (float)(s * scale);
+}
+
+// Adam on fp32 master weights, bf16 copy refreshed.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+// Mini-batch rows idx[0..b) of the device training set into the input/target buffers.
+__global__ void k_gather_batch(const float* x, const float* t, const int* idx, int b, float* xo, float* to) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b) return;
+    int r = idx[i];
+    for (int c = 0; c < NIN; ++c) xo[(size_t)i * NIN + c] = x[(size_t)r * NIN + c];
+    for (int c = 0; c < NOUT; ++c) to[(size_t)i * NOUT + c] = t[(size_t)r * NOUT + c];
+}
+
+__global__ void k_copy1(const float* a, float* b) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) b[0] = a[0];
+}
+
+__global__ void k_to_bf16(const float* a, size_t n, bf16* b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+}  // namespace ofx
+
+using namespace ofx;
+
+struct ofx_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int kin[NLAYER], kout[NLAYER];        // padded shapes of each layer
+    float *w[NLAYER], *b[NLAYER];          // fp32 master
+    bf16 *wb[NLAYER], *wt[NLAYER];         // bf16 copy and its transpose
+    float *mw[NLAYER], *vw[NLAYER], *mb[NLAYER], *vb[NLAYER];
+    float *gw[NLAYER], *gb[NLAYER];        // gradients
+    int step = 0;
+    // per-call buffers, sized for `cap` samples
+    int cap = 0, max_split = 1;
+    bf16* h[NLAYER];   // layer inputs: h[0] = packed inputs, h[l] = silu(z[l-1])
+    float* z[NLAYER - 1];
+    float* y = nullptr;
+    float* tgt = nullptr;
+    float* xin = nullptr;
+    bf16 *dyb = nullptr, *ht = nullptr;
+    float *dyf = nullptr, *gz = nullptr;  // fp32 gradient of the current layer's pre-activation
+    bf16 *gzb = nullptr, *gzb2 = nullptr;
+    float *part = nullptr, *lpart = nullptr, *loss = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int mlp_alloc(ofx_mlp* n, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) { set_error("mlp: out of device memory"); return OFX_ENOMEM; }
+    n->allocs.push_back(*p);
+    OFX_HIP(hipMemsetAsync(*p, 0, bytes, n->stream));
+    return 0;
+}
+
+int mlp_reserve(ofx_mlp* n, int samples) {
+    int Mp = (samples + 63) / 64 * 64;
+    if (Mp <= n->cap) return 0;
+    // grow: drop the old per-call buffers (parameters are the first allocations)
+    OFX_HIP(hipStreamSynchronize(n->stream));
+    size_t keep = NLAYER * 12;
+    for (size_t i = keep; i < n->allocs.size(); ++i) (void)hipFree(n->allocs[i]);
+    n->allocs.resize(std::min(keep, n->allocs.size()));
+    auto A = [&](void** p, size_t bytes) { return mlp_alloc(n, p, bytes); };
+    for (int l = 0; l < NLAYER; ++l) OFX_TRY(A((void**)&n->h[l], sizeof(bf16) * (size_t)Mp * n->kin[l]));
+    for (int l = 0; l < NLAYER - 1; ++l) OFX_TRY(A((void**)&n->z[l], sizeof(float) * (size_t)Mp * NHID));
+    OFX_TRY(A((void**)&n->y, sizeof(float) * (size_t)Mp * NOUT_P));
+    OFX_TRY(A((void**)&n->tgt, sizeof(float) * (size_t)Mp * NOUT));
+    OFX_TRY(A((void**)&n->xin, sizeof(float) * (size_t)Mp * NIN));
+    OFX_TRY(A((void**)&n->dyb, sizeof(bf16) * (size_t)Mp * NOUT_P));
+    OFX_TRY(A((void**)&n->dyf, sizeof(float) * (size_t)Mp * NOUT_P));
+    OFX_TRY(A((void**)&n->ht, sizeof(bf16) * (size_t)Mp * NHID));
+    OFX_TRY(A((void**)&n->gz, sizeof(float) * (size_t)Mp * NHID));
+    OFX_TRY(A((void**)&n->gzb, sizeof(bf16) * (size_t)Mp * NHID));
+    OFX_TRY(A((void**)&n->gzb2, sizeof(bf16) * (size_t)Mp * NHID));
+    // weight_grad doubles the split count while a split exceeds SPLITK_CHUNK
+    // samples, so it stays below 2 Mp / SPLITK_CHUNK; one more slot holds the sum
+    n->max_split = 2 * (Mp / SPLITK_CHUNK) + 1;
+    OFX_TRY(A((void**)&n->part, sizeof(float) * (size_t)(n->max_split + 1) * NHID * NHID));
+    OFX_TRY(A((void**)&n->lpart, sizeof(float) * ((size_t)Mp * NOUT_P / 256 + 1)));
+    OFX_TRY(A((void**)&n->loss, sizeof(float) * 4));
+    n->cap = Mp;
+    return 0;
+}
+
+template <int EPI>
+int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, int nsplit, float* Cf, bf16* Cb,
+         int ldc, const float* bias, const float* zin, float* zout) {
+    if (M % 64 || N % 64 || K % 16 || K % nsplit || (K / nsplit) % 16) {
+        set_error("mlp: GEMM shape not tiled");
+        return OFX_EINVAL;
+    }
+    dim3 grid(N / 64, M / 64, nsplit);
+    k_gemm<EPI><<<grid, 256, 0, n->stream>>>(A, lda, B, ldb, K / nsplit, M, Cf, Cb, ldc, bias, zin, zout);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+int transpose(ofx_mlp* n, const bf16* in, int R, int C, bf16* out, int Cp) {
+    dim3 grid((std::max(C, Cp) + 31) / 32, (R + 31) / 32);
+    k_transpose<<<grid, dim3(32, 8), 0, n->stream>>>(in, R, C, out, Cp);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// Weight gradient dW[kin x kout] = H^T[kin x Mp] G[Mp x kout], split over samples.
+int weight_grad(ofx_mlp* n, int l, const bf16* H, const bf16* G, int Mp) {
+    int ki = n->kin[l], ko = n->kout[l];
+    int rows = (ki + 63) / 64 * 64;  // the input layer's 16 rows padded to one tile
+    OFX_TRY(transpose(n, H, Mp, ki, n->ht, rows));
+    int nsplit = 1;
+    while (Mp / nsplit > SPLITK_CHUNK && Mp % (nsplit * 2) == 0 && (Mp / (nsplit * 2)) % 16 == 0) nsplit *= 2;
+    if (nsplit > n->max_split || Mp > n->cap) { set_error("mlp: split-K buffer too small"); return OFX_EINVAL; }
+    OFX_TRY(gemm<EPI_PARTIAL>(n, n->ht, Mp, G, ko, rows, ko, Mp, nsplit, n->part, nullptr, ko, nullptr, nullptr, nullptr));
+    size_t cnt = (size_t)rows * ko;
+    // rows beyond kin are zero inputs (their sums are zero); keep the first kin rows
+    k_splitk_sum<<<(unsigned)((cnt + 255) / 256), 256, 0, n->stream>>>(n->part, nsplit, cnt, n->part + (size_t)nsplit * cnt);
+    OFX_HIP(hipGetLastError());
+    OFX_HIP(hipMemcpyAsync(n->gw[l], n->part + (size_t)nsplit * cnt, sizeof(float) * (size_t)ki * ko,
+                           hipMemcpyDeviceToDevice, n->stream));
+    return 0;
+}
+
+int forward(ofx_mlp* n, int Mp) {
+    for (int l = 0; l < NLAYER - 1; ++l)
+        OFX_TRY(gemm<EPI_BIAS_SILU>(n, n->h[l], n->kin[l], n->wb[l], n->kout[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
+                                    n->h[l + 1], n->kout[l], n->b[l], nullptr, n->z[l]));
+    int L = NLAYER - 1;
+    OFX_TRY(gemm<EPI_BIAS>(n, n->h[L], n->kin[L], n->wb[L], n->kout[L], Mp, n->kout[L], n->kin[L], 1, n->y, nullptr,
+                           n->kout[L], n->b[L], nullptr, nullptr));
+    return 0;
+}
+
+int backward(ofx_mlp* n, int nsamp, int Mp) {
+    int L = NLAYER - 1;
+    int nb = (Mp * NOUT_P + 255) / 256;
+    k_mse<<<nb, 256, 0, n->stream>>>(n->y, n->tgt, nsamp, Mp, n->dyb, n->dyf, n->lpart);
+    k_sum_partials<<<1, 1, 0, n->stream>>>(n->lpart, nb, 1.0f / (float)(nsamp * NOUT), n->loss);
+    OFX_HIP(hipGetLastError());
+    // output layer
+    OFX_TRY(weight_grad(n, L, n->h[L], n->dyb, Mp));
+    k_colsum<<<1, 64, 0, n->stream>>>(n->dyf, Mp, NOUT_P, n->gb[L]);
+    const bf16* g = n->dyb;
+    bf16* gnext = n->gzb;
+    for (int l = L; l >= 1; --l) {
+        // gradient of layer l-1's pre-activation: (G W_l^T) * silu'(z_{l-1})
+        OFX_TRY(transpose(n, n->wb[l], n->kin[l], n->kout[l], n->wt[l], n->kout[l]));
+        OFX_TRY(gemm<EPI_DSILU>(n, g, n->kout[l], n->wt[l], n->kin[l], Mp, n->kin[l], n->kout[l], 1, n->gz, gnext,
+                                n->kin[l], nullptr, n->z[l - 1], nullptr));
+        OFX_TRY(weight_grad(n, l - 1, n->h[l - 1], gnext, Mp));
+        k_colsum<<<(NHID + 63) / 64, 64, 0, n->stream>>>(n->gz, Mp, NHID, n->gb[l - 1]);
+        OFX_HIP(hipGetLastError());
+        g = gnext;
+        gnext = (gnext == n->gzb) ? n->gzb2 : n->gzb;
+    }
+    return 0;
+}
+
+int load_batch(ofx_mlp* n, const float* x, const float* t, int nsamp, bool on_device) {
+    OFX_TRY(mlp_reserve(n, nsamp));
+    int Mp = (nsamp + 63) / 64 * 64;
+    hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    OFX_HIP(hipMemcpyAsync(n->xin, x, sizeof(float) * (size_t)nsamp * NIN, kind, n->stream));
+    if (t) OFX_HIP(hipMemcpyAsync(n->tgt, t, sizeof(float) * (size_t)nsamp * NOUT, kind, n->stream));
+    k_pack_inputs<<<(Mp * NIN_P + 255) / 256, 256, 0, n->stream>>>(n->xin, nsamp, Mp, n->h[0]);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ofx_mlp_create(ofx_mlp** out, int device, unsigned long long seed) {
+    if (!out) { set_error("null out"); return OFX_EINVAL; }
+    *out = nullptr;
+    ofx_mlp* n = new (std::nothrow) ofx_mlp();
+    if (!n) { set_error("out of host memory"); return OFX_ENOMEM; }
+    n->device = device;
+    int rc = 0;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("mlp: no such HIP device");
+        delete n;
+        return OFX_EHIP;
+    }
+    for (int l = 0; l < NLAYER; ++l) {
+        n->kin[l] = (l == 0) ? NIN_P : NHID;
+        n->kout[l] = (l == NLAYER - 1) ? NOUT_P : NHID;
+    }
+    // parameters first (mlp_reserve keeps the first NLAYER * 12 allocations)
+    std::vector<float> hw, hb;
+    uint64_t s = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto urand = [&]() {  // uniform in [-1, 1)
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        return (float)((double)(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+    };
+    for (int l = 0; l < NLAYER && !rc; ++l) {
+        size_t nw = (size_t)n->kin[l] * n->kout[l];
+        int fan_in = (l == 0) ? NIN : NHID;
+        int real_out = (l == NLAYER - 1) ? NOUT : NHID;
+        float a = std::sqrt(6.0f / (float)fan_in);  // uniform He-style initialisation
+        hw.assign(nw, 0.0f);
+        for (int i = 0; i < ((l == 0) ? NIN : NHID); ++i)
+            for (int j = 0; j < real_out; ++j) hw[(size_t)i * n->kout[l] + j] = a * urand();
+        hb.assign(n->kout[l], 0.0f);
+        void* p;
+        rc = mlp_alloc(n, &p, sizeof(float) * nw); n->w[l] = (float*)p;
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * n->kout[l]); n->b[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(bf16) * nw); n->wb[l] = (bf16*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(bf16) * nw); n->wt[l] = (bf16*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * nw); n->mw[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * nw); n->vw[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * n->kout[l]); n->mb[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * n->kout[l]); n->vb[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * nw); n->gw[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, sizeof(float) * n->kout[l]); n->gb[l] = (float*)p; }
+        if (!rc) { rc = mlp_alloc(n, &p, 16); }  // spare slots keep the count at 12 per layer
+        if (!rc) { rc = mlp_alloc(n, &p, 16); }
+        if (!rc && hipMemcpyAsync(n->w[l], hw.data(), sizeof(float) * nw, hipMemcpyHostToDevice, n->stream) != hipSuccess)
+            rc = OFX_EHIP;
+        if (!rc) {
+            k_to_bf16<<<(unsigned)((nw + 255) / 256), 256, 0, n->stream>>>(n->w[l], nw, n->wb[l]);
+            if (hipGetLastError() != hipSuccess) rc = OFX_EHIP;
+        }
+        if (!rc && hipStreamSynchronize(n->stream) != hipSuccess) rc = OFX_EHIP;
+    }
+    if (rc) {
+        ofx_mlp_destroy(n);
+        return rc;
+    }
+    *out = n;
+    return 0;
+}
+
+int ofx_mlp_destroy(ofx_mlp* n) {
+    if (!n) return 0;
+    (void)hipSetDevice(n->device);
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    for (void* p : n->allocs) (void)hipFree(p);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+    return 0;
+}
+
+int ofx_mlp_param_count(ofx_mlp* n, int* count) {
+    if (!n || !count) { set_error("null argument"); return OFX_EINVAL; }
+    int c = 0;
+    for (int l = 0; l < NLAYER; ++l) c += n->kin[l] * n->kout[l] + n->kout[l];
+    *count = c;
+    return 0;
+}
+
+// Parameters (or gradients) as one fp32 vector: for each layer W[kin_p][kout_p]
+// then b[kout_p] (padded shapes: 16/256/.../256 -> 256/.../64).
+static int mlp_copy(ofx_mlp* n, float* host, bool grads, bool to_device) {
+    size_t o = 0;
+    for (int l = 0; l < NLAYER; ++l) {
+        size_t nw = (size_t)n->kin[l] * n->kout[l];
+        float* dw = grads ? n->gw[l] : n->w[l];
+        float* db = grads ? n->gb[l] : n->b[l];
+        if (to_device) {
+            OFX_HIP(hipMemcpyAsync(dw, host + o, sizeof(float) * nw, hipMemcpyHostToDevice, n->stream));
+            OFX_HIP(hipMemcpyAsync(db, host + o + nw, sizeof(float) * n->kout[l], hipMemcpyHostToDevice, n->stream));
+            if (!grads) k_to_bf16<<<(unsigned)((nw + 255) / 256), 256, 0, n->stream>>>(n->w[l], nw, n->wb[l]);
+        } else {
+            OFX_HIP(hipMemcpyAsync(host + o, dw, sizeof(float) * nw, hipMemcpyDeviceToHost, n->stream));
+            OFX_HIP(hipMemcpyAsync(host + o + nw, db, sizeof(float) * n->kout[l], hipMemcpyDeviceToHost, n->stream));
+        }
+        o += nw + n->kout[l];
+    }
+    OFX_HIP(hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+int ofx_mlp_get_params(ofx_mlp* n, float* host) {
+    if (!n || !host) { set_error("null argument"); return OFX_EINVAL; }
+    return mlp_copy(n, host, false, false);
+}
+
+int ofx_mlp_set_params(ofx_mlp* n, const float* host) {
+    if (!n || !host) { set_error("null argument"); return OFX_EINVAL; }
+    n->step = 0;
+    return mlp_copy(n, const_cast<float*>(host), false, true);
+}
+
+int ofx_mlp_forward(ofx_mlp* n, const float* x, int nsamp, float* y) {
+    if (!n || !x || !y || nsamp < 1 || nsamp > MLP_MAXN) { set_error("mlp_forward: bad arguments"); return OFX_EINVAL; }
+    OFX_HIP(hipSetDevice(n->device));
+    OFX_TRY(load_batch(n, x, nullptr, nsamp, false));
+    int Mp = (nsamp + 63) / 64 * 64;
+    OFX_TRY(forward(n, Mp));
+    OFX_HIP(hipMemcpy2DAsync(y, sizeof(float) * NOUT, n->y, sizeof(float) * NOUT_P, sizeof(float) * NOUT, nsamp,
+                             hipMemcpyDeviceToHost, n->stream));
+    OFX_HIP(hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+int ofx_mlp_gradients(ofx_mlp* n, const float* x, const float* t, int nsamp, float* loss, float* grads) {
+    if (!n || !x || !t || nsamp < 1 || nsamp > MLP_MAXN) { set_error("mlp_gradients: bad arguments"); return OFX_EINVAL; }
+    OFX_HIP(hipSetDevice(n->device));
+    OFX_TRY(load_batch(n, x, t, nsamp, false));
+    int Mp = (nsamp + 63) / 64 * 64;
+    OFX_TRY(forward(n, Mp));
+    OFX_TRY(backward(n, nsamp, Mp));
+    if (loss) OFX_HIP(hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    OFX_HIP(hipStreamSynchronize(n->stream));
+    if (grads) OFX_TRY(mlp_copy(n, grads, true, false));
+    return 0;
+}
+
+static int train_step(ofx_mlp* n, int nsamp, float lr) {
+    int Mp = (nsamp + 63) / 64 * 64;
+    OFX_TRY(forward(n, Mp));
+    OFX_TRY(backward(n, nsamp, Mp));
+    ++n->step;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float c1 = 1.0f / (1.0f - std::pow(b1, (float)n->step));
+    float c2 = 1.0f / (1.0f - std::pow(b2, (float)n->step));
+    for (int l = 0; l < NLAYER; ++l) {
+        size_t nw = (size_t)n->kin[l] * n->kout[l];
+        k_adam<<<(unsigned)((nw + 255) / 256), 256, 0, n->stream>>>(n->w[l], n->gw[l], n->mw[l], n->vw[l], n->wb[l], nw,
+                                                                   lr, b1, b2, eps, c1, c2);
+        k_adam<<<(n->kout[l] + 255) / 256, 256, 0, n->stream>>>(n->b[l], n->gb[l], n->mb[l], n->vb[l], nullptr,
+                                                               (size_t)n->kout[l], lr, b1, b2, eps, c1, c2);
+    }
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+int ofx_mlp_train(ofx_mlp* n, const float* x, const float* t, int nsamp, int batch, int steps, float lr,
+                  unsigned long long seed, float* loss_out) {
+    if (!n || !x || !t || nsamp < 1 || batch < 64 || batch > MLP_MAXN || steps < 1) {
+        set_error("mlp_train: bad arguments");
+        return OFX_EINVAL;
+    }
+    OFX_HIP(hipSetDevice(n->device));
+    int bsz = std::min(batch, nsamp);
+    OFX_TRY(mlp_reserve(n, bsz));
+    // training set, epoch permutation and per-step losses live on the device
+    float *dx = nullptr, *dt = nullptr, *dl = nullptr;
+    int* dperm = nullptr;
+    int rc = 0;
+    if (hipMalloc(&dx, sizeof(float) * (size_t)nsamp * NIN) != hipSuccess ||
+        hipMalloc(&dt, sizeof(float) * (size_t)nsamp * NOUT) != hipSuccess ||
+        hipMalloc(&dperm, sizeof(int) * (size_t)nsamp) != hipSuccess ||
+        hipMalloc(&dl, sizeof(float) * (size_t)steps) != hipSuccess) {
+        set_error("mlp_train: out of device memory");
+        rc = OFX_ENOMEM;
+    }
+    if (!rc && (hipMemcpyAsync(dx, x, sizeof(float) * (size_t)nsamp * NIN, hipMemcpyHostToDevice, n->stream) != hipSuccess ||
+                hipMemcpyAsync(dt, t, sizeof(float) * (size_t)nsamp * NOUT, hipMemcpyHostToDevice, n->stream) != hipSuccess))
+        rc = OFX_EHIP;
+    std::vector<int> perm(nsamp);
+    for (int i = 0; i < nsamp; ++i) perm[i] = i;
+    uint64_t s = seed * 2862933555777941757ull + 3037000493ull;
+    int pos = nsamp;
+    int Mp = (bsz + 63) / 64 * 64;
+    for (int st = 0; st < steps && !rc; ++st) {
+        if (pos + bsz > nsamp) {  // new epoch: reshuffle (Fisher-Yates) and upload
+            for (int i = nsamp - 1; i > 0; --i) {
+                s = s * 6364136223846793005ull + 1442695040888963407ull;
+                int j = (int)((s >> 33) % (uint64_t)(i + 1));
+                std::swap(perm[i], perm[j]);
+            }
+            if (hipStreamSynchronize(n->stream) != hipSuccess ||
+                hipMemcpy(dperm, perm.data(), sizeof(int) * (size_t)nsamp, hipMemcpyHostToDevice) != hipSuccess) {
+                rc = OFX_EHIP;
+                break;
+            }
+            pos = 0;
+        }
+        k_gather_batch<<<(bsz + 255) / 256, 256, 0, n->stream>>>(dx, dt, dperm + pos, bsz, n->xin, n->tgt);
+        k_pack_inputs<<<(Mp * NIN_P + 255) / 256, 256, 0, n->stream>>>(n->xin, bsz, Mp, n->h[0]);
+        if (hipGetLastError() != hipSuccess) { rc = OFX_EHIP; break; }
+        pos += bsz;
+        rc = train_step(n, bsz, lr);
+        if (!rc) k_copy1<<<1, 64, 0, n->stream>>>(n->loss, dl + st);
+    }
+    if (!rc && hipStreamSynchronize(n->stream) != hipSuccess) rc = OFX_EHIP;
+    if (!rc && loss_out && hipMemcpy(loss_out, dl, sizeof(float) * (size_t)steps, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = OFX_EHIP;
+    if (rc == OFX_EHIP) set_error("mlp_train: HIP error");
+    (void)hipStreamSynchronize(n->stream);
+    if (dx) (void)hipFree(dx);
+    if (dt) (void)hipFree(dt);
+    if (dperm) (void)hipFree(dperm);
+    if (dl) (void)hipFree(dl);
+    return rc;
+}
+
+}  // extern "C"

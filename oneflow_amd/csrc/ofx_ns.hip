@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
 // the FP64 reconstruction and both face fluxes are stored as FP32 (the
 // gather accumulates them in FP64).
 template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32>
-__global__ void __launch_bounds__(TB) k_ns_face(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const int T = m.T;
@@ -570,13 +570,14 @@ __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
 
 template <int DIM, int LIM, bool SLAU2>
 static void launch_face(ofx_ctx* c, bool visc) {
-    int nb = nblk(c->m.F);
+    const int tb = c->face_tb;  // face tile, chosen by ofx_autotune
+    int nb = (c->m.F + tb - 1) / tb;
     if (c->k.flux_fp32) {
-        if (visc) k_ns_face<DIM, LIM, SLAU2, true, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
-        else k_ns_face<DIM, LIM, SLAU2, false, true><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+        if (visc) k_ns_face<DIM, LIM, SLAU2, true, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
+        else k_ns_face<DIM, LIM, SLAU2, false, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
     } else {
-        if (visc) k_ns_face<DIM, LIM, SLAU2, true, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
-        else k_ns_face<DIM, LIM, SLAU2, false, false><<<nb, TB, 0, c->stream>>>(c->m, c->d, c->k);
+        if (visc) k_ns_face<DIM, LIM, SLAU2, true, false><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
+        else k_ns_face<DIM, LIM, SLAU2, false, false><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
     }
 }
 

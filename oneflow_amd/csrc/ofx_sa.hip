@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
 // (UTurbVisFlux::CalcFaceVisFlux1Equ, codes/uturb/src/UTurbVisFlux.cpp:247-369).
 // saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
 template <int DIM>
-__global__ void __launch_bounds__(TBS) k_sa_face(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const int T = m.T, F = m.F;
@@ -345,7 +345,7 @@ template <int DIM, int MAXF>
 static void launch_sa(ofx_ctx* c) {
     const DevMesh& m = c->m;
     k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
-    k_sa_face<DIM><<<nblks(m.F), TBS, 0, c->stream>>>(m, c->d, c->k);
+    k_sa_face<DIM><<<(m.F + c->saface_tb - 1) / c->saface_tb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k);
     k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
 }
 

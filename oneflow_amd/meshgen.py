@@ -283,12 +283,28 @@ def naca_hybrid(n_surf: int = 512, n_quad: int = 40, ratio: float = 1.15, reynol
     X = np.zeros((nl, ns))
     Y = np.zeros((nl, ns))
     X[0], Y[0] = xs, ys
+    # a smoothing pass reaches one surface spacing: scale the passes with the
+    # square of the point density so the physical smoothing length is the same
+    dens = max(1.0, (ns / 512.0) ** 2)
+    def layer_ok(xa, ya, xb, yb):
+        # both triangles (a, b, c), (a, c, d) of every cell between two layers positive
+        bx, by = np.roll(xa, -1), np.roll(ya, -1)
+        cx, cy = np.roll(xb, -1), np.roll(yb, -1)
+        t1 = (bx - xa) * (cy - ya) - (cx - xa) * (by - ya)
+        t2 = (cx - xa) * (yb - ya) - (xb - xa) * (cy - ya)
+        return bool(np.all(t1 > 0) and np.all(t2 > 0))
+
     for j in range(1, nl):
-        passes = int(min(400, 4 + 2 * j + (40 * max(0.0, dists[j] - d_last) / radius) * 10))
-        nx, ny = _smooth_normals(X[j - 1], Y[j - 1], passes=passes)
+        passes = int(dens * min(400, 4 + 2 * j + (40 * max(0.0, dists[j] - d_last) / radius) * 10))
         step = dists[j] - dists[j - 1]
-        X[j] = X[j - 1] + step * nx
-        Y[j] = Y[j - 1] + step * ny
+        while True:
+            # more smoothing where a concave stretch would fold the next layer
+            nx, ny = _smooth_normals(X[j - 1], Y[j - 1], passes=passes)
+            X[j] = X[j - 1] + step * nx
+            Y[j] = Y[j - 1] + step * ny
+            if layer_ok(X[j - 1], Y[j - 1], X[j], Y[j]) or passes > 20000:
+                break
+            passes *= 2
     x = X.ravel()
     y = Y.ravel()
     nid = lambda j, i: j * ns + (i % ns)

@@ -64,6 +64,13 @@ def load_library():
     L.ofx_aero.argtypes = [P, dp]
     L.ofx_time_kernel.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
     L.ofx_copy_bench.argtypes = [P, ctypes.c_size_t, ctypes.c_int, dp]
+    L.ofx_set_face_tile.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.ofx_autotune.argtypes = [P, ctypes.c_int, ip]
+    L.ofx_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, ctypes.c_int]
+    L.ofx_batch_destroy.argtypes = [P]
+    L.ofx_batch_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ip]
+    L.ofx_batch_norms.argtypes = [P, ctypes.POINTER(OfxNorms)]
+    L.ofx_batch_autotune.argtypes = [P, ctypes.c_int, ip]
     _lib = L
     return L
 
@@ -188,8 +195,64 @@ class Solver:
         self._check(self._L.ofx_time_kernel(self.h, self.KERNELS[which], iters, _dp(a)))
         return float(a[0])
 
+    def set_face_tile(self, ns_tile, sa_tile):
+        """Threads per block of the NS / SA face-flux kernels (64..512)."""
+        self._check(self._L.ofx_set_face_tile(self.h, ns_tile, sa_tile))
+
+    def autotune(self, iters=20):
+        """Pick the fastest face tiles on the current state; returns (ns_tile, sa_tile)."""
+        a = np.zeros(2, np.int32)
+        self._check(self._L.ofx_autotune(self.h, iters, _ip(a)))
+        return int(a[0]), int(a[1])
+
     def copy_bench(self, nbytes, iters=20):
         """Average ms of a plain device copy of nbytes (bandwidth reference)."""
         a = np.zeros(1)
         self._check(self._L.ofx_copy_bench(self.h, ctypes.c_size_t(nbytes), iters, _dp(a)))
         return float(a[0])
+
+
+class Batch:
+    """Several solvers advanced together: steps past the CFL ramp are recorded
+    once into one HIP graph (one branch per case) and replayed.  Per case the
+    results equal `Solver.iterate` bit for bit."""
+
+    def __init__(self, solvers, steps_per_graph=2):
+        self._L = load_library()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.h for s in self.solvers])
+        self.h = ctypes.c_void_p()
+        rc = self._L.ofx_batch_create(ctypes.byref(self.h), arr, len(self.solvers), steps_per_graph)
+        if rc != 0:
+            raise OfxError(f"ofx_batch_create: {self._L.ofx_last_error().decode()}")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OfxError(f"ofx error {rc}: {self._L.ofx_last_error().decode()}")
+
+    def iterate(self, first_iter, n):
+        """Advance every case n steps; returns the number of graph replays used."""
+        k = np.zeros(1, np.int32)
+        self._check(self._L.ofx_batch_iterate(self.h, first_iter, n, _ip(k)))
+        return int(k[0])
+
+    def norms(self):
+        arr = (OfxNorms * len(self.solvers))()
+        self._check(self._L.ofx_batch_norms(self.h, arr))
+        return np.array([list(a.ns) + [a.sa] for a in arr])
+
+    def autotune(self, iters=20):
+        a = np.zeros(2 * len(self.solvers), np.int32)
+        self._check(self._L.ofx_batch_autotune(self.h, iters, _ip(a)))
+        return a.reshape(-1, 2)
+
+    def close(self):
+        if self.h:
+            self._L.ofx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
