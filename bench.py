@@ -42,10 +42,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = ROOT / "profiles" / "naca0012_c1_hbm_traffic.txt"
 
 
-def kernel_bytes(C, B, F, dim):
+def kernel_bytes(C, B, F, dim, flux_bytes=8):
     """Algorithmic bytes per launch of the NS residual kernels: every array the
-    kernel must read or write counted once (FP64 values, int32 indices).  In 2-D
-    the w component and z coordinates are neither read nor written."""
+    kernel must read or write counted once (FP64 values, int32 indices; the
+    face fluxes are FP32 in the FP32-flux variant, flux_bytes = 4).  In 2-D the
+    w component and z coordinates are neither read nor written."""
     T = C + B
     nnz = 2 * F - B                  # entries of the cell -> face lists
     nq = 4 if dim == 2 else 5        # primitive components carried
@@ -60,9 +61,9 @@ def kernel_bytes(C, B, F, dim):
         # lc/rc, face normal/area/centre, q, centroids, grads (nq + T), limiter,
         # laminar + eddy viscosity -> inviscid and viscous face fluxes
         "face": 8 * F + 8 * (D + 1) * F + 8 * D * F + 8 * nq * T + 8 * D * T + 8 * (nq + 1) * D * T
-                + 8 * nq * T + 16 * T + 2 * 8 * nq * F,
+                + 8 * nq * T + 16 * T + 2 * flux_bytes * nq * F,
         # c2f, both face fluxes -> residual
-        "gather": idx + 2 * 8 * nq * F + 8 * nq * C,
+        "gather": idx + 2 * flux_bytes * nq * F + 8 * nq * C,
     }
 
 
@@ -184,6 +185,129 @@ def batch_main(args):
     b.close()
 
 
+def _sweep_grid_o1(case):
+    return case.grid("multicolor")
+
+
+def warmstart_main(args):
+    """Time to a 1e-8 residual drop on unseen airfoils: the plain solver case
+    after case from freestream, the batched solver from freestream, and the
+    batched solver from the learned warm start.  The network is trained first
+    on fields the solver converges itself (a separate seeded training sweep).
+    Spatial order 1 (ilim = -1) -- the reference's second-order Venkat setting
+    does not converge to 1e-8 on these grids."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from oneflow_amd import warmstart as ws
+    from oneflow_amd.cases import airfoil_sweep
+    from oneflow_amd.mlp import Mlp
+    from oneflow_amd.solver import Batch, Solver
+
+    train = airfoil_sweep(args.n_train, seed=args.seed + 1, min_cells=20_000, max_cells=50_000)
+    test = airfoil_sweep(args.n_test, seed=args.seed, min_cells=20_000, max_cells=60_000)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        grids = list(ex.map(_sweep_grid_o1, train + test))
+    gtrain, gtest = grids[:len(train)], grids[len(train):]
+    log = lambda m: print(f"[warmstart] {m}", file=sys.stderr, flush=True)
+    log(f"grids: train {sum(g.n_cells for g in gtrain)} cells, test {sum(g.n_cells for g in gtest)} cells, "
+        f"{time.perf_counter() - t0:.1f} s")
+    cfg_of = lambda c: c.config(ilim=-1)
+
+    # 1. training fields: the plain solver, batched, a fixed number of steps
+    tr = [Solver(g, cfg_of(c)) for g, c in zip(gtrain, train)]
+    for s in tr:
+        s.init_freestream()
+    t0 = time.perf_counter()
+    b = Batch(tr, steps_per_graph=10)
+    b.iterate(1, 99)
+    done = 0
+    while done < args.train_steps:
+        k = min(5000, args.train_steps - done)
+        b.iterate(100 + done, k)
+        done += k
+        log(f"training fields: {done} steps, {time.perf_counter() - t0:.1f} s")
+    b.close()
+    t_gen = time.perf_counter() - t0
+    X = np.concatenate([ws.cell_features(s, c) for s, c in zip(tr, train)])
+    Y = np.concatenate([ws.cell_targets(s, c, s.cfg) for s, c in zip(tr, train)])
+    for s in tr:
+        s.close()
+    log(f"training fields: {X.shape[0]} samples in {t_gen:.1f} s")
+
+    # 2. train the network on the GPU
+    net = Mlp(seed=args.seed)
+    t0 = time.perf_counter()
+    losses = net.train(X, Y, batch=8192, steps=args.mlp_steps, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    flops_step = 3 * 2 * 8192 * (16 * 256 + 4 * 256 * 256 + 256 * 64)
+    log(f"trained {args.mlp_steps} steps in {t_train:.1f} s, loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
+
+    # 3. unseen cases: reference levels (untimed pre-pass of the plain run)
+    def fresh():
+        out = []
+        for g, c in zip(gtest, test):
+            s = Solver(g, cfg_of(c))
+            s.init_freestream()
+            out.append(s)
+        return out
+    pre = fresh()
+    refs = [float(s.iterate(1, 1000)[:, 0].max()) for s in pre]
+    for s in pre:
+        s.close()
+
+    seq = fresh()
+    prog = lambda i, n: log(f"  case {i} done at step {n}")
+    t_seq, st_seq = ws.solve_to_tolerance(seq, refs, tol=args.tol, batched=False, progress=prog)
+    aero_seq = [s.aero() for s in seq]
+    for s in seq:
+        s.close()
+    log(f"plain, case after case: {t_seq:.1f} s")
+    bp = fresh()
+    t_bp, st_bp = ws.solve_to_tolerance(bp, refs, tol=args.tol, batched=True, progress=prog)
+    for s in bp:
+        s.close()
+    log(f"plain, batched: {t_bp:.1f} s")
+    bl = fresh()
+    t0 = time.perf_counter()
+    for s, c in zip(bl, test):
+        ws.apply_prediction(s, c, s.cfg, net.forward(ws.cell_features(s, c)))
+    t_inf = time.perf_counter() - t0
+    t_bl, st_bl = ws.solve_to_tolerance(bl, refs, tol=args.tol, batched=True, progress=prog)
+    aero_bl = [s.aero() for s in bl]
+    log(f"learned, batched: {t_inf:.2f} s inference + {t_bl:.1f} s")
+    dcl = max(abs(a[0] - b[0]) / max(abs(a[0]), 1e-12) for a, b in zip(aero_seq, aero_bl))
+    dcd = max(abs(a[1] - b[1]) / max(abs(a[1]), 1e-12) for a, b in zip(aero_seq, aero_bl))
+    # network throughput (forward of 2^20 samples, MFMA)
+    xt = np.random.default_rng(0).normal(0, 1, (1 << 20, 9)).astype(np.float32)
+    net.forward(xt)  # sizes the activation buffers
+    t0 = time.perf_counter()
+    net.forward(xt)
+    t_fwd = time.perf_counter() - t0
+    out = {
+        "metric": "time to a 1e-8 residual drop, learned vs plain",
+        "value": t_seq / (t_inf + t_bl),
+        "unit": "x speedup (plain case-after-case / batched learned warm start)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64 solver, bf16 network",
+        "data": "synthetic",
+        "config": {"workload": f"{len(test)} unseen NACA 4-digit airfoils (20k-60k cells), Roe + SA, first order",
+                   "train_cases": len(train), "train_samples": int(X.shape[0]), "tol": args.tol},
+        "seconds": {"plain_sequential": t_seq, "plain_batched": t_bp, "learned_batched": t_bl,
+                    "inference": t_inf, "training_fields": t_gen, "training": t_train},
+        "steps": {"plain": int(np.sum(st_seq)), "plain_batched": int(np.sum(st_bp)), "learned": int(np.sum(st_bl)),
+                  "plain_per_case": [int(v) for v in st_seq], "learned_per_case": [int(v) for v in st_bl]},
+        "speedup": {"batching": t_seq / t_bp, "warm_start": t_bp / (t_inf + t_bl), "total": t_seq / (t_inf + t_bl)},
+        "aero_max_rel_diff": {"CL": dcl, "CD": dcd},
+        "mlp": {"final_loss": float(losses[-50:].mean()), "first_loss": float(losses[:10].mean()),
+                "train_tflops": flops_step * args.mlp_steps / t_train / 1e12,
+                "forward_1M_ms": 1e3 * t_fwd,
+                "forward_tflops": 2.0 * (1 << 20) * (16 * 256 + 4 * 256 * 256 + 256 * 64) / t_fwd / 1e12},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,8 +319,15 @@ def main():
     ap.add_argument("--numbering", default="multicolor", choices=["natural", "multicolor"])
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
-    ap.add_argument("--workload", default="c1", choices=["c1", "batch"],
-                    help="c1: single-grid residual throughput (default); batch: the 64-case sweep")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "batch", "warmstart"],
+                    help="c1: single-grid residual throughput (default); c2: the 3-D swept wing; "
+                         "batch: the 64-case sweep; "
+                         "warmstart: time to 1e-8, learned vs plain")
+    ap.add_argument("--n-train", type=int, default=48)
+    ap.add_argument("--n-test", type=int, default=16)
+    ap.add_argument("--train-steps", type=int, default=30000, help="solver steps for the training fields")
+    ap.add_argument("--mlp-steps", type=int, default=3000)
+    ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cases", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--graph-steps", type=int, default=10, help="steps recorded per graph replay (batch)")
@@ -206,18 +337,58 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return batch_main(args)
+    if args.workload == "warmstart":
+        return warmstart_main(args)
 
+    if args.workload == "c2":
+        return c2_main(args)
     from oneflow_amd.solver import Solver
 
     cfg = c1_config()
     grid = c1_grid(args.numbering)
     print(f"[bench] grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces}", file=sys.stderr, flush=True)
     s = Solver(grid, cfg, device=0)
+    r = residual_bench(s, grid, args)
+    cpu = None
+    if not args.no_cpu:
+        n, dt = cpu_baseline(grid, cfg, args.cpu_budget)
+        cpu = {"value": s.n_tcell * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
+               "sample": f"{n} NS+SA residual evaluations of the same {grid.n_cells}-cell grid, CPU oracle, "
+                         f"1 thread of {os.cpu_count()} host cores, {dt:.1f} s"}
+    out = {
+        "metric": "cell-residual evals/s + % HBM peak; time-to-1e-8 residual, learned vs plain",
+        "value": r["value"],
+        "unit": "cell-residual evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "NACA0012 hybrid quad/tri grid, Roe + SA + Venkat, FP64 residual",
+                   "numbering": args.numbering,
+                   "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": s.n_tcell,
+                   "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
+        "roofline": r["roofline"],
+        "step_roofline": r["step_roofline"],
+        "kernels": r["kernels"],
+        "copy_kernel_gbs": r["copy_kernel_gbs"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def residual_bench(s, grid, args, flux_bytes=8):
+    """Time `args.steps` NS + SA residual evaluations of one solver (after
+    `args.pre_iters` pseudo-steps and `args.warmup` untimed evaluations), then
+    each NS residual kernel over back-to-back launches."""
     s.init_freestream()
     s.iterate(1, args.pre_iters, with_norms=False)
     s.timestep(args.pre_iters + 1)
     T, F = s.n_tcell, s.n_faces
-
     for _ in range(args.warmup):
         s.residual_ns()
         s.residual_sa()
@@ -227,15 +398,12 @@ def main():
         s.residual_ns()
         s.residual_sa()
     s.sync()
-    t1 = time.perf_counter()
-    sec = t1 - t0
+    sec = time.perf_counter() - t0
     ms_per_step = 1e3 * sec / args.steps
-    value = T * args.steps / sec
-
     copy_ms = s.copy_bench(args.copy_bytes, iters=20)
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms * 1e-3) / 1e9
-    kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, 2)
-    step_bytes = 552.0 * T + 64.0 * F
+    kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, grid.dim, flux_bytes)
+    step_bytes = 552.0 * T + 64.0 * F if grid.dim == 2 else 693.0 * T + 64.0 * F
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
     # each NS residual kernel timed with HIP events on the solver's stream over
     # back-to-back launches on the state the timed steps left
@@ -251,38 +419,66 @@ def main():
     dom = max(names, key=lambda k: prof[k])
     ach = kb[dom] / (prof[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(names[dom]),
+                "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(names[dom]) if grid.dim == 2 else None,
                 "kernel": names[dom], "avg_us": 1e3 * prof[dom], "bytes_per_launch": kb[dom],
                 "launches_timed": args.kernel_iters}
+    return {"value": T * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline, "kernels": kernels,
+            "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+            "copy_kernel_gbs": copy_gbs}
 
+
+def c2_config():
+    from oneflow_amd.config import CaseConfig
+    return CaseConfig(dim=3, ischeme=1, ilim=2, ivischeme=0, vismodel=3, mach_ref=0.84, aoa_degree=3.0,
+                      reynolds=1.2e7, centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
+
+
+def c2_grid(n_span=34):
+    from oneflow_amd import meshgen
+    g = meshgen.swept_wing(n_surf=256, n_quad=25, ratio=1.2, reynolds=1.2e7, n_outer=40, radius=20.0,
+                           n_span=n_span, sweep_deg=30.0, taper=0.5, aspect=4.0, t=0.12)
+    return meshgen.renumber(g, meshgen.multicolor_order(g))
+
+
+def c2_main(args):
+    """The 3-D swept wing: NACA0012 section, sweep 30 deg, taper 0.5, AR 4,
+    25 wall prism layers plus tets, 2.52M cells, between symmetry planes;
+    Mach 0.84, AoA 3 deg, Re 1.2e7, Roe + SA + Venkat.  NS + SA residual
+    throughput in FP64 and in the FP32-flux variant."""
+    from oneflow_amd.solver import Solver
+    cfg = c2_config()
+    t0 = time.perf_counter()
+    grid = c2_grid()
+    print(f"[bench] wing: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces} "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    s = Solver(grid, cfg, device=0)
+    r64 = residual_bench(s, grid, args)
+    s.close()
+    print(f"[bench] fp64 {r64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    s = Solver(grid, cfg.replace(flux_fp32=1), device=0)
+    r32 = residual_bench(s, grid, args, flux_bytes=4)
+    s.close()
+    print(f"[bench] fp32 flux {r32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
-        n, dt = cpu_baseline(grid, cfg, args.cpu_budget)
-        cpu = {"value": T * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
-               "sample": f"{n} NS+SA residual evaluations of the same {grid.n_cells}-cell grid, CPU oracle, "
-                         f"1 thread of {os.cpu_count()} host cores, {dt:.1f} s"}
-
+        sample = c2_grid(n_span=2)
+        n, dt = cpu_baseline(sample, cfg, args.cpu_budget)
+        cpu = {"value": sample.n_tcell * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
+               "sample": f"{n} NS+SA residual evaluations of a {sample.n_cells}-cell 2-layer span slice of the "
+                         f"same wing grid, CPU oracle, 1 thread, {dt:.1f} s"}
     out = {
-        "metric": "cell-residual evals/s + % HBM peak; time-to-1e-8 residual, learned vs plain",
-        "value": value,
-        "unit": "cell-residual evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "NACA0012 hybrid quad/tri grid, Roe + SA + Venkat, FP64 residual",
-                   "numbering": args.numbering,
-                   "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": T,
-                   "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
-        "roofline": roofline,
-        "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
-        "kernels": kernels,
-        "copy_kernel_gbs": copy_gbs,
+        "metric": "cell-residual evals/s + % HBM peak (3-D swept wing)",
+        "value": r64["value"], "unit": "cell-residual evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r64["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "swept wing NACA0012, sweep 30, taper 0.5, AR 4, prism layers + tets, "
+                               "Roe + SA + Venkat", "cells": grid.n_cells, "faces": grid.n_faces,
+                   "nTCell": grid.n_tcell, "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree,
+                   "reynolds": cfg.reynolds},
+        "roofline": r64["roofline"], "step_roofline": r64["step_roofline"], "kernels": r64["kernels"],
+        "fp32_flux": {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "roofline": r32["roofline"],
+                      "kernels": r32["kernels"]},
+        "copy_kernel_gbs": r64["copy_kernel_gbs"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -127,12 +127,28 @@ __global__ void k_splitk_sum(const float* part, int nsplit, size_t n, float* out
     out[i] = s;
 }
 
-// Column sums of an M x N fp32 matrix (bias gradients), one thread per column.
-__global__ void k_colsum(const float* g, int M, int N, float* out) {
+// Column sums of an M x N fp32 matrix (bias gradients), two stages in a fixed
+// order: block (x, y) sums columns 64x..64x+63 over rows 256y..256y+255
+// (4 row lanes of 64 rows, then the 4 lanes in order) into part[y][N]; then
+// one thread per column adds the row-chunk partials in order.
+constexpr int COLSUM_ROWS = 256;
+__global__ void __launch_bounds__(256) k_colsum_part(const float* g, int M, int N, float* part) {
+    __shared__ float sm[4][64];
+    int c = blockIdx.x * 64 + (threadIdx.x & 63), lane4 = threadIdx.x >> 6;
+    int r0 = blockIdx.y * COLSUM_ROWS + lane4 * (COLSUM_ROWS / 4);
+    float s = 0.f;
+    if (c < N)
+        for (int r = r0; r < r0 + COLSUM_ROWS / 4 && r < M; ++r) s += g[(size_t)r * N + c];
+    sm[lane4][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < N)
+        part[(size_t)blockIdx.y * N + c] = ((sm[0][threadIdx.x] + sm[1][threadIdx.x]) + sm[2][threadIdx.x]) + sm[3][threadIdx.x];
+}
+__global__ void k_colsum_final(const float* part, int chunks, int N, float* out) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     float s = 0.f;
-    for (int r = 0; r < M; ++r) s += g[(size_t)r * N + c];
+    for (int k = 0; k < chunks; ++k) s += part[(size_t)k * N + c];
     out[c] = s;
 }
 
@@ -169,11 +185,18 @@ __global__ void k_mse(const float* y, const float* tgt, int n, int Mp, bf16* dyb
     if (threadIdx.x == 0) lpart[blockIdx.x] = sm[0];
 }
 
-__global__ void k_sum_partials(const float* p, int n, float scale, float* out) {
-    if (threadIdx.x || blockIdx.x) return;
+// Sum of n partials by one block (strided, then a tree in LDS; fixed order).
+__global__ void __launch_bounds__(256) k_sum_partials(const float* p, int n, float scale, float* out) {
+    __shared__ double sm[256];
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s += p[i];
-    out[0] = (float)(s * scale);
+    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (float)(sm[0] * scale);
 }
 
 // Adam on fp32 master weights, bf16 copy refreshed.
@@ -232,7 +255,7 @@ struct ofx_mlp {
     bf16 *dyb = nullptr, *ht = nullptr;
     float *dyf = nullptr, *gz = nullptr;  // fp32 gradient of the current layer's pre-activation
     bf16 *gzb = nullptr, *gzb2 = nullptr;
-    float *part = nullptr, *lpart = nullptr, *loss = nullptr;
+    float *part = nullptr, *lpart = nullptr, *loss = nullptr, *cpart = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -272,6 +295,7 @@ int mlp_reserve(ofx_mlp* n, int samples) {
     OFX_TRY(A((void**)&n->part, sizeof(float) * (size_t)(n->max_split + 1) * NHID * NHID));
     OFX_TRY(A((void**)&n->lpart, sizeof(float) * ((size_t)Mp * NOUT_P / 256 + 1)));
     OFX_TRY(A((void**)&n->loss, sizeof(float) * 4));
+    OFX_TRY(A((void**)&n->cpart, sizeof(float) * ((size_t)Mp / COLSUM_ROWS * NHID + NHID)));
     n->cap = Mp;
     return 0;
 }
@@ -314,6 +338,15 @@ int weight_grad(ofx_mlp* n, int l, const bf16* H, const bf16* G, int Mp) {
     return 0;
 }
 
+int colsum(ofx_mlp* n, const float* g, int M, int N, float* out) {
+    int chunks = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
+    if ((size_t)chunks * N > (size_t)n->cap / COLSUM_ROWS * NHID + NHID) { set_error("mlp: column-sum buffer too small"); return OFX_EINVAL; }
+    k_colsum_part<<<dim3((N + 63) / 64, chunks), 256, 0, n->stream>>>(g, M, N, n->cpart);
+    k_colsum_final<<<(N + 63) / 64, 64, 0, n->stream>>>(n->cpart, chunks, N, out);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
 int forward(ofx_mlp* n, int Mp) {
     for (int l = 0; l < NLAYER - 1; ++l)
         OFX_TRY(gemm<EPI_BIAS_SILU>(n, n->h[l], n->kin[l], n->wb[l], n->kout[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
@@ -328,11 +361,11 @@ int backward(ofx_mlp* n, int nsamp, int Mp) {
     int L = NLAYER - 1;
     int nb = (Mp * NOUT_P + 255) / 256;
     k_mse<<<nb, 256, 0, n->stream>>>(n->y, n->tgt, nsamp, Mp, n->dyb, n->dyf, n->lpart);
-    k_sum_partials<<<1, 1, 0, n->stream>>>(n->lpart, nb, 1.0f / (float)(nsamp * NOUT), n->loss);
+    k_sum_partials<<<1, 256, 0, n->stream>>>(n->lpart, nb, 1.0f / (float)(nsamp * NOUT), n->loss);
     OFX_HIP(hipGetLastError());
     // output layer
     OFX_TRY(weight_grad(n, L, n->h[L], n->dyb, Mp));
-    k_colsum<<<1, 64, 0, n->stream>>>(n->dyf, Mp, NOUT_P, n->gb[L]);
+    OFX_TRY(colsum(n, n->dyf, Mp, NOUT_P, n->gb[L]));
     const bf16* g = n->dyb;
     bf16* gnext = n->gzb;
     for (int l = L; l >= 1; --l) {
@@ -341,7 +374,7 @@ int backward(ofx_mlp* n, int nsamp, int Mp) {
         OFX_TRY(gemm<EPI_DSILU>(n, g, n->kout[l], n->wt[l], n->kin[l], Mp, n->kin[l], n->kout[l], 1, n->gz, gnext,
                                 n->kin[l], nullptr, n->z[l - 1], nullptr));
         OFX_TRY(weight_grad(n, l - 1, n->h[l - 1], gnext, Mp));
-        k_colsum<<<(NHID + 63) / 64, 64, 0, n->stream>>>(n->gz, Mp, NHID, n->gb[l - 1]);
+        OFX_TRY(colsum(n, n->gz, Mp, NHID, n->gb[l - 1]));
         OFX_HIP(hipGetLastError());
         g = gnext;
         gnext = (gnext == n->gzb) ? n->gzb2 : n->gzb;

@@ -138,13 +138,13 @@ def renumber(g: Grid, order) -> Grid:
     lc = new_of_old[g.lc]
     rc = g.rc.astype(np.int64).copy()
     rc[nb:] = new_of_old[g.rc[nb:]]
-    fn = g.face_nodes.reshape(-1, 2).copy() if g.dim == 2 else None
+    if g.dim == 3:
+        return _renumber_faces_nd(g, lc, rc)
+    fn = g.face_nodes.reshape(-1, 2).copy()
     # interior faces: keep lc < rc (flip node order when swapping) and sort
     li, ri = lc[nb:].copy(), rc[nb:].copy()
     sw = li > ri
     li[sw], ri[sw] = ri[sw].copy(), li[sw].copy()
-    if fn is None:
-        raise NotImplementedError("renumber is implemented for 2-D grids")
     fi = fn[nb:].copy()
     fi[sw] = fi[sw][:, ::-1]
     srt = np.lexsort((ri, li))
@@ -153,6 +153,44 @@ def renumber(g: Grid, order) -> Grid:
     srtb = np.lexsort((lb, g.bc_type))
     out = Grid(dim=g.dim, x=g.x, y=g.y, z=g.z, face_node_count=g.face_node_count.copy(),
                face_nodes=np.concatenate([fb[srtb], fi[srt]]).astype(np.int32).ravel(),
+               lc=np.concatenate([lb[srtb], li[srt]]).astype(np.int32),
+               rc=np.concatenate([-np.ones(nb, np.int64), ri[srt]]).astype(np.int32),
+               bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_type[srtb].copy(), n_cells=g.n_cells)
+    out.check()
+    return out
+
+
+def _renumber_faces_nd(g: Grid, lc, rc) -> Grid:
+    """Face re-sort after a cell renumbering for grids with any face node
+    counts: interior faces keep lc < rc (node order reversed when the sides
+    swap, which flips the normal), boundary faces grouped by type then owner,
+    interior faces sorted by (lc, rc)."""
+    nb, nf = g.n_bfaces, g.n_faces
+    cnt = g.face_node_count.astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(cnt)])
+    width = int(cnt.max())
+    pad = np.full((nf, width), -1, np.int64)
+    col = np.arange(width)[None, :]
+    valid = col < cnt[:, None]
+    pad[valid] = g.face_nodes
+    li, ri = lc[nb:].copy(), rc[nb:].copy()
+    sw = li > ri
+    li[sw], ri[sw] = ri[sw].copy(), li[sw].copy()
+    fi = pad[nb:].copy()
+    if np.any(sw):
+        rows = fi[sw]
+        c = cnt[nb:][sw]
+        rev = np.full_like(rows, -1)
+        for k in np.unique(c):
+            m = c == k
+            rev[m, :k] = rows[m, :k][:, ::-1]
+        fi[sw] = rev
+    srt = np.lexsort((ri, li))
+    lb = lc[:nb]
+    srtb = np.lexsort((lb, g.bc_type))
+    nodes = np.concatenate([pad[:nb][srtb], fi[srt]])
+    fc = np.concatenate([cnt[:nb][srtb], cnt[nb:][srt]]).astype(np.int32)
+    out = Grid(dim=g.dim, x=g.x, y=g.y, z=g.z, face_node_count=fc, face_nodes=nodes[nodes >= 0].astype(np.int32),
                lc=np.concatenate([lb[srtb], li[srt]]).astype(np.int32),
                rc=np.concatenate([-np.ones(nb, np.int64), ri[srt]]).astype(np.int32),
                bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_type[srtb].copy(), n_cells=g.n_cells)
@@ -267,6 +305,8 @@ def naca_hybrid(n_surf: int = 512, n_quad: int = 40, ratio: float = 1.15, reynol
     step0 = d_in[-1] - d_in[-2]
     # solve growth g: step0 * (g^n_outer - 1)/(g-1) = radius - d_last (approximately, by bisection)
     lo, hi = 1.0001, 2.0
+    while step0 * (hi ** n_outer - 1.0) / (hi - 1.0) <= radius:  # few outer layers: widen the bracket
+        hi *= 2.0
     for _ in range(100):
         g = 0.5 * (lo + hi)
         s = step0 * (g ** n_outer - 1.0) / (g - 1.0)
@@ -490,3 +530,163 @@ def box3d(nx: int, ny: int, nz: int, kind: str = "hex", extent=(1.0, 1.0, 1.0), 
                     cells.extend(hex_to_tets(h, p))
     bcf = bc if bc is not None else (lambda xc, yc, zc, n: BC_FARFIELD)
     return grid_from_polyhedra(p[:, 0], p[:, 1], p[:, 2], cells, bcf)
+
+
+# ---------------------------------------------------------------- 3-D, vectorised
+
+def _rows_key(a):
+    """Sort each row (node tuples padded with -1) so equal faces compare equal."""
+    s = np.sort(a, axis=1)
+    return s
+
+
+def grid_from_polyhedra_np(x, y, z, tets, prisms, bc_of_faces):
+    """Vectorised 3-D assembly for large grids.  tets [n,4] and prisms [n,6]
+    (positively oriented, node orders as TET_FACES / PRISM_FACES); cells are
+    numbered tets first, then prisms.  bc_of_faces(nodes[k, 4]) -> bc type per
+    boundary face (nodes padded with -1).  Same conventions as
+    grid_from_polyhedra: interior faces lc < rc with the normal from lc to rc,
+    boundary faces first, grouped by bc type then owner."""
+    x, y, z = (np.asarray(v, np.float64) for v in (x, y, z))
+    tets = np.asarray(tets, np.int64).reshape(-1, 4)
+    prisms = np.asarray(prisms, np.int64).reshape(-1, 6)
+    nt, npr = len(tets), len(prisms)
+    fl, owner = [], []
+    for loc in TET_FACES:
+        f = np.full((nt, 4), -1, np.int64)
+        f[:, :3] = tets[:, list(loc)]
+        fl.append(f)
+        owner.append(np.arange(nt))
+    for loc in PRISM_FACES:
+        f = np.full((npr, 4), -1, np.int64)
+        f[:, :len(loc)] = prisms[:, list(loc)]
+        fl.append(f)
+        owner.append(nt + np.arange(npr))
+    faces = np.concatenate(fl)
+    own = np.concatenate(owner)
+    key = _rows_key(faces)
+    order = np.lexsort((key[:, 3], key[:, 2], key[:, 1], key[:, 0]))
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    start = np.concatenate([[True], ~same])
+    grp = np.cumsum(start) - 1
+    cnt = np.bincount(grp)
+    if np.any(cnt > 2):
+        raise ValueError("non-manifold face")
+    first = order[start]
+    single = cnt == 1
+    # boundary faces
+    bidx = first[single]
+    bnodes = faces[bidx]
+    bown = own[bidx]
+    bct = np.asarray(bc_of_faces(bnodes), np.int32)
+    srt = np.lexsort((bown, bct))
+    bnodes, bown, bct = bnodes[srt], bown[srt], bct[srt]
+    # interior pairs
+    pos = np.nonzero(~single)[0]
+    starts = np.nonzero(start)[0][pos]
+    a, b = order[starts], order[starts + 1]
+    oa, ob = own[a], own[b]
+    lcell = np.where(oa < ob, oa, ob)
+    rcell = np.where(oa < ob, ob, oa)
+    inodes = np.where((oa < ob)[:, None], faces[a], faces[b])
+    srt = np.lexsort((rcell, lcell))
+    lcell, rcell, inodes = lcell[srt], rcell[srt], inodes[srt]
+    allnodes = np.concatenate([bnodes, inodes])
+    fnc = (allnodes >= 0).sum(axis=1).astype(np.int32)
+    fn = allnodes[allnodes >= 0].astype(np.int32)
+    nb = len(bnodes)
+    g = Grid(dim=3, x=x, y=y, z=z, face_node_count=fnc, face_nodes=fn,
+             lc=np.concatenate([bown, lcell]).astype(np.int32),
+             rc=np.concatenate([-np.ones(nb, np.int64), rcell]).astype(np.int32),
+             bc_type=bct, bc_name_id=bct.copy(), n_cells=nt + npr)
+    g.check()
+    return g
+
+
+def swept_wing(n_surf: int = 256, n_quad: int = 25, ratio: float = 1.2, reynolds: float = 1.2e7,
+               n_outer: int = 40, radius: float = 20.0, n_span: int = 34, sweep_deg: float = 30.0,
+               taper: float = 0.5, aspect: float = 4.0, t: float = 0.12):
+    """Tapered swept wing (NACA 4-digit section, root chord 1) between two
+    symmetry planes: the hybrid airfoil section grid of `naca_hybrid` extruded
+    over the semi-span b = aspect (1 + taper) / 4 in n_span layers; each section
+    is scaled by the local chord and shifted by the swept leading edge.  The
+    n_quad wall layers become prism layers (2 prisms per extruded quad, their
+    triangles on the wall-parallel rings), the outer triangles become prisms
+    split into 3 tets.  Cells: n_span * n_surf * (2 n_quad + 6 n_outer).
+    No wing tip: the span ends on the second symmetry plane."""
+    sec = naca_hybrid(n_surf=n_surf, n_quad=n_quad, ratio=ratio, reynolds=reynolds, n_outer=n_outer,
+                      radius=radius, m=0.0, p=0.4, t=t)
+    ns = n_surf + 0  # nodes per layer ring
+    xs2, ys2 = sec.x, sec.y
+    n2 = len(xs2)
+    ns = n2 // (n_quad + n_outer + 1)
+    semi = aspect * (1.0 + taper) / 4.0
+    zk = np.linspace(0.0, semi, n_span + 1)
+    chord = 1.0 - (1.0 - taper) * zk / semi
+    xle = zk * math.tan(math.radians(sweep_deg))
+    X = (xle[:, None] + chord[:, None] * xs2[None, :]).ravel()
+    Y = (chord[:, None] * ys2[None, :]).ravel()
+    Z = np.repeat(zk, n2)
+    nid = lambda k, v: k * n2 + v
+    # 2-D cells of the section: quads (j < n_quad) and triangles, in naca_hybrid's order
+    jj = np.arange(n_quad + n_outer)
+    i = np.arange(ns)
+    lay = lambda j, ii: j * ns + (ii % ns)
+    prisms, tets = [], []
+    P = np.stack([X, Y, Z], axis=1)
+    for k in range(n_span):
+        for j in range(n_quad + n_outer):
+            a, b, c, d = lay(j, i), lay(j, i + 1), lay(j + 1, i + 1), lay(j + 1, i)
+            if j < n_quad:
+                # wall layer: the hex (a, b on ring j; d, c above them on ring j+1)
+                # -> 2 prisms whose triangles lie on the rings, split along the
+                # diagonal u_k - v_{k+1} of each ring quad, u < v by 2-D node id
+                # (the same rule the tets use, so the rings conform)
+                sw = b < a
+                u, v = np.where(sw, b, a), np.where(sw, a, b)
+                uo, vo = np.where(sw, c, d), np.where(sw, d, c)
+                prisms.append(np.stack([nid(k, u), nid(k, v), nid(k + 1, v),
+                                        nid(k, uo), nid(k, vo), nid(k + 1, vo)], axis=1))
+                prisms.append(np.stack([nid(k, u), nid(k + 1, v), nid(k + 1, u),
+                                        nid(k, uo), nid(k + 1, vo), nid(k + 1, uo)], axis=1))
+            else:
+                for tri in ((a, b, c), (a, c, d)):
+                    v = np.sort(np.stack(tri, axis=1), axis=1)  # by global 2-D node id
+                    v0, v1, v2 = v[:, 0], v[:, 1], v[:, 2]
+                    lo, hi = lambda u: nid(k, u), lambda u: nid(k + 1, u)
+                    for tt in ((lo(v0), lo(v1), lo(v2), hi(v2)), (lo(v0), lo(v1), hi(v1), hi(v2)),
+                               (lo(v0), hi(v0), hi(v1), hi(v2))):
+                        tets.append(np.stack(tt, axis=1))
+    prisms = np.concatenate(prisms)
+    tets = np.concatenate(tets)
+    # orientation: positive volumes
+    def fix(cells, swap):
+        p = P[cells]
+        if cells.shape[1] == 4:
+            vol = np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0])
+        else:  # prism: bottom triangle normal against the extrusion direction
+            vol = np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0])
+        neg = vol < 0
+        cells[neg] = cells[neg][:, swap]
+        return cells
+    tets = fix(tets, [0, 2, 1, 3])
+    prisms = fix(prisms, [0, 2, 1, 3, 5, 4])
+    jmax = n_quad + n_outer
+
+    def bcs(nodes):
+        v = nodes[:, 0]
+        k = v // n2
+        r = (v % n2) // ns
+        out = np.full(len(nodes), BC_FARFIELD, np.int32)
+        valid = nodes >= 0
+        kk = np.where(valid, nodes // n2, -1)
+        rr = np.where(valid, (nodes % n2) // ns, -1)
+        all_root = np.all(np.where(valid, kk == 0, True), axis=1)
+        all_tip = np.all(np.where(valid, kk == n_span, True), axis=1)
+        all_wall = np.all(np.where(valid, rr == 0, True), axis=1)
+        out[all_wall] = BC_SOLID_SURFACE
+        out[all_root | all_tip] = BC_SYMMETRY
+        return out
+
+    return grid_from_polyhedra_np(X, Y, Z, tets, prisms, bcs)

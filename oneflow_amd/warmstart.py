@@ -1,0 +1,138 @@
+"""Learned warm start for the airfoil sweep.
+
+Pipeline (all compute on the GPU through libofx.so):
+  1. solve a training set of airfoil cases with the plain solver (batched);
+  2. sample their converged cell fields into (features, targets);
+  3. train the BF16 MLP (ofx_mlp_*) on them with the library's own kernels;
+  4. for unseen cases, predict the field, write it as the initial state, and
+     let the plain solver iterate from there to the same residual tolerance as
+     from freestream -- the solver is both the fallback and the judge.
+
+Inputs per cell (9): the centroid folded into the unit disk around mid-chord,
+x' = (x - 0.5) / (1 + r), y' = y / (1 + r), r = |(x - 0.5, y)|; the wall
+distance as log10(d + 1e-6) / 3 + 1; the case: 25 m, p, 10 t,
+2 (log10 Re - 6.5), 4 (Mach - 0.55), 10 AoA[rad].  The four airfoil
+parameters are (m, p, t, Re).
+Targets per cell (5): rho - 1, u - cos(AoA), v - sin(AoA), p / p_inf - 1 and
+log10(nu_tilde / nu_tilde_inf) / 2 (nu_tilde floored at 1e-3 nu_tilde_inf).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from .cases import AirfoilCase
+from .solver import Batch, Solver
+
+
+def cell_features(s: Solver, case: AirfoilCase) -> np.ndarray:
+    C = s.n_cells
+    xc = s.get("XCC")[:, :C]
+    d = s.get("DIST")[:C]
+    x = xc[0] - 0.5
+    y = xc[1]
+    r = np.sqrt(x * x + y * y)
+    f = np.empty((C, 9), np.float32)
+    f[:, 0] = x / (1.0 + r)
+    f[:, 1] = y / (1.0 + r)
+    f[:, 2] = np.log10(d + 1e-6) / 3.0 + 1.0
+    f[:, 3] = 25.0 * case.m
+    f[:, 4] = case.p
+    f[:, 5] = 10.0 * case.t
+    f[:, 6] = 2.0 * (math.log10(case.reynolds) - 6.5)
+    f[:, 7] = 4.0 * (case.mach - 0.55)
+    f[:, 8] = 10.0 * math.radians(case.aoa)
+    return f
+
+
+def _free(case: AirfoilCase, cfg):
+    a = math.radians(case.aoa)
+    p_inf = 1.0 / (cfg.gama_ref * case.mach * case.mach)
+    return math.cos(a), math.sin(a), p_inf, cfg.ref_sa
+
+
+def cell_targets(s: Solver, case: AirfoilCase, cfg) -> np.ndarray:
+    C = s.n_cells
+    q = s.get("Q")[:, :C]
+    nu = s.get("TURBQ")[:C]
+    ca, sa, p_inf, nu_inf = _free(case, cfg)
+    t = np.empty((C, 5), np.float32)
+    t[:, 0] = q[0] - 1.0
+    t[:, 1] = q[1] - ca
+    t[:, 2] = q[2] - sa
+    t[:, 3] = q[4] / p_inf - 1.0
+    t[:, 4] = np.log10(np.maximum(nu, 1e-3 * nu_inf) / nu_inf) / 2.0
+    return t
+
+
+def apply_prediction(s: Solver, case: AirfoilCase, cfg, y: np.ndarray) -> None:
+    """Write a predicted field as the initial state (positivity-floored), then
+    the boundary tasks fill ghosts, temperature and viscosities."""
+    C, T = s.n_cells, s.n_tcell
+    ca, sa, p_inf, nu_inf = _free(case, cfg)
+    q = s.get("Q")
+    nu = s.get("TURBQ")
+    q[0, :C] = np.maximum(1.0 + y[:, 0], 0.05)
+    q[1, :C] = ca + y[:, 1]
+    q[2, :C] = sa + y[:, 2]
+    q[3, :C] = 0.0
+    q[4, :C] = p_inf * np.maximum(1.0 + y[:, 3], 0.05)
+    nu[:C] = nu_inf * np.power(10.0, 2.0 * np.clip(y[:, 4], -1.5, 2.5))
+    s.set("Q", q)
+    s.set("TURBQ", nu)
+    s.ns_boundary()
+    s.sa_boundary()
+
+
+def solve_to_tolerance(solvers, refs, tol=1e-8, chunk=500, max_steps=150_000, batched=True, steps_per_graph=10,
+                       progress=None):
+    """Iterate every case until its density residual norm falls below
+    tol * refs[i] (an absolute level per case).  Batched: all unconverged
+    cases advance together in one recorded graph; converged cases leave the
+    batch.  Sequential: one case after another.  `progress(case, steps)` is
+    called as each case finishes.  Returns (wall seconds, steps per case)."""
+    n = len(solvers)
+    steps = [0] * n
+    done = [False] * n
+    for s in solvers:
+        s.sync()
+    t0 = time.perf_counter()
+    ramp = max(s.cfg.ncfl for s in solvers)
+    chunk = max(steps_per_graph, chunk // steps_per_graph * steps_per_graph)
+    if batched:
+        live = list(range(n))
+        while live:
+            b = Batch([solvers[i] for i in live], steps_per_graph=steps_per_graph)
+            it = steps[live[0]] + 1
+            while True:
+                # after the ramp, whole graphs only: no leftover step moves the
+                # reduction parities, so the recorded graph stays valid
+                nstep = chunk + (ramp - it if it < ramp else 0)
+                b.iterate(it, nstep)
+                it += nstep
+                nr = b.norms()[:, 0]
+                hit = [i for i, r in zip(live, nr) if r <= tol * refs[i] or it > max_steps]
+                for i in live:
+                    steps[i] = it - 1
+                if hit:
+                    for i in hit:
+                        done[i] = True
+                        if progress:
+                            progress(i, steps[i])
+                    live = [i for i in live if not done[i]]
+                    break
+            b.close()
+    else:
+        for i, s in enumerate(solvers):
+            it = 1
+            while True:
+                s.iterate(it, chunk, with_norms=False)
+                it += chunk
+                if s.norms()[0] <= tol * refs[i] or it > max_steps:
+                    break
+            steps[i] = it - 1
+            if progress:
+                progress(i, steps[i])
+    return time.perf_counter() - t0, steps

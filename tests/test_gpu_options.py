@@ -2,7 +2,7 @@
 do not exercise: Barth and zero-order reconstruction, VIS_AVER / VIS_TEST face
 gradients, isothermal walls, inflow / outflow / extrapolation boundaries, the
 other viscous spectral-radius models and SA diffusion methods, and 3-D grids
-of hexes, tets, prisms and prism layers under tets.
+of hexes, tets, prisms, prism layers under tets and a small swept wing.
 
 Each case runs on small jittered synthetic grids: one pseudo-time step task by
 task with the oracle's state given to the GPU before every task (tolerance
@@ -48,6 +48,9 @@ GRIDS = {
     "tet": lambda: meshgen.box3d(4, 3, 3, "tet", jitter=0.15, seed=14, bc=box_bc3),
     "prism": lambda: meshgen.box3d(4, 4, 3, "prism", jitter=0.15, seed=15, bc=box_bc3),
     "mixed": lambda: meshgen.box3d(4, 3, 4, "mixed", prism_layers=2, jitter=0.15, seed=16, bc=box_bc3),
+    # the swept-wing generator of the 3-D benchmark, small, multicolour numbered
+    "wing": lambda: meshgen.renumber(*(lambda g: (g, meshgen.multicolor_order(g)))(
+        meshgen.swept_wing(n_surf=32, n_quad=4, n_outer=6, n_span=3))),
 }
 
 OPTS = {
@@ -62,8 +65,9 @@ OPTS = {
 }
 
 CASES = [(g, o) for g in ("quad", "tri") for o in OPTS] + \
-        [(g, o) for g in ("hex", "tet", "prism", "mixed") for o in ("roe_venkat_sa_test", "slau2_o1_lam_aver_iso",
-                                                                      "roe_barth_lam_std")]
+        [(g, o) for g in ("hex", "tet", "prism", "mixed", "wing") for o in ("roe_venkat_sa_test",
+                                                                              "slau2_o1_lam_aver_iso",
+                                                                              "roe_barth_lam_std")]
 
 
 def make(grid, opt):

@@ -162,3 +162,16 @@ def test_synthetic_twenty_steps(case):
     e = report("norms", ng, no)
     assert e <= 1.0e-10, e
     cmp(gpu, ora, ["Q", "TURBQ"], tol=1.0e-10)
+
+
+def test_continue_from_reference_restart():
+    """The reference's restart file (plate after 34950 steps) through the GPU:
+    same pins as tests/test_restart.py, and the oracle's norms and forces."""
+    from oneflow_amd.solver import Solver
+    from tests.test_restart import check_against_goldens, continued
+    norms, aero = continued(Solver)
+    onorms, oaero = continued(OracleCase)
+    e = report("norms", norms, onorms)
+    assert e <= 1.0e-10, e
+    assert np.allclose(aero, oaero, rtol=1e-10, atol=1e-16)
+    check_against_goldens(norms, aero)
