@@ -209,6 +209,25 @@ int ofx_time_kernel(ofx_ctx* ctx, int which, int iters, double* ms_out);
  * Allocates and frees its own two buffers. */
 int ofx_copy_bench(ofx_ctx* ctx, size_t bytes, int iters, double* ms_out);
 
+/* Several independent cases in one context (no counterpart in the
+ * reference, which solves zones one after another, codes/task/src/SimpleTask.cpp):
+ * upload the cases' grids concatenated zone after zone -- cells, then all
+ * zones' boundary faces, then all zones' interior faces, each part in zone
+ * order with the zone's own order kept -- with ofx_mesh_upload, set the shared
+ * options with ofx_config_set, then declare the zones: cellStart[n + 1] and
+ * bfaceStart[n + 1] (zone z owns cells [cellStart[z], cellStart[z+1]) and
+ * boundary faces [bfaceStart[z], bfaceStart[z+1])) and one configuration
+ * per zone, which may differ from the shared one only in mach_ref,
+ * aoa_degree, sideslip_degree and reynolds.  Every kernel then runs over all
+ * zones at once with per-zone freestream constants, per-zone time-step and
+ * limiter reductions and per-zone wall distance; each zone's results are
+ * bit-identical to solving it alone.  Call again after ofx_config_set. */
+int ofx_zones_set(ofx_ctx* ctx, int n, const int* cellStart, const int* bfaceStart, const ofx_config* cfgs);
+/* Residual norms of each zone's current residual (out[n]). */
+int ofx_zones_norms(ofx_ctx* ctx, ofx_norms* out);
+/* Force coefficients of each zone (out[4 n]: CL, CD, CDp, CDf per zone). */
+int ofx_zones_aero(ofx_ctx* ctx, double* out);
+
 /* Face-kernel tile, the threads per block of the NS and SA face-flux kernels:
  * 64, 128, 256 (default) or 512.  Results do not depend on it (no
  * counterpart in the reference; a launch parameter). */

@@ -138,14 +138,43 @@ static void set_consts(ofx_ctx* c) {
 
 int init_freestream(ofx_ctx* c) {
     const long T = c->m.T;
-    for (int e = 0; e < 5; ++e)
-        k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.q + e * T, T, c->k.inflow[e]);
+    if (c->m.nz > 1) {
+        // each zone's cells and ghosts take that zone's freestream
+        const long C = c->m.C;
+        for (int z = 0; z < c->m.nz; ++z) {
+            long c0 = c->zc[z], c1 = c->zc[z + 1], g0 = C + c->zb[z], g1 = C + c->zb[z + 1];
+            for (int e = 0; e < 5; ++e) {
+                double v = c->zkh[z].inflow[e];
+                if (c1 > c0) k_fill_field<<<(int)((c1 - c0 + 255) / 256), 256, 0, c->stream>>>(c->d.q + e * T + c0, c1 - c0, v);
+                if (g1 > g0) k_fill_field<<<(int)((g1 - g0 + 255) / 256), 256, 0, c->stream>>>(c->d.q + e * T + g0, g1 - g0, v);
+            }
+        }
+    } else {
+        for (int e = 0; e < 5; ++e)
+            k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.q + e * T, T, c->k.inflow[e]);
+    }
     k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.turbq, T, c->k.sa_inflow);
     k_fill_field<<<(int)((T + 255) / 256), 256, 0, c->stream>>>(c->d.vist, T, 0.0);
     OFX_HIP(hipGetLastError());
     OFX_TRY(ns_boundary(c));
     OFX_TRY(sa_boundary(c));
     return 0;
+}
+
+// The case-dependent constants of one configuration (as set_consts).
+ZoneK zone_consts(const ofx_config& g) {
+    ZoneK z;
+    double aoa = g.aoa_degree * PI / 180.0, aos = g.sideslip_degree * PI / 180.0;
+    z.statecoef = 1.0 / (g.gama_ref * (g.mach_ref * g.mach_ref));
+    z.reynolds = g.reynolds;
+    z.oreynolds = 1.0 / g.reynolds;
+    z.const_cp = 1.0 / ((g.gama_ref - 1.0) * (g.mach_ref * g.mach_ref));
+    z.inflow[0] = 1.0;
+    z.inflow[1] = 1.0 * std::cos(aoa) * std::cos(aos);
+    z.inflow[2] = 1.0 * std::sin(aoa) * std::cos(aos);
+    z.inflow[3] = 1.0 * std::sin(aos);
+    z.inflow[4] = z.statecoef * 1.0 * 1.0 / 1.0;
+    return z;
 }
 
 }  // namespace ofx
@@ -450,6 +479,100 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     if (c->k.vismodel == 3) OFX_TRY(sa_residual(c));
     OFX_HIP(hipStreamSynchronize(c->stream));
     if (chosen2) { chosen2[0] = c->face_tb; chosen2[1] = c->saface_tb; }
+    return 0;
+}
+
+int ofx_zones_set(ofx_ctx* c, int n, const int* cellStart, const int* bfaceStart, const ofx_config* cfgs) {
+    CHECK_READY(c);
+    DevMesh& m = c->m;
+    const int C = m.C, B = m.B, F = m.F, T = m.T;
+    if (n < 1 || !cellStart || !bfaceStart || !cfgs) { set_error("zones_set: bad arguments"); return OFX_EINVAL; }
+    if (cellStart[0] != 0 || cellStart[n] != C || bfaceStart[0] != 0 || bfaceStart[n] != B) {
+        set_error("zones_set: zone starts must begin at 0 and end at nCells / nBFaces");
+        return OFX_EINVAL;
+    }
+    for (int z = 0; z < n; ++z)
+        if (cellStart[z + 1] <= cellStart[z] || bfaceStart[z + 1] < bfaceStart[z]) {
+            set_error("zones_set: every zone needs cells; starts must not decrease");
+            return OFX_EINVAL;
+        }
+    for (int z = 0; z < n; ++z) {  // only the freestream (Mach, angles, Reynolds) may differ
+        ofx_config a = cfgs[z], b = c->cfg;
+        a.mach_ref = b.mach_ref; a.aoa_degree = b.aoa_degree; a.sideslip_degree = b.sideslip_degree;
+        a.reynolds = b.reynolds;
+        if (std::memcmp(&a, &b, sizeof(a)) != 0) {
+            set_error("zones_set: zone configurations may differ only in mach_ref, aoa_degree, sideslip_degree, reynolds");
+            return OFX_EINVAL;
+        }
+        if (!(cfgs[z].mach_ref > 0) || !(cfgs[z].reynolds > 0)) { set_error("zones_set: bad freestream"); return OFX_EINVAL; }
+    }
+    OFX_HIP(hipSetDevice(c->device));
+    std::vector<int> lc(F), rc(F), wall(std::max(1, m.nWall));
+    OFX_HIP(hipMemcpy(lc.data(), m.lc, sizeof(int) * F, hipMemcpyDeviceToHost));
+    OFX_HIP(hipMemcpy(rc.data(), m.rc, sizeof(int) * F, hipMemcpyDeviceToHost));
+    if (m.nWall) OFX_HIP(hipMemcpy(wall.data(), m.wallFaces, sizeof(int) * m.nWall, hipMemcpyDeviceToHost));
+    std::vector<int> zcell(T), zface(F), zwall(n + 1, 0);
+    for (int z = 0; z < n; ++z) {
+        for (int i = cellStart[z]; i < cellStart[z + 1]; ++i) zcell[i] = z;
+        for (int f = bfaceStart[z]; f < bfaceStart[z + 1]; ++f) { zface[f] = z; zcell[C + f] = z; }
+    }
+    for (int f = 0; f < F; ++f) {
+        if (f >= B) zface[f] = zcell[lc[f]];
+        if (zcell[lc[f]] != zface[f] || zcell[rc[f]] != zface[f]) {
+            set_error("zones_set: a face joins two zones (zones must be disconnected, boundary faces zone by zone)");
+            return OFX_EINVAL;
+        }
+    }
+    for (int z = 0, w = 0; z <= n; ++z) {  // wall faces are in face order, so zone by zone
+        int fz = (z < n) ? bfaceStart[z] : B;
+        while (w < m.nWall && wall[w] < fz) ++w;
+        zwall[z] = w;
+    }
+    c->zkh.resize(n);
+    for (int z = 0; z < n; ++z) c->zkh[z] = zone_consts(cfgs[z]);
+    auto A = [&](void** p, size_t bytes) { return alloc_bytes(c, p, bytes); };
+    OFX_TRY(A((void**)&m.zcell, sizeof(int) * T));
+    OFX_TRY(A((void**)&m.zface, sizeof(int) * F));
+    OFX_TRY(A((void**)&m.zwall, sizeof(int) * (n + 1)));
+    ZoneK* zk = nullptr;
+    OFX_TRY(A((void**)&zk, sizeof(ZoneK) * n));
+    OFX_TRY(upload(c, m.zcell, zcell.data(), sizeof(int) * T));
+    OFX_TRY(upload(c, m.zface, zface.data(), sizeof(int) * F));
+    OFX_TRY(upload(c, m.zwall, zwall.data(), sizeof(int) * (n + 1)));
+    OFX_TRY(upload(c, zk, c->zkh.data(), sizeof(ZoneK) * n));
+    m.zk = zk;
+    // per-zone reduction slots (both parities; the min quantity starts at LARGE) and results
+    DevFields& d = c->d;
+    OFX_TRY(alloc_bytes(c, (void**)&d.slots, sizeof(unsigned long long) * 2 * (size_t)n * RQ_N * NSLOT));
+    for (int p = 0; p < 2; ++p)
+        for (int z = 0; z < n; ++z)
+            k_fill_field<<<1, 64, 0, c->stream>>>(reinterpret_cast<double*>(d.slots) +
+                                                      (((size_t)p * n + z) * RQ_N + RQ_DTMIN) * NSLOT, NSLOT, LARGE);
+    OFX_HIP(hipGetLastError());
+    OFX_TRY(A((void**)&d.zred, sizeof(double) * (size_t)n * RQ_N));
+    c->par_dt = c->par_lim = c->par_fix = 0;
+    m.nz = n;
+    c->zc.assign(cellStart, cellStart + n + 1);
+    c->zb.assign(bfaceStart, bfaceStart + n + 1);
+    c->zcfg.assign(cfgs, cfgs + n);
+    OFX_TRY(wall_distance(c));
+    OFX_HIP(hipStreamSynchronize(c->stream));
+    ++c->gen;
+    return 0;
+}
+
+int ofx_zones_norms(ofx_ctx* c, ofx_norms* out) {
+    CHECK_READY(c);
+    if (!out) { set_error("null out"); return OFX_EINVAL; }
+    if (c->m.nz == 1) return ofx_norms_get(c, out);
+    for (int z = 0; z < c->m.nz; ++z) OFX_TRY(norms_range(c, c->zc[z], c->zc[z + 1], out + z));
+    return 0;
+}
+
+int ofx_zones_aero(ofx_ctx* c, double* out) {
+    CHECK_READY(c);
+    if (!out) { set_error("null out"); return OFX_EINVAL; }
+    for (int z = 0; z < c->m.nz; ++z) OFX_TRY(aero_zone(c, z, out + 4 * z));
     return 0;
 }
 

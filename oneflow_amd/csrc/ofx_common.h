@@ -53,6 +53,18 @@ struct DevMesh {
     // nodes (mesh build + wall distance only)
     double *xn = nullptr, *yn = nullptr, *zn = nullptr;
     int *fnOff = nullptr, *fnNodes = nullptr, *wallFaces = nullptr;
+    // several independent zones (cases) in one context: zone of every cell
+    // (ghosts included) and face, per-zone constants, per-zone wall-face
+    // ranges [zwall[z], zwall[z+1]) of wallFaces.  nz == 1: all unused.
+    int nz = 1;
+    int *zcell = nullptr, *zface = nullptr, *zwall = nullptr;
+    const struct ZoneK* zk = nullptr;
+};
+
+// The case-dependent constants (Mach, AoA, sideslip, Reynolds) of one zone.
+struct ZoneK {
+    double statecoef, reynolds, oreynolds, const_cp;
+    double inflow[5];
 };
 
 // Host copies of the level schedule sizes for launch decisions.
@@ -97,7 +109,8 @@ struct DevFields {
     double *saface = nullptr;   // [5][F]: radius, matrix_l, matrix_r, conv flux, diff flux
     double *red = nullptr;      // reduction results (see RED_* in ofx_ns.hip)
     double *norms = nullptr;    // [6] residual norm accumulators
-    unsigned long long *slots = nullptr;  // [2 parities][RED_NQ][NSLOT] grid-reduction slots
+    unsigned long long *slots = nullptr;  // [2 parities][nz][RQ_N][NSLOT] grid-reduction slots
+    double *zred = nullptr;     // [nz][RQ_N] per-zone reduced values (several zones only)
 };
 
 }  // namespace ofx
@@ -115,6 +128,11 @@ struct ofx_ctx {
     int par_dt = 0, par_lim = 0, par_fix = 0;  // parity of the reduction buffers in use
     int face_tb = 256, saface_tb = 256;        // face-kernel tile (threads per block)
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
+    // several zones: cell / boundary-face starts (nz + 1 each), host copies of
+    // the per-zone constants and configurations
+    std::vector<int> zc, zb;
+    std::vector<ofx::ZoneK> zkh;
+    std::vector<ofx_config> zcfg;
 };
 
 #define OFX_HIP(call)                                                              \
@@ -240,6 +258,94 @@ __device__ __forceinline__ double group_max(double x) {
     return x;
 }
 
+// Grid-reduction quantities (slot_reduce / slot_result):
+//   0      : min time step (min)
+//   1..5   : -qmin per primitive (Venkat, max)
+//   6..10  : qmax per primitive (Venkat, max)
+//   11     : 1 if some cell needs SolutionFix (max; one flag per context)
+constexpr int RQ_DTMIN = 0, RQ_QMIN = 1, RQ_QMAX = 6, RQ_FIX = 11, RQ_N = 12;
+
+// Zone of a cell / face (contexts holding several independent cases).
+__device__ __forceinline__ int cell_zone(const DevMesh& m, int i) { return m.nz > 1 ? m.zcell[i] : 0; }
+__device__ __forceinline__ int face_zone(const DevMesh& m, int f) { return m.nz > 1 ? m.zface[f] : 0; }
+
+// Minimum time step of cell i's zone: the block-reduced value of a
+// single-zone context, or the zone's extracted value.
+__device__ __forceinline__ double zone_mindt(const DevMesh& m, const DevFields& d, int i, double mindt1) {
+    return m.nz > 1 ? d.zred[(size_t)m.zcell[i] * RQ_N + RQ_DTMIN] : mindt1;
+}
+
+// Constants of the zone a cell or face belongs to (single-zone contexts: k).
+__device__ __forceinline__ Consts zoned(const Consts& k, const DevMesh& m, int zone) {
+    Consts r = k;
+    if (m.nz > 1 && zone >= 0) {
+        const ZoneK& z = m.zk[zone];
+        r.statecoef = z.statecoef;
+        r.reynolds = z.reynolds;
+        r.oreynolds = z.oreynolds;
+        r.const_cp = z.const_cp;
+        for (int e = 0; e < 5; ++e) r.inflow[e] = z.inflow[e];
+    }
+    return r;
+}
+
+// Zone-aware grid reductions (contexts with several zones).  Buffer layout
+// [2 parities][nz][nq_total][NSLOT].  Every lane of the wave calls; lanes
+// without a cell pass zone = -1.  Each wave reduces the lanes of each zone it
+// holds (zones are contiguous cell ranges, so usually one) and issues one
+// atomic per zone and quantity.  The other parity's entries of these
+// quantities are re-armed by all blocks together.
+template <int NQ, bool IS_MAX>
+__device__ __forceinline__ void zslot_reduce(const double* v, int zone, unsigned long long* buf, int par,
+                                             int nq_total, int q0, int nz) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int slot = (int)(blockIdx.x % NSLOT);
+    unsigned long long* mine = buf + (size_t)par * nz * nq_total * NSLOT;
+    unsigned long long todo = __ballot(zone >= 0);
+    while (todo) {
+        int first = __ffsll((long long)todo) - 1;
+        int z = __shfl(zone, first);
+        bool in = (zone == z);
+        for (int q = 0; q < NQ; ++q) {
+            double x = in ? v[q] : (IS_MAX ? 0.0 : LARGE);
+            for (int o = 32; o > 0; o >>= 1) {
+                double y = __shfl_xor(x, o);
+                x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+            }
+            if (lane == first) {
+                unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+                unsigned long long* a = mine + ((size_t)z * nq_total + q0 + q) * NSLOT + slot;
+                if (IS_MAX) atomicMax(a, bits); else atomicMin(a, bits);
+            }
+        }
+        todo &= ~__ballot(in);
+    }
+    unsigned long long* other = buf + (size_t)(1 - par) * nz * nq_total * NSLOT;
+    const unsigned long long ident = IS_MAX ? 0ull : (unsigned long long)__double_as_longlong(LARGE);
+    size_t n = (size_t)nz * NQ * NSLOT;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        size_t z = t / (NQ * NSLOT), r = t % (NQ * NSLOT);
+        other[(z * nq_total + q0) * NSLOT + r] = ident;
+    }
+}
+
+// Consumer side for several zones: one block per zone reduces the NSLOT slots
+// of quantities q0..q0+nq-1 into zred[zone][nq_total] (read by the consumers).
+template <bool IS_MAX>
+__global__ void k_zone_extract(const unsigned long long* buf, int par, int nq_total, int q0, int nq, int nz,
+                               double* zred) {
+    const int z = blockIdx.x, lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const unsigned long long* mine = buf + (size_t)par * nz * nq_total * NSLOT;
+    for (int q = w; q < nq; q += (int)(blockDim.x >> 6)) {
+        double x = __longlong_as_double((long long)mine[((size_t)z * nq_total + q0 + q) * NSLOT + lane]);
+        for (int o = 32; o > 0; o >>= 1) {
+            double y = __shfl_xor(x, o);
+            x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+        }
+        if (lane == 0) zred[(size_t)z * nq_total + q0 + q] = x;
+    }
+}
+
 int alloc_bytes(ofx_ctx* c, void** p, size_t bytes);
 int upload(ofx_ctx* c, void* dst, const void* src, size_t bytes);
 
@@ -258,6 +364,9 @@ int sa_lusgs(ofx_ctx* c);
 int ns_update(ofx_ctx* c);
 int sa_update(ofx_ctx* c);
 int norms(ofx_ctx* c, ofx_norms* out);
+int wall_distance(ofx_ctx* c);
+int norms_range(ofx_ctx* c, int c0, int c1, ofx_norms* out);
+int aero_zone(ofx_ctx* c, int z, double* out4);
 int fill_ghost_grad(ofx_ctx* c);
 int materialize_timestep(ofx_ctx* c);
 int ns_update_fused(ofx_ctx* c);

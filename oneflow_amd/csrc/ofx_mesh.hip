@@ -247,7 +247,9 @@ __device__ double tri_dist2(const double P[3], const double V0[3], const double 
 
 // CalcWallDist: brute force over all wall faces, min of squared distance, sqrt
 // (codes/geometry/src/WallDist.cpp:136-358).  One thread per cell; wall-face
-// node coordinates are staged through LDS in tiles shared by the block.
+// node coordinates are staged through LDS in tiles shared by the block.  With
+// several zones a cell sees only its own zone's wall faces (same faces, same
+// order as a single-zone run); the block tiles the union of its zones' ranges.
 constexpr int WD_TILE = 256;
 __global__ void __launch_bounds__(256) k_wall_distance(DevMesh m) {
     __shared__ double sx[WD_TILE * 4], sy[WD_TILE * 4], sz[WD_TILE * 4];
@@ -257,8 +259,16 @@ __global__ void __launch_bounds__(256) k_wall_distance(DevMesh m) {
     double p[3] = {0, 0, 0};
     if (live) { p[0] = m.xcc[i]; p[1] = m.ycc[i]; p[2] = m.zcc[i]; }
     double best = LARGE;
-    for (int base = 0; base < m.nWall; base += WD_TILE) {
-        int nt = min(WD_TILE, m.nWall - base);
+    int w0 = 0, w1 = m.nWall, my0 = 0, my1 = m.nWall;
+    if (m.nz > 1) {
+        int first = blockIdx.x * blockDim.x, last = min(first + (int)blockDim.x, m.C) - 1;
+        w0 = m.zwall[m.zcell[first]];
+        w1 = m.zwall[m.zcell[last] + 1];
+        my0 = live ? m.zwall[m.zcell[i]] : 0;
+        my1 = live ? m.zwall[m.zcell[i] + 1] : 0;
+    }
+    for (int base = w0; base < w1; base += WD_TILE) {
+        int nt = min(WD_TILE, w1 - base);
         __syncthreads();
         for (int t = threadIdx.x; t < nt; t += blockDim.x) {
             int f = m.wallFaces[base + t];
@@ -272,6 +282,7 @@ __global__ void __launch_bounds__(256) k_wall_distance(DevMesh m) {
         __syncthreads();
         if (!live) continue;
         for (int t = 0; t < nt; ++t) {
+            if (base + t < my0 || base + t >= my1) continue;
             int n = sn[t];
             double w;
             double a[3] = {sx[t * 4], sy[t * 4], sz[t * 4]};
@@ -448,6 +459,13 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     k_wall_distance<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());
     OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int wall_distance(ofx_ctx* c) {
+    const int TB = 256;
+    if (c->m.C) k_wall_distance<<<(c->m.C + TB - 1) / TB, TB, 0, c->stream>>>(c->m);
+    OFX_HIP(hipGetLastError());
     return 0;
 }
 

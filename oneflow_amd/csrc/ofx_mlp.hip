@@ -227,6 +227,12 @@ __global__ void k_copy1(const float* a, float* b) {
     if (threadIdx.x == 0 && blockIdx.x == 0) b[0] = a[0];
 }
 
+// y[i][0..NOUT) of the padded output rows, packed densely for one host copy
+__global__ void k_pack_outputs(const float* __restrict__ yp, int nsamp, float* __restrict__ y) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (size_t)nsamp * NOUT) y[i] = yp[(i / NOUT) * NOUT_P + i % NOUT];
+}
+
 __global__ void k_to_bf16(const float* a, size_t n, bf16* b) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = f2bf(a[i]);
@@ -515,8 +521,11 @@ int ofx_mlp_forward(ofx_mlp* n, const float* x, int nsamp, float* y) {
     OFX_TRY(load_batch(n, x, nullptr, nsamp, false));
     int Mp = (nsamp + 63) / 64 * 64;
     OFX_TRY(forward(n, Mp));
-    OFX_HIP(hipMemcpy2DAsync(y, sizeof(float) * NOUT, n->y, sizeof(float) * NOUT_P, sizeof(float) * NOUT, nsamp,
-                             hipMemcpyDeviceToHost, n->stream));
+    // tgt (Mp x NOUT floats) is free in a forward-only call: pack there, copy once
+    size_t cnt = (size_t)nsamp * NOUT;
+    k_pack_outputs<<<(unsigned)((cnt + 255) / 256), 256, 0, n->stream>>>(n->y, nsamp, n->tgt);
+    OFX_HIP(hipGetLastError());
+    OFX_HIP(hipMemcpyAsync(y, n->tgt, sizeof(float) * cnt, hipMemcpyDeviceToHost, n->stream));
     OFX_HIP(hipStreamSynchronize(n->stream));
     return 0;
 }

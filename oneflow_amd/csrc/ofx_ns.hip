@@ -18,9 +18,10 @@ static inline int nblk(int n) { return (n + TB - 1) / TB; }
 // ============================================================ boundary
 
 // NsCalcGamaT + CalcLaminarViscosity on interior cells (codes/ns/src/NsSolverImp.cpp:72-79)
-__global__ void __launch_bounds__(TB) k_ns_cellstate(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(TB) k_ns_cellstate(DevMesh m, DevFields d, Consts k0) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
+    const Consts k = zoned(k0, m, cell_zone(m, i));
     double t = temperature(d.q[i], d.q[4 * m.T + i], k.statecoef);
     d.tempr[i] = t;
     d.visl[i] = sutherland(t, k.ts_suth);
@@ -100,9 +101,10 @@ __device__ __forceinline__ void ns_bc_face(const DevMesh& m, const DevFields& d,
     d.visl[r] = sutherland(tt, k.ts_suth);
 }
 
-__global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k0) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.B) return;
+    const Consts k = zoned(k0, m, face_zone(m, f));
     int l = m.lc[f];
     double s[5];
     for (int e = 0; e < 5; ++e) s[e] = d.q[e * m.T + l];
@@ -119,19 +121,16 @@ int ns_boundary(ofx_ctx* c) {
 
 // ============================================================ time step
 
-// Grid-reduction quantities (slot_reduce / slot_result):
-//   0      : min time step (min)
-//   1..5   : -qmin per primitive (Venkat, max)
-//   6..10  : qmax per primitive (Venkat, max)
-//   11     : 1 if some cell needs SolutionFix (max)
-constexpr int RQ_DTMIN = 0, RQ_QMIN = 1, RQ_QMAX = 6, RQ_FIX = 11, RQ_N = 12;
+
 
 // UTimeStep::CalcInvSpectrumField / CalcVisSpectrumField / CalcLocalTimeStep
 // (codes/uns/src/UTimeStep.cpp:190-416, codes/ns/src/TimeStep.cpp:43-133):
 // per cell, the face spectral radii are summed in ascending face order.
-__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k, int par) {
+__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k0, int par) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     const int T = m.T;
+    const int zone = (i < m.C) ? cell_zone(m, i) : -1;
+    const Consts k = zoned(k0, m, zone);
     double dtv = LARGE;
     if (i < m.C) {
         double invsr = 0.0, vissr = 0.0;
@@ -184,8 +183,10 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
         d.timestep[i] = dt;
         dtv = dt;
     }
-    slot_reduce<1, false>(&dtv, d.slots, par, RQ_N, RQ_DTMIN);
+    if (m.nz > 1) zslot_reduce<1, false>(&dtv, zone, d.slots, par, RQ_N, RQ_DTMIN, m.nz);
+    else slot_reduce<1, false>(&dtv, d.slots, par, RQ_N, RQ_DTMIN);
 }
+
 
 // Time step after UTimeStep::ModifyTimeStep (codes/uns/src/UTimeStep.cpp:368-381)
 // and the LU-SGS diagonal of UNsSpectrum::CalcImplicitSpectrum
@@ -208,9 +209,10 @@ __device__ __forceinline__ double ns_impsr(const DevMesh& m, const DevFields& d,
 // Materialise the clamped time step and the LU-SGS diagonal as fields
 // (for download; the solver evaluates them where they are consumed).
 __global__ void __launch_bounds__(TB) k_ns_dtclamp(DevMesh m, DevFields d, Consts k, int par) {
-    double mindt = slot_result<false>(d.slots, par, RQ_N, RQ_DTMIN);
+    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
+    mindt = zone_mindt(m, d, i, mindt);
     double dt = clamped_dt(d, k, i, mindt);
     double imp = ns_impsr(m, d, k, i, mindt);
     d.timestep[i] = dt;
@@ -233,6 +235,8 @@ int ns_timestep(ofx_ctx* c, int iter) {
     c->k.cfl = cfl;
     c->par_dt ^= 1;
     k_ns_timestep<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_dt);
+    if (c->m.nz > 1)
+        k_zone_extract<false><<<c->m.nz, 64, 0, c->stream>>>(c->d.slots, c->par_dt, RQ_N, RQ_DTMIN, 1, c->m.nz, c->d.zred);
     OFX_HIP(hipGetLastError());
     return 0;
 }
@@ -247,9 +251,11 @@ int ns_timestep(ofx_ctx* c, int iter) {
 // before the contributions are added in ascending face order.  DIM == 2 skips
 // the z components and w, which are identically zero in 2-D.
 template <int DIM, int MAXF, bool MINMAX>
-__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k, int par) {
+__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k0, int par) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     const int T = m.T;
+    const int zone = (i < m.C) ? cell_zone(m, i) : -1;
+    const Consts k = zoned(k0, m, zone);
     double mn[5], mx[5];
     bool live = i < m.C;
     if (live) {
@@ -322,7 +328,8 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
     if (MINMAX) {
         double v[10];
         for (int s = 0; s < 5; ++s) { v[s] = live ? 0.0 - mn[s] : 0.0; v[5 + s] = live ? mx[s] : 0.0; }
-        slot_reduce<10, true>(v, d.slots, par, RQ_N, RQ_QMIN);
+        if (m.nz > 1) zslot_reduce<10, true>(v, zone, d.slots, par, RQ_N, RQ_QMIN, m.nz);
+        else slot_reduce<10, true>(v, d.slots, par, RQ_N, RQ_QMIN);
     }
 }
 
@@ -334,7 +341,7 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
     __shared__ double sext[10];
     {
         // the 10 limiter extrema: one wave per quantity group, one latency round
-        const unsigned long long* mine = d.slots + (size_t)par * RQ_N * NSLOT;
+        const unsigned long long* mine = d.slots + (size_t)par * m.nz * RQ_N * NSLOT;
         int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
         double x[3];
         for (int r = 0; r < 3; ++r) {
@@ -381,7 +388,13 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
             double mx = d.minmax[(5 + s) * m.C + i];
             double dq = gx * dx + gy * dy + gz * dz;
             if (LIM == 2) {
-                double eps = k.vencat_coef * dmax(sext[5 + s] - (-sext[s]), 1.0);
+                double lo = sext[s], hi = sext[5 + s];
+                if (m.nz > 1) {
+                    const double* zr = d.zred + (size_t)m.zcell[i] * RQ_N;
+                    lo = zr[RQ_QMIN + s];
+                    hi = zr[RQ_QMAX + s];
+                }
+                double eps = k.vencat_coef * dmax(hi - (-lo), 1.0);
                 double eps1 = eps * eps + SMALL;
                 lim = vencat_fn(dq, mn, mx, eps1);
             } else {
@@ -403,9 +416,10 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
 // the FP64 reconstruction and both face fluxes are stored as FP32 (the
 // gather accumulates them in FP64).
 template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32>
-__global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k0) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
+    const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T;
     int l = m.lc[f], r = m.rc[f];
     bool bnd = f < m.B;
@@ -603,6 +617,9 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
         if (lim) {
             c->par_lim ^= 1;
             k_ns_grad<DIM, MAXF, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            if (c->m.nz > 1)
+                k_zone_extract<true><<<c->m.nz, 256, 0, c->stream>>>(c->d.slots, c->par_lim, RQ_N, RQ_QMIN, 10,
+                                                                     c->m.nz, c->d.zred);
         } else {
             k_ns_grad<DIM, MAXF, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
         }
@@ -740,8 +757,10 @@ __device__ __forceinline__ void ns_lusgs_face(const DevMesh& m, const DevFields&
 // skipped in both sweeps: their ghost increment is zero, so the reference's
 // upper-sweep contribution through them is exactly +0.
 template <bool UPPER>
-__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k, int i, double mindt) {
+__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int i, double mindt1) {
     const int T = m.T;
+    const Consts k = zoned(k0, m, cell_zone(m, i));
+    const double mindt = zone_mindt(m, d, i, mindt1);
     double rhs0[5] = {0, 0, 0, 0, 0};
     for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
         int e = m.c2f[kk];
@@ -763,7 +782,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
 // levels (every cell of a level depends only on earlier levels).
 template <bool UPPER>
 __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k, int par) {
-    double mindt = slot_result<false>(d.slots, par, RQ_N, RQ_DTMIN);
+    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     const int* off = UPPER ? m.upOff : m.loOff;
     const int* ord = UPPER ? m.upOrder : m.loOrder;
     int nl = UPPER ? m.nLevUp : m.nLevLo;
@@ -777,7 +796,7 @@ __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, C
 // One level per launch (wide levels on large meshes).
 template <bool UPPER>
 __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int par) {
-    double mindt = slot_result<false>(d.slots, par, RQ_N, RQ_DTMIN);
+    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= e) return;
     ns_lusgs_cell<UPPER>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx], mindt);
@@ -849,7 +868,7 @@ __global__ void __launch_bounds__(TB) k_ns_update_check(DevMesh m, DevFields d, 
         double prim[5];
         if (!ns_update_cell(d, k, m.T, i, prim)) flag = 1.0;
     }
-    slot_reduce<1, true>(&flag, d.slots, par, RQ_N, RQ_FIX);
+    slot_reduce<1, true>(&flag, d.slots, par, m.nz * RQ_N, RQ_FIX);
 }
 
 __device__ void ns_solution_fix(const DevMesh& m, const DevFields& d, int i, double* prim) {
@@ -876,7 +895,7 @@ __device__ void ns_solution_fix(const DevMesh& m, const DevFields& d, int i, dou
 template <bool BC>
 __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, Consts k, int par) {
     const int T = m.T;
-    double anyfix = slot_result<true>(d.slots, par, RQ_N, RQ_FIX);
+    double anyfix = slot_result<true>(d.slots, par, m.nz * RQ_N, RQ_FIX);
     if (anyfix == 0.0) {
         int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
         if (i >= m.C) return;
@@ -884,15 +903,16 @@ __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, 
         ns_update_cell(d, k, T, i, prim);
         for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
         if (BC) {
-            double t = temperature(prim[0], prim[4], k.statecoef);
+            const Consts kz = zoned(k, m, cell_zone(m, i));
+            double t = temperature(prim[0], prim[4], kz.statecoef);
             d.tempr[i] = t;
-            d.visl[i] = sutherland(t, k.ts_suth);
+            d.visl[i] = sutherland(t, kz.ts_suth);
             for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
                 int f = m.c2f[kk];
                 // boundary faces (owner side, f < B) come first; entries of faces
                 // where the cell is the right cell are stored as ~f < 0
                 if (f < 0 || f >= m.B) break;
-                ns_bc_face(m, d, k, f, prim, t);
+                ns_bc_face(m, d, kz, f, prim, t);
             }
         }
         return;
@@ -905,7 +925,7 @@ __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, 
     }
     if (BC) {
         for (int i = 0; i < m.C; ++i) {
-            double t = temperature(d.q[i], d.q[4 * T + i], k.statecoef);
+            double t = temperature(d.q[i], d.q[4 * T + i], zoned(k, m, cell_zone(m, i)).statecoef);
             d.tempr[i] = t;
             d.visl[i] = sutherland(t, k.ts_suth);
         }
@@ -913,7 +933,7 @@ __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, 
             int l = m.lc[f];
             double s[5];
             for (int e = 0; e < 5; ++e) s[e] = d.q[e * T + l];
-            ns_bc_face(m, d, k, f, s, d.tempr[l]);
+            ns_bc_face(m, d, zoned(k, m, face_zone(m, f)), f, s, d.tempr[l]);
         }
     }
 }
@@ -940,10 +960,10 @@ int ns_update_fused(ofx_ctx* c) {
 // fixed-shape two-level sum (block partials, then one block in order), so the
 // result is identical run to run.
 constexpr int NORM_BLOCKS = 512;
-__global__ void __launch_bounds__(TB) k_norm_partial(DevMesh m, DevFields d, double* part) {
+__global__ void __launch_bounds__(TB) k_norm_partial(DevMesh m, DevFields d, double* part, int c0, int c1) {
     __shared__ double sm[6][TB];
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.C; i += gridDim.x * blockDim.x) {
+    for (int i = c0 + blockIdx.x * blockDim.x + threadIdx.x; i < c1; i += gridDim.x * blockDim.x) {
         for (int e = 0; e < 5; ++e) { double v = d.res[e * m.T + i]; acc[e] += v * v; }
         double v = d.turbres[i];
         acc[5] += v * v;
@@ -966,10 +986,11 @@ __global__ void k_norm_final(const double* part, double* out, int nCells) {
     out[e] = sqrt(s / nCells);
 }
 
-int norms(ofx_ctx* c, ofx_norms* out) {
+// Norms over cells [c0, c1) (one zone of a multi-zone context, or all cells).
+int norms_range(ofx_ctx* c, int c0, int c1, ofx_norms* out) {
     double* part = c->d.norms + 8;
-    k_norm_partial<<<NORM_BLOCKS, TB, 0, c->stream>>>(c->m, c->d, part);
-    k_norm_final<<<1, 64, 0, c->stream>>>(part, c->d.norms, c->m.C);
+    k_norm_partial<<<NORM_BLOCKS, TB, 0, c->stream>>>(c->m, c->d, part, c0, c1);
+    k_norm_final<<<1, 64, 0, c->stream>>>(part, c->d.norms, c1 - c0);
     OFX_HIP(hipGetLastError());
     double h[6];
     OFX_HIP(hipMemcpyAsync(h, c->d.norms, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -979,13 +1000,15 @@ int norms(ofx_ctx* c, ofx_norms* out) {
     return 0;
 }
 
+int norms(ofx_ctx* c, ofx_norms* out) { return norms_range(c, 0, c->m.C, out); }
+
 // CalcAeroForce per wall face (codes/force/src/AeroForceTask.cpp:186-440), the
 // sum in face order on one thread as the reference loops.
-__global__ void k_aero(DevMesh m, DevFields d, Consts k, double* out) {
+__global__ void k_aero(DevMesh m, DevFields d, Consts k, double* out, int f0, int f1) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int T = m.T;
     double tot[3] = {0, 0, 0}, pres[3] = {0, 0, 0};
-    for (int f = 0; f < m.B; ++f) {
+    for (int f = f0; f < f1; ++f) {
         if (m.bct[f] != BC_SOLID_SURFACE) continue;
         int l = m.lc[f], r = m.C + f;
         double area = m.area[f], fnx = m.xfn[f], fny = m.yfn[f], fnz = m.zfn[f];
@@ -1027,14 +1050,26 @@ __global__ void k_aero(DevMesh m, DevFields d, Consts k, double* out) {
     for (int a = 0; a < 3; ++a) { out[a] = tot[a]; out[3 + a] = pres[a]; }
 }
 
-int aero(ofx_ctx* c, double* out4) {
+// Forces of zone z (its boundary faces [f0, f1), its constants and angles).
+int aero_zone(ofx_ctx* c, int z, double* out4) {
     double* buf = c->d.norms + 8;
-    k_aero<<<1, 64, 0, c->stream>>>(c->m, c->d, c->k, buf);
+    int f0 = 0, f1 = c->m.B;
+    Consts k = c->k;
+    const ofx_config* g = &c->cfg;
+    if (c->m.nz > 1) {
+        f0 = c->zb[z];
+        f1 = c->zb[z + 1];
+        const ZoneK& zk = c->zkh[z];
+        k.statecoef = zk.statecoef; k.reynolds = zk.reynolds; k.oreynolds = zk.oreynolds; k.const_cp = zk.const_cp;
+        for (int e = 0; e < 5; ++e) k.inflow[e] = zk.inflow[e];
+        g = &c->zcfg[z];
+    }
+    k_aero<<<1, 64, 0, c->stream>>>(c->m, c->d, k, buf, f0, f1);
     OFX_HIP(hipGetLastError());
     double h[6];
     OFX_HIP(hipMemcpyAsync(h, buf, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     OFX_HIP(hipStreamSynchronize(c->stream));
-    double aoa = c->cfg.aoa_degree * PI / 180.0, aos = c->cfg.sideslip_degree * PI / 180.0;
+    double aoa = g->aoa_degree * PI / 180.0, aos = g->sideslip_degree * PI / 180.0;
     double sina = std::sin(aoa), cosa = std::cos(aoa), sinb = std::sin(aos), cosb = std::cos(aos);
     double cl = -sina * h[0] + cosa * h[1];
     double cd = cosa * cosb * h[0] + sina * cosb * h[1] + sinb * h[2];
@@ -1042,5 +1077,7 @@ int aero(ofx_ctx* c, double* out4) {
     out4[0] = cl; out4[1] = cd; out4[2] = cdp; out4[3] = cd - cdp;
     return 0;
 }
+
+int aero(ofx_ctx* c, double* out4) { return aero_zone(c, 0, out4); }
 
 }  // namespace ofx

@@ -58,9 +58,10 @@ __device__ __forceinline__ double sa_bc_value(const DevMesh& m, const DevFields&
     return (vnb >= 0.0) ? s : k.sa_inflow;
 }
 
-__global__ void __launch_bounds__(TBS) k_sa_bc(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(TBS) k_sa_bc(DevMesh m, DevFields d, Consts k0) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.B) return;
+    const Consts k = zoned(k0, m, face_zone(m, f));
     int l = m.lc[f];
     d.turbq[m.C + f] = sa_bc_value(m, d, k, f, l, d.turbq[l]);
 }
@@ -78,9 +79,10 @@ int sa_boundary(ofx_ctx* c) {
 // codes/uturb/src/UTurbGrad.cpp:51-74): each cell sets the ghosts of its own
 // boundary faces (they depend only on the owner) and uses them.
 template <int DIM, int MAXF>
-__global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts k0) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
+    const Consts k = zoned(k0, m, cell_zone(m, i));
     const int T = m.T;
     int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
     int ent[MAXF], nb[MAXF];
@@ -139,9 +141,10 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
 // (UTurbVisFlux::CalcFaceVisFlux1Equ, codes/uturb/src/UTurbVisFlux.cpp:247-369).
 // saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
 template <int DIM>
-__global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k0) {
     int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
+    const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T, F = m.F;
     int l = m.lc[f], r = m.rc[f];
     bool bnd = f < m.B;
@@ -295,10 +298,12 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
 // gathered in ascending face order) -- TurbCalcRHS order
 // (codes/turb/src/TurbRhs.cpp:57-71).
 template <int MAXF>
-__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k, int par) {
-    double mindt = slot_result<false>(d.slots, par, 12, 0);
+__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k0, int par) {
+    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
+    mindt = zone_mindt(m, d, i, mindt);
+    const Consts k = zoned(k0, m, cell_zone(m, i));
     const int F = m.F;
     int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
     int ent[MAXF];
@@ -440,9 +445,10 @@ __global__ void __launch_bounds__(TBS) k_sa_update(DevMesh m, DevFields d) {
 // UPDATE_FLOWFIELD + CALC_BOUNDARY of the SA solver: new nu_tilde, eddy
 // viscosity, and the ghosts of the cell's boundary faces (vist copied from
 // the owner, nu_tilde from the SA boundary conditions).
-__global__ void __launch_bounds__(TBS) k_sa_update_fused(DevMesh m, DevFields d, Consts k) {
+__global__ void __launch_bounds__(TBS) k_sa_update_fused(DevMesh m, DevFields d, Consts k0) {
     int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
+    const Consts k = zoned(k0, m, cell_zone(m, i));
     double qOld = d.turbq[i], dqNew = d.turbdq[i];
     double qNew = qOld + dqNew;
     qNew = (1.0 - 1.0) * qOld + 1.0 * qNew;

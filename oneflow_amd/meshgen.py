@@ -690,3 +690,42 @@ def swept_wing(n_surf: int = 256, n_quad: int = 25, ratio: float = 1.2, reynolds
         return out
 
     return grid_from_polyhedra_np(X, Y, Z, tets, prisms, bcs)
+
+
+# ---------------------------------------------------------------- several zones
+
+def concat_grids(grids):
+    """One grid holding independent zones, in the layout ofx_zones_set expects:
+    cells zone after zone, then every zone's boundary faces, then every zone's
+    interior faces, each part in zone order with the zone's own order kept.
+    Returns (grid, cell_start[n+1], bface_start[n+1])."""
+    dim = grids[0].dim
+    if any(g.dim != dim for g in grids):
+        raise ValueError("concat_grids: all zones must have the same dimension")
+    n = len(grids)
+    node_off = np.concatenate([[0], np.cumsum([g.n_nodes for g in grids])])
+    cell_off = np.concatenate([[0], np.cumsum([g.n_cells for g in grids])])
+    bface_off = np.concatenate([[0], np.cumsum([g.n_bfaces for g in grids])])
+    fnc_b, fnc_i, fn_b, fn_i, lc_b, lc_i, rc_i, bct = [], [], [], [], [], [], [], []
+    for z, g in enumerate(grids):
+        nb = g.n_bfaces
+        off = np.concatenate([[0], np.cumsum(g.face_node_count)])
+        fnc_b.append(g.face_node_count[:nb])
+        fnc_i.append(g.face_node_count[nb:])
+        fn_b.append(g.face_nodes[:off[nb]] + node_off[z])
+        fn_i.append(g.face_nodes[off[nb]:] + node_off[z])
+        lc_b.append(g.lc[:nb] + cell_off[z])
+        lc_i.append(g.lc[nb:] + cell_off[z])
+        rc_i.append(g.rc[nb:] + cell_off[z])
+        bct.append(g.bc_type)
+    B = int(bface_off[-1])
+    out = Grid(dim=dim, x=np.concatenate([g.x for g in grids]), y=np.concatenate([g.y for g in grids]),
+               z=np.concatenate([g.z for g in grids]),
+               face_node_count=np.concatenate(fnc_b + fnc_i).astype(np.int32),
+               face_nodes=np.concatenate(fn_b + fn_i).astype(np.int32),
+               lc=np.concatenate(lc_b + lc_i).astype(np.int32),
+               rc=np.concatenate([-np.ones(B, np.int64)] + rc_i).astype(np.int32),
+               bc_type=np.concatenate(bct).astype(np.int32), bc_name_id=np.concatenate(bct).astype(np.int32),
+               n_cells=int(cell_off[-1]))
+    out.check()
+    return out, cell_off.astype(np.int32), bface_off.astype(np.int32)

@@ -64,6 +64,9 @@ def load_library():
     L.ofx_aero.argtypes = [P, dp]
     L.ofx_time_kernel.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
     L.ofx_copy_bench.argtypes = [P, ctypes.c_size_t, ctypes.c_int, dp]
+    L.ofx_zones_set.argtypes = [P, ctypes.c_int, ip, ip, ctypes.c_void_p]
+    L.ofx_zones_norms.argtypes = [P, ctypes.POINTER(OfxNorms)]
+    L.ofx_zones_aero.argtypes = [P, dp]
     L.ofx_set_face_tile.argtypes = [P, ctypes.c_int, ctypes.c_int]
     L.ofx_autotune.argtypes = [P, ctypes.c_int, ip]
     L.ofx_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, ctypes.c_int]
@@ -256,3 +259,52 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class ZoneSolver(Solver):
+    """Several independent cases in one context: their grids concatenated
+    (meshgen.concat_grids) and every kernel run over all of them at once, with
+    per-case freestream constants and reductions.  Each case's results equal
+    a Solver on that case alone, bit for bit."""
+
+    def __init__(self, grids, cfgs, device: int = 0):
+        from .meshgen import concat_grids
+        g, cs, bs = concat_grids(grids)
+        super().__init__(g, cfgs[0], device)
+        self.n_zones = len(grids)
+        self.cell_start, self.bface_start = cs, bs
+        self.zone_grids = list(grids)
+        self.zone_cfgs = list(cfgs)
+        arr = (type(self._c) * len(cfgs))(*[c.to_c() for c in cfgs])
+        self._zc = arr
+        self._check(self._L.ofx_zones_set(self.h, len(grids), _ip(cs), _ip(bs), ctypes.cast(arr, ctypes.c_void_p)))
+
+    def zone_norms(self):
+        arr = (OfxNorms * self.n_zones)()
+        self._check(self._L.ofx_zones_norms(self.h, arr))
+        return np.array([list(a.ns) + [a.sa] for a in arr])
+
+    def zone_aero(self):
+        a = np.zeros(4 * self.n_zones)
+        self._check(self._L.ofx_zones_aero(self.h, _dp(a)))
+        return a.reshape(-1, 4)
+
+    def zone_slice(self, z):
+        """(cell slice, ghost slice) of zone z in cell-indexed fields."""
+        C = self.n_cells
+        return (slice(int(self.cell_start[z]), int(self.cell_start[z + 1])),
+                slice(C + int(self.bface_start[z]), C + int(self.bface_start[z + 1])))
+
+    def get_zone(self, name, z):
+        """A cell field of zone z in the zone's own layout (cells, then ghosts)."""
+        a = self.get(name)
+        cs, gs = self.zone_slice(z)
+        return np.concatenate([a[..., cs], a[..., gs]], axis=-1)
+
+    def set_zone(self, name, z, value):
+        a = self.get(name)
+        cs, gs = self.zone_slice(z)
+        nc = cs.stop - cs.start
+        a[..., cs] = value[..., :nc]
+        a[..., gs] = value[..., nc:]
+        self.set(name, a)

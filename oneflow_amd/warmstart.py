@@ -86,6 +86,62 @@ def apply_prediction(s: Solver, case: AirfoilCase, cfg, y: np.ndarray) -> None:
     s.sa_boundary()
 
 
+def apply_prediction_zone(zs, z: int, case: AirfoilCase, cfg, y: np.ndarray) -> None:
+    """apply_prediction for zone z of a ZoneSolver (ghost cells are filled
+    afterwards, for all zones at once, by zs.ns_boundary / zs.sa_boundary)."""
+    nc = int(zs.cell_start[z + 1] - zs.cell_start[z])
+    ca, sa, p_inf, nu_inf = _free(case, cfg)
+    q = zs.get_zone("Q", z)
+    nu = zs.get_zone("TURBQ", z)
+    q[0, :nc] = np.maximum(1.0 + y[:, 0], 0.05)
+    q[1, :nc] = ca + y[:, 1]
+    q[2, :nc] = sa + y[:, 2]
+    q[3, :nc] = 0.0
+    q[4, :nc] = p_inf * np.maximum(1.0 + y[:, 3], 0.05)
+    nu[:nc] = nu_inf * np.power(10.0, 2.0 * np.clip(y[:, 4], -1.5, 2.5))
+    zs.set_zone("Q", z, q)
+    zs.set_zone("TURBQ", z, nu)
+
+
+class ZoneView:
+    """Zone z of a ZoneSolver seen as a one-case solver by the field readers
+    above (cell fields in the zone's own layout: cells, then ghosts)."""
+
+    def __init__(self, zs, z: int):
+        self.zs, self.z = zs, z
+        self.n_cells = int(zs.cell_start[z + 1] - zs.cell_start[z])
+        self.cfg = zs.zone_cfgs[z]
+
+    def get(self, name):
+        return self.zs.get_zone(name, self.z)
+
+
+def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None):
+    """All cases as zones of one context advance together; the run ends when
+    every zone's density residual norm is below tol * refs[z] (zones that
+    converge earlier keep iterating with the rest).  Returns (wall seconds,
+    step at which each zone first met its tolerance)."""
+    n = zs.n_zones
+    steps = [0] * n
+    zs.sync()
+    t0 = time.perf_counter()
+    ramp = zs.cfg.ncfl
+    it = 1
+    while True:
+        nstep = chunk + (ramp - it if it < ramp else 0)
+        zs.iterate(it, nstep, with_norms=False)
+        it += nstep
+        nr = zs.zone_norms()[:, 0]
+        for z in range(n):
+            if not steps[z] and (nr[z] <= tol * refs[z] or it > max_steps):
+                steps[z] = it - 1
+                if progress:
+                    progress(z, steps[z])
+        if all(steps):
+            break
+    return time.perf_counter() - t0, steps
+
+
 def solve_to_tolerance(solvers, refs, tol=1e-8, chunk=500, max_steps=150_000, batched=True, steps_per_graph=10,
                        progress=None):
     """Iterate every case until its density residual norm falls below
