@@ -297,11 +297,7 @@ def warmstart_main(args):
     zl = zones()
     zl.sync()
     t0 = time.perf_counter()
-    for z, c in enumerate(test):
-        v = ws.ZoneView(zl, z)
-        ws.apply_prediction_zone(zl, z, c, v.cfg, net.forward(ws.cell_features(v, c)))
-    zl.ns_boundary()
-    zl.sa_boundary()
+    ws.apply_predictions_zones(zl, test, net)
     zl.sync()
     t_inf = time.perf_counter() - t0
     t_zl, st_zl = ws.solve_zones_to_tolerance(zl, refs, tol=args.tol, progress=prog)

@@ -42,42 +42,67 @@ enum { EPI_PARTIAL = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3 };
 
 // C[M x N] = A[M x K] B[K x N]; A, B bf16 row-major (lda, ldb); M and N
 // multiples of 64; blockIdx.z selects the K range [z*kchunk, (z+1)*kchunk)
-// (kchunk a multiple of 16).  256 threads per 64 x 64 tile; wave w owns rows
+// (kchunk a multiple of KC).  256 threads per 64 x 64 tile; wave w owns rows
 // 32*(w>>1).. and columns 32*(w&1).. as 2 x 2 MFMA 16x16x16 tiles.  MFMA
 // operands: lane l supplies A[l%16][4*(l/16)+0..3] and B[4*(l/16)+0..3][l%16]
 // and receives D[4*(l/16)+r][l%16], r = 0..3.
-template <int EPI>
+// K advances KC at a time through LDS (one barrier pair per KC/16 MFMA
+// steps); the next chunk's global loads are issued into registers before the
+// current chunk's MFMAs, so their latency overlaps the arithmetic.  The MFMA
+// chain runs over k in ascending order of 16-wide steps for any KC.
+template <int EPI, int KC>
 __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
                                               int kchunk, int M, float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc,
                                               const float* __restrict__ bias, const float* __restrict__ zin,
                                               float* __restrict__ zout) {
-    __shared__ __align__(16) bf16 sA[64][16];
-    __shared__ __align__(16) bf16 sB[64][16];  // transposed: [n][k]
+    constexpr int PAD = 8;            // bf16 elements of row padding (LDS bank spread)
+    constexpr int NV = KC / 16;       // 4-element vectors per thread per operand tile
+    __shared__ __align__(16) bf16 sA[64][KC + PAD];
+    __shared__ __align__(16) bf16 sB[64][KC + PAD];  // transposed: [n][k]
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    const int kb = blockIdx.z * kchunk;
+    const int kb = blockIdx.z * kchunk, ke = kb + kchunk;
     v4f acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < kb + kchunk; k0 += 16) {
-        {   // A tile 64 x 16: thread t loads row t/4, k 4*(t%4)..+3
-            int r = t >> 2, kq = t & 3;
-            *(v4s*)&sA[r][4 * kq] = *(const v4s*)&A[(size_t)(m0 + r) * lda + k0 + 4 * kq];
-            // B tile 16 x 64: thread t loads k t/16, n 4*(t%16)..+3, stores transposed
-            int kk = t >> 4, nq = t & 15;
-            v4s v = *(const v4s*)&B[(size_t)(k0 + kk) * ldb + n0 + 4 * nq];
-            sB[4 * nq + 0][kk] = (bf16)v.x;
-            sB[4 * nq + 1][kk] = (bf16)v.y;
-            sB[4 * nq + 2][kk] = (bf16)v.z;
-            sB[4 * nq + 3][kk] = (bf16)v.w;
+    // A tile 64 x KC: vector e = u*256 + t is row e / (KC/4), k 4*(e % (KC/4))
+    // B tile KC x 64: vector e is k e / 16, n 4*(e % 16)
+    v4s ra[NV], rb[NV];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+            int e = u * 256 + t;
+            int r = e / (KC / 4), kq = e % (KC / 4);
+            ra[u] = *(const v4s*)&A[(size_t)(m0 + r) * lda + k0 + 4 * kq];
+            int kk = e >> 4, nq = e & 15;
+            rb[u] = *(const v4s*)&B[(size_t)(k0 + kk) * ldb + n0 + 4 * nq];
+        }
+    };
+    gload(kb);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+        __syncthreads();  // the previous chunk's MFMAs have read LDS
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+            int e = u * 256 + t;
+            int r = e / (KC / 4), kq = e % (KC / 4);
+            *(v4s*)&sA[r][4 * kq] = ra[u];
+            int kk = e >> 4, nq = e & 15;
+            sB[4 * nq + 0][kk] = (bf16)rb[u].x;
+            sB[4 * nq + 1][kk] = (bf16)rb[u].y;
+            sB[4 * nq + 2][kk] = (bf16)rb[u].z;
+            sB[4 * nq + 3][kk] = (bf16)rb[u].w;
         }
         __syncthreads();
-        v4s a[2], b[2];
-        for (int i = 0; i < 2; ++i) a[i] = *(const v4s*)&sA[32 * (w >> 1) + 16 * i + (lane & 15)][4 * (lane >> 4)];
-        for (int j = 0; j < 2; ++j) b[j] = *(const v4s*)&sB[32 * (w & 1) + 16 * j + (lane & 15)][4 * (lane >> 4)];
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-        __syncthreads();
+        if (k0 + KC < ke) gload(k0 + KC);
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 16) {
+            v4s a[2], b[2];
+            for (int i = 0; i < 2; ++i) a[i] = *(const v4s*)&sA[32 * (w >> 1) + 16 * i + (lane & 15)][ks + 4 * (lane >> 4)];
+            for (int j = 0; j < 2; ++j) b[j] = *(const v4s*)&sB[32 * (w & 1) + 16 * j + (lane & 15)][ks + 4 * (lane >> 4)];
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
     }
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
@@ -314,7 +339,11 @@ int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* B, int ldb, int M, int 
         return OFX_EINVAL;
     }
     dim3 grid(N / 64, M / 64, nsplit);
-    k_gemm<EPI><<<grid, 256, 0, n->stream>>>(A, lda, B, ldb, K / nsplit, M, Cf, Cb, ldc, bias, zin, zout);
+    int kc = K / nsplit;
+    if (kc % 64 == 0)
+        k_gemm<EPI, 64><<<grid, 256, 0, n->stream>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+    else
+        k_gemm<EPI, 16><<<grid, 256, 0, n->stream>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
     OFX_HIP(hipGetLastError());
     return 0;
 }

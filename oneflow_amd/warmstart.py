@@ -67,40 +67,60 @@ def cell_targets(s: Solver, case: AirfoilCase, cfg) -> np.ndarray:
     return t
 
 
+def _write_prediction(q, nu, case: AirfoilCase, cfg, y: np.ndarray) -> None:
+    """Predicted primitive state and nu_tilde of a case's cells into q[5][k], nu[k] (views)."""
+    ca, sa, p_inf, nu_inf = _free(case, cfg)
+    q[0] = np.maximum(1.0 + y[:, 0], 0.05)
+    q[1] = ca + y[:, 1]
+    q[2] = sa + y[:, 2]
+    q[3] = 0.0
+    q[4] = p_inf * np.maximum(1.0 + y[:, 3], 0.05)
+    nu[:] = nu_inf * np.power(10.0, 2.0 * np.clip(y[:, 4], -1.5, 2.5))
+
+
 def apply_prediction(s: Solver, case: AirfoilCase, cfg, y: np.ndarray) -> None:
     """Write a predicted field as the initial state (positivity-floored), then
     the boundary tasks fill ghosts, temperature and viscosities."""
-    C, T = s.n_cells, s.n_tcell
-    ca, sa, p_inf, nu_inf = _free(case, cfg)
+    C = s.n_cells
     q = s.get("Q")
     nu = s.get("TURBQ")
-    q[0, :C] = np.maximum(1.0 + y[:, 0], 0.05)
-    q[1, :C] = ca + y[:, 1]
-    q[2, :C] = sa + y[:, 2]
-    q[3, :C] = 0.0
-    q[4, :C] = p_inf * np.maximum(1.0 + y[:, 3], 0.05)
-    nu[:C] = nu_inf * np.power(10.0, 2.0 * np.clip(y[:, 4], -1.5, 2.5))
+    _write_prediction(q[:, :C], nu[:C], case, cfg, y)
     s.set("Q", q)
     s.set("TURBQ", nu)
     s.ns_boundary()
     s.sa_boundary()
 
 
-def apply_prediction_zone(zs, z: int, case: AirfoilCase, cfg, y: np.ndarray) -> None:
-    """apply_prediction for zone z of a ZoneSolver (ghost cells are filled
-    afterwards, for all zones at once, by zs.ns_boundary / zs.sa_boundary)."""
-    nc = int(zs.cell_start[z + 1] - zs.cell_start[z])
-    ca, sa, p_inf, nu_inf = _free(case, cfg)
-    q = zs.get_zone("Q", z)
-    nu = zs.get_zone("TURBQ", z)
-    q[0, :nc] = np.maximum(1.0 + y[:, 0], 0.05)
-    q[1, :nc] = ca + y[:, 1]
-    q[2, :nc] = sa + y[:, 2]
-    q[3, :nc] = 0.0
-    q[4, :nc] = p_inf * np.maximum(1.0 + y[:, 3], 0.05)
-    nu[:nc] = nu_inf * np.power(10.0, 2.0 * np.clip(y[:, 4], -1.5, 2.5))
-    zs.set_zone("Q", z, q)
-    zs.set_zone("TURBQ", z, nu)
+def apply_predictions_zones(zs, cases, net) -> None:
+    """The learned initial state of every zone of a ZoneSolver: the fields are
+    read once, one network forward covers all zones' cells, the state is
+    written once and the boundary tasks fill the ghosts of all zones."""
+    xcc, dist = zs.get("XCC"), zs.get("DIST")
+    q, nu = zs.get("Q"), zs.get("TURBQ")
+    feats, spans = [], []
+    for z, case in enumerate(cases):
+        cs, _ = zs.zone_slice(z)
+        feats.append(cell_features(_Fields(xcc[:, cs], dist[cs]), case))
+        spans.append(cs)
+    y = net.forward(np.concatenate(feats))
+    o = 0
+    for z, (case, cs) in enumerate(zip(cases, spans)):
+        n = cs.stop - cs.start
+        _write_prediction(q[:, cs], nu[cs], case, zs.zone_cfgs[z], y[o:o + n])
+        o += n
+    zs.set("Q", q)
+    zs.set("TURBQ", nu)
+    zs.ns_boundary()
+    zs.sa_boundary()
+
+
+class _Fields:
+    def __init__(self, xcc, dist):
+        self.n_cells = xcc.shape[1]
+        self._f = {"XCC": xcc, "DIST": dist}
+
+    def get(self, name):
+        return self._f[name]
 
 
 class ZoneView:
