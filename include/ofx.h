@@ -233,10 +233,16 @@ int ofx_zones_aero(ofx_ctx* ctx, double* out);
  * counterpart in the reference; a launch parameter). */
 int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
 
-/* Autotuner (no counterpart in the reference): times the NS face kernel and
- * the SA residual on the current state for each face tile over `iters`
- * launches, keeps the fastest, and reports them in chosen2[0..1] (may be
- * NULL).  Leaves res / turbres as one evaluation of the current state. */
+/* Kernel variant (no counterpart in the reference; results do not depend on
+ * it).  kernel 0, the NS gradient: variant 0 (default) one thread per cell,
+ * variant 1 one wave per quantity (more waves in flight on small grids). */
+int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
+
+/* Autotuner (no counterpart in the reference): times the NS face kernel for
+ * each face tile, the NS gradient for each variant and the SA residual for
+ * each SA face tile on the current state over `iters` launches, keeps the
+ * fastest, and reports the face tiles in chosen2[0..1] (may be NULL).
+ * Leaves res / turbres as one evaluation of the current state. */
 int ofx_autotune(ofx_ctx* ctx, int iters, int* chosen2);
 
 /* Batched cases (the reference solves zones one after another,

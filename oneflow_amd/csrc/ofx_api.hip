@@ -451,6 +451,17 @@ int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
     return 0;
 }
 
+int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
+    CHECK_CTX(c);
+    if (kernel != 0 || variant < 0 || variant > 1) {
+        set_error("kernel variant: kernel 0 (NS gradient) takes variant 0 or 1");
+        return OFX_EINVAL;
+    }
+    c->grad_variant = variant;
+    ++c->gen;
+    return 0;
+}
+
 int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     CHECK_READY(c);
     if (iters < 1) { set_error("autotune: iters >= 1"); return OFX_EINVAL; }
@@ -463,6 +474,14 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
         if (ms < best) { best = ms; pick = t; }
     }
     c->face_tb = pick;
+    best = 1e300;
+    pick = c->grad_variant;
+    for (int v = 0; v < 2; ++v) {
+        c->grad_variant = v;
+        OFX_TRY(ofx_time_kernel(c, 0, iters, &ms));
+        if (ms < best) { best = ms; pick = v; }
+    }
+    c->grad_variant = pick;
     if (c->k.vismodel == 3) {
         best = 1e300;
         pick = c->saface_tb;

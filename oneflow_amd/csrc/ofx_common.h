@@ -34,6 +34,8 @@ void set_error(const std::string& msg);
 
 // Device-resident mesh of one zone.  Face arrays are SoA of length F, cell
 // arrays of length T = C + B (ghost of boundary face f is cell C + f).
+constexpr int MAXSEG = 9;  // cell segments <= 8 (colours); faces: boundary + 8
+
 struct DevMesh {
     int dim = 2, nNodes = 0, C = 0, F = 0, B = 0, T = 0;
     int maxFacesPerCell = 0;
@@ -59,6 +61,12 @@ struct DevMesh {
     int nz = 1;
     int *zcell = nullptr, *zface = nullptr, *zwall = nullptr;
     const struct ZoneK* zk = nullptr;
+    // spatially interleaved segments of the cell and face numbering (the
+    // colours of a multicolour numbering; one segment otherwise): item starts,
+    // segc[nsegc] = C, segf[nsegf] = F.  Used to place one spatial region of
+    // every segment on the same XCD (xcd_block_seg).
+    int nsegc = 1, nsegf = 1;
+    int segc[MAXSEG + 1] = {0}, segf[MAXSEG + 1] = {0};
 };
 
 // The case-dependent constants (Mach, AoA, sideslip, Reynolds) of one zone.
@@ -127,6 +135,7 @@ struct ofx_ctx {
     std::vector<void*> allocs;
     int par_dt = 0, par_lim = 0, par_fix = 0;  // parity of the reduction buffers in use
     int face_tb = 256, saface_tb = 256;        // face-kernel tile (threads per block)
+    int grad_variant = 0;  // NS gradient: 0 one thread per cell, 1 one wave per quantity
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
     // several zones: cell / boundary-face starts (nz + 1 each), host copies of
     // the per-zone constants and configurations
@@ -162,6 +171,44 @@ __host__ __device__ __forceinline__ int xcd_remap(int b, int nblk) {
     return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 __device__ __forceinline__ int xcd_block(int nblk) { return xcd_remap((int)blockIdx.x, nblk); }
+
+// The same for an item range cut into nseg segments whose items are
+// spatially interleaved (a multicolour numbering stores each colour as one
+// segment): XCD x walks the x-th eighth of every segment, so one XCD's L2
+// holds one spatial region of every colour.  Segment s owns logical blocks
+// [ceil(start[s] / per), ceil(start[s+1] / per)); per segment each XCD gets
+// q_s blocks and the r_s remainder blocks go to the next XCDs round robin
+// (continuing across segments), so every XCD receives exactly the blocks
+// the hardware dispatches to it (physical block b on XCD b % 8).  Bijective on
+// [0, nblk) when nblk = ceil(start[nseg] / per); nseg = 1 is xcd_remap.
+__host__ __device__ __forceinline__ int xcd_remap_seg(int b, int nblk, int per, const int* start, int nseg) {
+    const int x = b & 7;
+    int slot = b >> 3, rr = 0;
+#pragma unroll
+    for (int s = 0; s < MAXSEG; ++s) {
+        if (s < nseg) {
+            int b0 = (start[s] + per - 1) / per;
+            int b1 = (s + 1 == nseg) ? nblk : (start[s + 1] + per - 1) / per;
+            int n = b1 - b0, q = n >> 3, r = n & 7;
+            int cnt = q + ((((x - rr) & 7) < r) ? 1 : 0);
+            if (slot < cnt) {
+                int before = x * q;
+                for (int y = 0; y < x; ++y) before += (((y - rr) & 7) < r) ? 1 : 0;
+                return b0 + before + slot;
+            }
+            slot -= cnt;
+            rr = (rr + r) & 7;
+        }
+    }
+    return b;  // not reached for a consistent launch
+}
+// Logical block of a launch over the cells (faces), `per` items per block.
+__device__ __forceinline__ int cell_block(const DevMesh& m, int per) {
+    return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segc, m.nsegc);
+}
+__device__ __forceinline__ int face_block(const DevMesh& m, int per) {
+    return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segf, m.nsegf);
+}
 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }

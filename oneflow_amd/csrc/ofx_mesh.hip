@@ -401,6 +401,28 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     order_by(lev, nlo, loOrd, c->sched.loOff);
     order_by(levU, nup, upOrd, c->sched.upOff);
     m.nLevLo = nlo; m.nLevUp = nup;
+    // Segments for XCD placement: a numbering whose cells are sorted by
+    // lower-sweep level with at most 8 levels is a colouring stored colour by
+    // colour; each colour is one segment, and the interior faces (sorted by
+    // left cell) split at the same cells after the boundary-face segment.
+    m.nsegc = 1; m.segc[0] = 0; m.segc[1] = C;
+    m.nsegf = 1; m.segf[0] = 0; m.segf[1] = F;
+    bool sorted = nlo <= MAXSEG - 1;
+    for (int i = 1; i < C && sorted; ++i) sorted = lev[i] >= lev[i - 1];
+    for (int f = B + 1; f < F && sorted; ++f) sorted = lc[f] >= lc[f - 1];
+    if (sorted && nlo > 1) {
+        m.nsegc = nlo;
+        for (int l = 0; l <= nlo; ++l) m.segc[l] = c->sched.loOff[l];
+        int ns = 0;
+        m.segf[ns++] = 0;
+        int f = B;
+        for (int l = 0; l < nlo; ++l) {
+            if (B > 0 || l > 0) m.segf[ns++] = f;
+            while (f < F && lc[f] < m.segc[l + 1]) ++f;
+        }
+        m.segf[ns] = F;
+        m.nsegf = ns;
+    }
     std::vector<int> wall;
     for (int f = 0; f < B; ++f) if (bct[f] == BC_SOLID_SURFACE) wall.push_back(f);
     m.nWall = (int)wall.size();

@@ -126,55 +126,85 @@ int ns_boundary(ofx_ctx* c) {
 // UTimeStep::CalcInvSpectrumField / CalcVisSpectrumField / CalcLocalTimeStep
 // (codes/uns/src/UTimeStep.cpp:190-416, codes/ns/src/TimeStep.cpp:43-133):
 // per cell, the face spectral radii are summed in ascending face order.
+template <int DIM, int MAXF>
 __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k0, int par) {
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     const int T = m.T;
     const int zone = (i < m.C) ? cell_zone(m, i) : -1;
     const Consts k = zoned(k0, m, zone);
+    const bool visc = k.vismodel > 0;
     double dtv = LARGE;
     if (i < m.C) {
+        // every load of the cell's faces is issued before the sums; each face
+        // is then evaluated with its own left / right roles, in face order.
+        // In 2-D w and the z components are zero and left out.
+        int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+        int ent[MAXF], nb[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            ent[j] = (j < n) ? m.c2f[b + j] : 0;
+            nb[j] = (j < n) ? m.c2n[b + j] : i;
+        }
+        double qi[5], vli = 0.0, vti = 0.0;
+        for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
+        if (visc) { vli = d.visl[i]; vti = d.vist[i]; }
+        double qn[MAXF][5], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF], vln[MAXF], vtn[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
+                for (int q = 0; q < 5; ++q) qn[j][q] = (DIM == 2 && q == 3) ? 0.0 : d.q[q * T + c];
+                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
+                ar[j] = m.area[f];
+                if (visc) { vln[j] = d.visl[c]; vtn[j] = d.vist[c]; }
+            }
+        }
         double invsr = 0.0, vissr = 0.0;
-        for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-            int e = m.c2f[kk];
-            int f = e >= 0 ? e : ~e;
-            int l = m.lc[f], r = m.rc[f];
-            double rl = d.q[l], ul = d.q[T + l], vl = d.q[2 * T + l], wl = d.q[3 * T + l], pl = d.q[4 * T + l];
-            double rr = d.q[r], ur = d.q[T + r], vr = d.q[2 * T + r], wr = d.q[3 * T + r], pr = d.q[4 * T + r];
-            double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], area = m.area[f];
-            double vnl = xfn * ul + yfn * vl + zfn * wl - 0.0;
-            double vnr = xfn * ur + yfn * vr + zfn * wr - 0.0;
-            double gama = 0.5 * (k.gama + k.gama);
-            double pm = 0.5 * (pl + pr), rm = 0.5 * (rl + rr);
-            double cm = sqrt(gama * pm / rm);
-            double vn = 0.5 * (vnl + vnr);
-            invsr += 0.5 * area * (dabs(vn) + cm);
-            if (k.vismodel > 0) {
-                double visl = 0.5 * (d.visl[l] + d.visl[r]);
-                double vist = 0.5 * (d.vist[l] + d.vist[r]);
-                double sr = 0.0;
-                if (k.visSRModel == 1) {
-                    double density = 0.5 * (rl + rr);
-                    double c1 = 4.0 / 3.0 * (visl + vist);
-                    double c2 = k.gama * (visl * k.oprl + vist * k.oprt);
-                    double c3 = 2.0 * dmax(c1, c2) / (k.reynolds * density);
-                    sr = (area * area) * c3;
-                } else if (k.visSRModel == 2) {
-                    double dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) +
-                                       zfn * (m.zcc[r] - m.zcc[l]));
-                    double viscosity = visl + vist;
-                    double density = 0.5 * (rl + rr);
-                    double c1 = 2.0 * viscosity / (density * dist * k.reynolds + SMALL);
-                    sr = 0.5 * c1 * area;
-                } else if (k.visSRModel == 3) {
-                    double density = 0.5 * (rl + rr);
-                    sr = (area * area) / (k.reynolds * density + SMALL);
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                bool own = ent[j] >= 0;  // this cell is the face's left cell
+                const double* ql = own ? qi : qn[j];
+                const double* qr = own ? qn[j] : qi;
+                double rl = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
+                double rr = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
+                double xfn = nx[j], yfn = ny[j], zfn = nz[j], area = ar[j];
+                double vnl = xfn * ul + yfn * vl + zfn * wl - 0.0;
+                double vnr = xfn * ur + yfn * vr + zfn * wr - 0.0;
+                double gama = 0.5 * (k.gama + k.gama);
+                double pm = 0.5 * (pl + pr), rm = 0.5 * (rl + rr);
+                double cm = sqrt(gama * pm / rm);
+                double vn = 0.5 * (vnl + vnr);
+                invsr += 0.5 * area * (dabs(vn) + cm);
+                if (visc) {
+                    double visl = own ? 0.5 * (vli + vln[j]) : 0.5 * (vln[j] + vli);
+                    double vist = own ? 0.5 * (vti + vtn[j]) : 0.5 * (vtn[j] + vti);
+                    double sr = 0.0;
+                    if (k.visSRModel == 1) {
+                        double density = 0.5 * (rl + rr);
+                        double c1 = 4.0 / 3.0 * (visl + vist);
+                        double c2 = k.gama * (visl * k.oprl + vist * k.oprt);
+                        double c3 = 2.0 * dmax(c1, c2) / (k.reynolds * density);
+                        sr = (area * area) * c3;
+                    } else if (k.visSRModel == 2) {
+                        int l = own ? i : nb[j], r = own ? nb[j] : i;
+                        double zl = (DIM == 3) ? m.zcc[l] : 0.0, zr = (DIM == 3) ? m.zcc[r] : 0.0;
+                        double dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) + zfn * (zr - zl));
+                        double viscosity = visl + vist;
+                        double density = 0.5 * (rl + rr);
+                        double c1 = 2.0 * viscosity / (density * dist * k.reynolds + SMALL);
+                        sr = 0.5 * c1 * area;
+                    } else if (k.visSRModel == 3) {
+                        double density = 0.5 * (rl + rr);
+                        sr = (area * area) / (k.reynolds * density + SMALL);
+                    }
+                    vissr += sr;
                 }
-                vissr += sr;
             }
         }
         double vol = m.vol[i];
         double dt = k.cfl * vol / invsr;
-        if (k.vismodel > 0 && k.visTimeStepModel > 0) {
+        if (visc && k.visTimeStepModel > 0) {
             double vts = k.cfl * vol / vissr;
             dt *= vts / (dt + vts);
         }
@@ -234,7 +264,17 @@ int ns_timestep(ofx_ctx* c, int iter) {
     }
     c->k.cfl = cfl;
     c->par_dt ^= 1;
-    k_ns_timestep<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_dt);
+    const int mf = c->m.maxFacesPerCell;
+    const DevMesh& m = c->m;
+    if (m.dim == 2) {
+        if (mf <= 4) k_ns_timestep<2, 4><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+        else if (mf <= 8) k_ns_timestep<2, 8><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+        else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
+    } else {
+        if (mf <= 6) k_ns_timestep<3, 6><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+        else if (mf <= 16) k_ns_timestep<3, 16><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+        else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
+    }
     if (c->m.nz > 1)
         k_zone_extract<false><<<c->m.nz, 64, 0, c->stream>>>(c->d.slots, c->par_dt, RQ_N, RQ_DTMIN, 1, c->m.nz, c->d.zred);
     OFX_HIP(hipGetLastError());
@@ -252,7 +292,7 @@ int ns_timestep(ofx_ctx* c, int iter) {
 // the z components and w, which are identically zero in 2-D.
 template <int DIM, int MAXF, bool MINMAX>
 __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k0, int par) {
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     const int T = m.T;
     const int zone = (i < m.C) ? cell_zone(m, i) : -1;
     const Consts k = zoned(k0, m, zone);
@@ -333,6 +373,82 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
     }
 }
 
+// The same gradient with one wave per quantity: a block covers 64 cells and
+// holds one wave for each of rho, u, v, (w,) p and T, so every thread gathers
+// a single quantity over its cell's faces.  Six (3-D) or five (2-D) times the
+// waves of k_ns_grad and a fraction of its registers: more loads in flight on
+// grids too small to fill the chip with one thread per cell.  Each quantity's
+// sums are formed as in k_ns_grad, bit for bit; the neighbour min/max of the
+// five primitives are reduced per wave.
+template <int DIM, int MAXF, bool MINMAX>
+__global__ void __launch_bounds__(384) k_ns_grad_q(DevMesh m, DevFields d, Consts k0, int par) {
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    // wave -> quantity: rho, u, v, (w,) p, T (index 5)
+    const int s = (DIM == 2) ? (w < 3 ? w : w + 1) : w;
+    const int i = cell_block(m, 64) * 64 + lane;
+    const int T = m.T;
+    const bool live = i < m.C;
+    const int zone = live ? cell_zone(m, i) : -1;
+    const Consts k = zoned(k0, m, zone);
+    double mn = 0.0, mx = 0.0;
+    if (live) {
+        int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+        int ent[MAXF], nb[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            ent[j] = (j < n) ? m.c2f[b + j] : 0;
+            nb[j] = (j < n) ? m.c2n[b + j] : i;
+        }
+        double vi = (s < 5) ? d.q[s * T + i] : temperature(d.q[i], d.q[4 * T + i], k.statecoef);
+        double xi = m.xcc[i], yi = m.ycc[i], zi = (DIM == 3) ? m.zcc[i] : 0.0;
+        double fx[MAXF], fy[MAXF], fz[MAXF], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF];
+        double cx[MAXF], cy[MAXF], cz[MAXF], vn[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
+                fx[j] = m.xfc[f]; fy[j] = m.yfc[f]; fz[j] = (DIM == 3) ? m.zfc[f] : 0.0;
+                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
+                ar[j] = m.area[f];
+                cx[j] = m.xcc[c]; cy[j] = m.ycc[c]; cz[j] = (DIM == 3) ? m.zcc[c] : 0.0;
+                vn[j] = (s < 5) ? d.q[s * T + c] : temperature(d.q[c], d.q[4 * T + c], k.statecoef);
+            }
+        }
+        mn = vi; mx = vi;
+        double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                bool own = ent[j] >= 0;
+                double xl = own ? xi : cx[j], yl = own ? yi : cy[j], zl = own ? zi : cz[j];
+                double xr = own ? cx[j] : xi, yr = own ? cy[j] : yi, zr = own ? cz[j] : zi;
+                double delt1 = sqrt(sqr3(fx[j] - xl, fy[j] - yl, fz[j] - zl));
+                double delt2 = sqrt(sqr3(fx[j] - xr, fy[j] - yr, fz[j] - zr));
+                double delta = 1.0 / (delt1 + delt2 + SMALL);
+                double cl = delt2 * delta, cr = delt1 * delta;
+                double fnxa = nx[j] * ar[j], fnya = ny[j] * ar[j], fnza = nz[j] * ar[j];
+                double value = own ? (cl * vi + cr * vn[j]) : (cl * vn[j] + cr * vi);
+                if (own) { acc[0] += fnxa * value; acc[1] += fnya * value; acc[2] += fnza * value; }
+                else { acc[0] -= fnxa * value; acc[1] -= fnya * value; acc[2] -= fnza * value; }
+                int f = own ? ent[j] : ~ent[j];
+                if (MINMAX && s < 5 && f >= m.B) { mn = dmin(mn, vn[j]); mx = dmax(mx, vn[j]); }
+            }
+        }
+        double ovol = 1.0 / m.vol[i];
+        for (int a = 0; a < DIM; ++a) d.grad[(a * 6 + s) * T + i] = acc[a] * ovol;
+        if (MINMAX && s < 5) {
+            mn -= vi; mx -= vi;
+            d.minmax[s * m.C + i] = mn;
+            d.minmax[(5 + s) * m.C + i] = mx;
+        }
+    }
+    if (MINMAX && s < 5) {  // wave-uniform: s depends on the wave only
+        double lo = live ? 0.0 - mn : 0.0, hi = live ? mx : 0.0;
+        zslot_reduce<1, true>(&lo, zone, d.slots, par, RQ_N, RQ_QMIN + s, m.nz);
+        zslot_reduce<1, true>(&hi, zone, d.slots, par, RQ_N, RQ_QMAX + s, m.nz);
+    }
+}
+
 // Venkatakrishnan / Barth limiter per cell: minimum over the cell's faces
 // (Limiter::CalcVencatLimiter / CalcBarthLimiter, codes/usolver/src/ULimiter.cpp:464-595).
 // LPC lanes per cell; the minimum is exact in any order.
@@ -357,7 +473,7 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
         __syncthreads();
     }
     const int T = m.T;
-    int gid = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int gid = cell_block(m, blockDim.x / LPC) * blockDim.x + threadIdx.x;
     int i = gid / LPC, j = gid % LPC;
     bool live = i < m.C;
     int b = 0, n = 0;
@@ -417,7 +533,7 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
 // gather accumulates them in FP64).
 template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32>
 __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k0) {
-    int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int f = face_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T;
@@ -546,7 +662,7 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
 template <int LPC, bool VISC, bool F32>
 __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
     const int T = m.T, F = m.F;
-    int gid = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int gid = cell_block(m, blockDim.x / LPC) * blockDim.x + threadIdx.x;
     int i = gid / LPC, j = gid % LPC;
     bool live = i < m.C;
     int b = 0, n = 0;
@@ -614,14 +730,19 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
     bool visc = k.vismodel != 0;
     bool lim = k.ilim == 1 || k.ilim == 2;
     if (parts & 1u) {
+        const bool split = c->grad_variant == 1;
+        const int qw = (DIM == 2) ? 5 : 6;  // waves (quantities) per block of k_ns_grad_q
+        const int nbq = (C1 + 63) / 64;
         if (lim) {
             c->par_lim ^= 1;
-            k_ns_grad<DIM, MAXF, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            if (split) k_ns_grad_q<DIM, MAXF, true><<<nbq, 64 * qw, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            else k_ns_grad<DIM, MAXF, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
             if (c->m.nz > 1)
                 k_zone_extract<true><<<c->m.nz, 256, 0, c->stream>>>(c->d.slots, c->par_lim, RQ_N, RQ_QMIN, 10,
                                                                      c->m.nz, c->d.zred);
         } else {
-            k_ns_grad<DIM, MAXF, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
+            if (split) k_ns_grad_q<DIM, MAXF, false><<<nbq, 64 * qw, 0, c->stream>>>(c->m, c->d, k, 0);
+            else k_ns_grad<DIM, MAXF, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
         }
     }
     if ((parts & 2u) && lim) {

@@ -80,7 +80,7 @@ int sa_boundary(ofx_ctx* c) {
 // boundary faces (they depend only on the owner) and uses them.
 template <int DIM, int MAXF>
 __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts k0) {
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const int T = m.T;
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
 // saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
 template <int DIM>
 __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k0) {
-    int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int f = face_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T, F = m.F;
@@ -300,7 +300,7 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
 template <int MAXF>
 __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k0, int par) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
-    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     mindt = zone_mindt(m, d, i, mindt);
     const Consts k = zoned(k0, m, cell_zone(m, i));

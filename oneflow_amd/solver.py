@@ -68,6 +68,7 @@ def load_library():
     L.ofx_zones_norms.argtypes = [P, ctypes.POINTER(OfxNorms)]
     L.ofx_zones_aero.argtypes = [P, dp]
     L.ofx_set_face_tile.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.ofx_set_kernel_variant.argtypes = [P, ctypes.c_int, ctypes.c_int]
     L.ofx_autotune.argtypes = [P, ctypes.c_int, ip]
     L.ofx_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, ctypes.c_int]
     L.ofx_batch_destroy.argtypes = [P]
@@ -202,8 +203,12 @@ class Solver:
         """Threads per block of the NS / SA face-flux kernels (64..512)."""
         self._check(self._L.ofx_set_face_tile(self.h, ns_tile, sa_tile))
 
+    def set_grad_variant(self, variant):
+        """NS gradient kernel: 0 one thread per cell, 1 one wave per quantity."""
+        self._check(self._L.ofx_set_kernel_variant(self.h, 0, variant))
+
     def autotune(self, iters=20):
-        """Pick the fastest face tiles on the current state; returns (ns_tile, sa_tile)."""
+        """Pick the fastest face tiles and gradient variant on the current state; returns (ns_tile, sa_tile)."""
         a = np.zeros(2, np.int32)
         self._check(self._L.ofx_autotune(self.h, iters, _ip(a)))
         return int(a[0]), int(a[1])
