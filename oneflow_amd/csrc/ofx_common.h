@@ -336,12 +336,14 @@ __device__ __forceinline__ Consts zoned(const Consts& k, const DevMesh& m, int z
     return r;
 }
 
-// Zone-aware grid reductions (contexts with several zones).  Buffer layout
-// [2 parities][nz][nq_total][NSLOT].  Every lane of the wave calls; lanes
-// without a cell pass zone = -1.  Each wave reduces the lanes of each zone it
-// holds (zones are contiguous cell ranges, so usually one) and issues one
-// atomic per zone and quantity.  The other parity's entries of these
-// quantities are re-armed by all blocks together.
+// Wave-level grid reductions (several zones, or kernels whose waves reduce
+// different quantities).  Buffer layout [2 parities][nz][nq_total][NSLOT].
+// Every lane of the wave calls; lanes without a cell pass zone = -1.  Each
+// wave reduces the lanes of each zone it holds (zones are contiguous cell
+// ranges, so usually one) and issues one atomic per zone and quantity.  The
+// other parity's entries of these quantities are re-armed by the calling
+// waves of all blocks together (waves of one block that reduce the same
+// quantities write the same identity values).
 template <int NQ, bool IS_MAX>
 __device__ __forceinline__ void zslot_reduce(const double* v, int zone, unsigned long long* buf, int par,
                                              int nq_total, int q0, int nz) {
@@ -367,10 +369,12 @@ __device__ __forceinline__ void zslot_reduce(const double* v, int zone, unsigned
         }
         todo &= ~__ballot(in);
     }
+    // re-arm by the calling wave's lanes in every block (a block may hold
+    // waves that reduce different quantities), grid-stride over the entries
     unsigned long long* other = buf + (size_t)(1 - par) * nz * nq_total * NSLOT;
     const unsigned long long ident = IS_MAX ? 0ull : (unsigned long long)__double_as_longlong(LARGE);
     size_t n = (size_t)nz * NQ * NSLOT;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    for (size_t t = (size_t)blockIdx.x * 64 + lane; t < n; t += (size_t)gridDim.x * 64) {
         size_t z = t / (NQ * NSLOT), r = t % (NQ * NSLOT);
         other[(z * nq_total + q0) * NSLOT + r] = ident;
     }

@@ -800,15 +800,14 @@ int fill_ghost_grad(ofx_ctx* c) {
 // PrepareData/PrepareDataFacePrim, NsLusgs::GetStandardFluxIncrement(-1) with
 // CalcFaceEigenValue and CalcDH, then CalcViscousTerm and AddFluxIncrement
 // (codes/uns/src/UNsLusgs.cpp:189-335, codes/ns/src/NsLusgs.cpp:121-228,340-368).
-__device__ __forceinline__ void ns_lusgs_face(const DevMesh& m, const DevFields& d, const Consts& k, int i, int f,
-                                              double* rhs0) {
-    const int T = m.T;
-    int l = m.lc[f], r = m.rc[f];
-    double xfn = m.xfn[f], yfn = m.yfn[f], zfn = m.zfn[f], vfn = 0.0, farea = m.area[f];
-    if (r == i) { int t = l; l = r; r = t; xfn = -xfn; yfn = -yfn; zfn = -zfn; vfn = -vfn; }
-    double pj[5], dqj[5];
-    for (int e = 0; e < 5; ++e) { pj[e] = d.q[e * T + r]; dqj[e] = d.dq[e * T + r]; }
-    double rl = d.q[l], ul = d.q[T + l], vl = d.q[2 * T + l], wl = d.q[3 * T + l], pl = d.q[4 * T + l];
+// The face term of UNsLusgs::Solve for cell i = l (after GCom::SetGeometry
+// has put i on the left): the split flux increment of the neighbour r and,
+// with viscosity, its viscous radius.  Own state (rl..pl) and neighbour data
+// are passed in, loaded by the caller.
+__device__ __forceinline__ void ns_lusgs_face(const Consts& k, double xfn, double yfn, double zfn, double vfn,
+                                              double farea, double rl, double ul, double vl, double wl, double pl,
+                                              const double* pj, const double* dqj, double visl, double vist,
+                                              double volr, double dist, double* rhs0) {
     double rr = pj[0], ur = pj[1], vr = pj[2], wr = pj[3], pr = pj[4];
     double gl = k.gama, gr = k.gama;
     double hl = gl / (gl - 1.0) * pl / rl + 0.5 * sqr3(ul, vl, wl);
@@ -854,16 +853,14 @@ __device__ __forceinline__ void ns_lusgs_face(const DevMesh& m, const DevFields&
     dfj[4] = lmd1 * dqj[4] - hmj * term1 + vn_fluid * term2;
     for (int e = 0; e < 5; ++e) dfj[e] *= farea;
     if (k.vismodel != 0) {
-        double visl = d.visl[r], vist = d.vist[r];
         double visrad;
         if (k.visSRModel == 1) {
             double density = 0.5 * (rl + rr);
             double c1 = 4.0 / 3.0 * (visl + vist);
             double cc2 = k.gama * (visl * k.oprl + vist * k.oprt);
             double c3 = 2.0 * dmax(c1, cc2) / (k.reynolds * density);
-            visrad = (farea * farea) * c3 / m.vol[r];
+            visrad = (farea * farea) * c3 / volr;
         } else {
-            double dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) + zfn * (m.zcc[r] - m.zcc[l]));
             double viscosity = visl + vist;
             double density = 0.5 * (rl + rr);
             double c1 = 2.0 * viscosity / (density * dist * k.reynolds + SMALL);
@@ -876,20 +873,59 @@ __device__ __forceinline__ void ns_lusgs_face(const DevMesh& m, const DevFields&
 
 // One cell of the lower (UPPER=false) or upper sweep.  Boundary faces are
 // skipped in both sweeps: their ghost increment is zero, so the reference's
-// upper-sweep contribution through them is exactly +0.
-template <bool UPPER>
+// upper-sweep contribution through them is exactly +0.  The cell's faces are
+// unrolled to MAXF: all neighbour loads are issued first, then the face
+// terms are added in ascending face order.
+template <bool UPPER, int MAXF>
 __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int i, double mindt1) {
     const int T = m.T;
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const double mindt = zone_mindt(m, d, i, mindt1);
+    const bool visc = k.vismodel != 0, sr1 = k.visSRModel == 1;
+    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    int ent[MAXF], nb[MAXF];
+    bool act[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        ent[j] = (j < n) ? m.c2f[b + j] : 0;
+        nb[j] = (j < n) ? m.c2n[b + j] : i;
+        int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+        act[j] = j < n && f >= m.B && (UPPER ? nb[j] > i : nb[j] < i);
+    }
+    double rl = d.q[i], ul = d.q[T + i], vl = d.q[2 * T + i], wl = d.q[3 * T + i], pl = d.q[4 * T + i];
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (visc && !sr1) { xi = m.xcc[i]; yi = m.ycc[i]; zi = m.zcc[i]; }
+    double nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF], pj[MAXF][5], dqj[MAXF][5], vl_[MAXF], vt_[MAXF];
+    double rx[MAXF], ry[MAXF], rz[MAXF];  // SR model 1: rx = vol of r; otherwise the centroid of r
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (act[j]) {
+            int f = ent[j] >= 0 ? ent[j] : ~ent[j], r = nb[j];
+            nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = m.zfn[f]; ar[j] = m.area[f];
+            for (int e = 0; e < 5; ++e) { pj[j][e] = d.q[e * T + r]; dqj[j][e] = d.dq[e * T + r]; }
+            if (visc) {
+                vl_[j] = d.visl[r]; vt_[j] = d.vist[r];
+                if (sr1) rx[j] = m.vol[r];
+                else { rx[j] = m.xcc[r]; ry[j] = m.ycc[r]; rz[j] = m.zcc[r]; }
+            }
+        }
+    }
     double rhs0[5] = {0, 0, 0, 0, 0};
-    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-        int e = m.c2f[kk];
-        int f = e >= 0 ? e : ~e;
-        if (f < m.B) continue;
-        int j = e >= 0 ? m.rc[f] : m.lc[f];
-        if (UPPER ? (j < i) : (j > i)) continue;
-        ns_lusgs_face(m, d, k, i, f, rhs0);
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (act[j]) {
+            // GCom::SetGeometry: the normal points out of i
+            bool own = ent[j] >= 0;
+            double xfn = own ? nx[j] : -nx[j], yfn = own ? ny[j] : -ny[j], zfn = own ? nz[j] : -nz[j];
+            double vfn = own ? 0.0 : -0.0;
+            double volr = 0.0, dist = 0.0;
+            if (visc) {
+                if (sr1) volr = rx[j];
+                else dist = dabs(xfn * (rx[j] - xi) + yfn * (ry[j] - yi) + zfn * (rz[j] - zi));
+            }
+            ns_lusgs_face(k, xfn, yfn, zfn, vfn, ar[j], rl, ul, vl, wl, pl, pj[j], dqj[j], vl_[j], vt_[j], volr, dist,
+                          rhs0);
+        }
     }
     double rad = ns_impsr(m, d, k, i, mindt);
     if (!UPPER) {
@@ -901,7 +937,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
 
 // Whole sweep in one workgroup: levels in order, a workgroup barrier between
 // levels (every cell of a level depends only on earlier levels).
-template <bool UPPER>
+template <bool UPPER, int MAXF>
 __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k, int par) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     const int* off = UPPER ? m.upOff : m.loOff;
@@ -909,18 +945,18 @@ __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, C
     int nl = UPPER ? m.nLevUp : m.nLevLo;
     for (int L = 0; L < nl; ++L) {
         for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
-            ns_lusgs_cell<UPPER>(m, d, k, ord[idx], mindt);
+            ns_lusgs_cell<UPPER, MAXF>(m, d, k, ord[idx], mindt);
         __syncthreads();
     }
 }
 
 // One level per launch (wide levels on large meshes).
-template <bool UPPER>
+template <bool UPPER, int MAXF>
 __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int par) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= e) return;
-    ns_lusgs_cell<UPPER>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx], mindt);
+    ns_lusgs_cell<UPPER, MAXF>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx], mindt);
 }
 
 // Launch policy: few wide levels (multicolour numbering) -> one grid-wide
@@ -931,24 +967,34 @@ bool use_level_launches(const std::vector<int>& off) {
     return nl <= 64;
 }
 
-int ns_lusgs(ofx_ctx* c) {
+template <int MAXF>
+static void launch_lusgs(ofx_ctx* c) {
     const DevMesh& m = c->m;
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
         const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
         if (!use_level_launches(off)) {
-            if (up) k_ns_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
-            else k_ns_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            if (up) k_ns_sweep_1wg<true, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            else k_ns_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
         } else {
             for (size_t L = 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
-                if (up) k_ns_sweep_level<true><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
-                else k_ns_sweep_level<false><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
+                if (up) k_ns_sweep_level<true, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
+                else k_ns_sweep_level<false, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
             }
         }
-        OFX_HIP(hipGetLastError());
     }
+}
+
+int ns_lusgs(ofx_ctx* c) {
+    int mf = c->m.maxFacesPerCell;
+    if (mf <= 4) launch_lusgs<4>(c);
+    else if (mf <= 6) launch_lusgs<6>(c);
+    else if (mf <= 8) launch_lusgs<8>(c);
+    else if (mf <= 16) launch_lusgs<16>(c);
+    else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
+    OFX_HIP(hipGetLastError());
     return 0;
 }
 

@@ -373,62 +373,86 @@ int sa_residual(ofx_ctx* c) {
 // UTurbLusgs lower / upper cell (codes/uturb/src/UTurbLusgs.cpp:90-304,
 // codes/turb/src/TurbLusgs.cpp:123-254): matrix_l when the cell is the face's
 // right cell (GCom::SetGeometry swap), matrix_r otherwise.
-template <bool UPPER>
+// The cell's faces are unrolled to MAXF: loads first, then the ordered sum.
+template <bool UPPER, int MAXF>
 __device__ __forceinline__ void sa_lusgs_cell(const DevMesh& m, const DevFields& d, int i) {
     const int F = m.F;
-    double rhs0 = 0.0;
-    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-        int e = m.c2f[kk];
-        int f = e >= 0 ? e : ~e;
-        if (f < m.B) continue;
-        int j = e >= 0 ? m.rc[f] : m.lc[f];
-        if (UPPER ? (j < i) : (j > i)) continue;
-        double mat = (e >= 0) ? d.saface[2 * F + f] : d.saface[1 * F + f];
-        rhs0 += mat * d.turbdq[j];
+    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    int ent[MAXF], nb[MAXF];
+    bool act[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        ent[j] = (j < n) ? m.c2f[b + j] : 0;
+        nb[j] = (j < n) ? m.c2n[b + j] : i;
+        int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+        act[j] = j < n && f >= m.B && (UPPER ? nb[j] > i : nb[j] < i);
     }
+    double mat[MAXF], dqn[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (act[j]) {
+            int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+            mat[j] = (ent[j] >= 0) ? d.saface[2 * F + f] : d.saface[1 * F + f];
+            dqn[j] = d.turbdq[nb[j]];
+        }
+    }
+    double rhs0 = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j)
+        if (act[j]) rhs0 += mat[j] * dqn[j];
     double rad = d.turbimpsr[i];
     if (!UPPER) { double t = d.turbres[i] - rhs0; t /= rad; d.turbdq[i] = t; }
     else { double t = -rhs0; t /= rad; d.turbdq[i] += t; }
 }
 
-template <bool UPPER>
+template <bool UPPER, int MAXF>
 __global__ void __launch_bounds__(1024) k_sa_sweep_1wg(DevMesh m, DevFields d) {
     const int* off = UPPER ? m.upOff : m.loOff;
     const int* ord = UPPER ? m.upOrder : m.loOrder;
     int nl = UPPER ? m.nLevUp : m.nLevLo;
     for (int L = 0; L < nl; ++L) {
         for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
-            sa_lusgs_cell<UPPER>(m, d, ord[idx]);
+            sa_lusgs_cell<UPPER, MAXF>(m, d, ord[idx]);
         __syncthreads();
     }
 }
 
-template <bool UPPER>
+template <bool UPPER, int MAXF>
 __global__ void __launch_bounds__(TBS) k_sa_sweep_level(DevMesh m, DevFields d, int b, int e) {
     int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= e) return;
-    sa_lusgs_cell<UPPER>(m, d, (UPPER ? m.upOrder : m.loOrder)[idx]);
+    sa_lusgs_cell<UPPER, MAXF>(m, d, (UPPER ? m.upOrder : m.loOrder)[idx]);
 }
 
-int sa_lusgs(ofx_ctx* c) {
-    if (c->k.vismodel != 3) return 0;
+template <int MAXF>
+static void launch_sa_lusgs(ofx_ctx* c) {
     const DevMesh& m = c->m;
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
         const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
         if (!use_level_launches(off)) {
-            if (up) k_sa_sweep_1wg<true><<<1, 1024, 0, c->stream>>>(m, c->d);
-            else k_sa_sweep_1wg<false><<<1, 1024, 0, c->stream>>>(m, c->d);
+            if (up) k_sa_sweep_1wg<true, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d);
+            else k_sa_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d);
         } else {
             for (size_t L = 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
-                if (up) k_sa_sweep_level<true><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
-                else k_sa_sweep_level<false><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
+                if (up) k_sa_sweep_level<true, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
+                else k_sa_sweep_level<false, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
             }
         }
-        OFX_HIP(hipGetLastError());
     }
+}
+
+int sa_lusgs(ofx_ctx* c) {
+    if (c->k.vismodel != 3) return 0;
+    int mf = c->m.maxFacesPerCell;
+    if (mf <= 4) launch_sa_lusgs<4>(c);
+    else if (mf <= 6) launch_sa_lusgs<6>(c);
+    else if (mf <= 8) launch_sa_lusgs<8>(c);
+    else if (mf <= 16) launch_sa_lusgs<16>(c);
+    else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
+    OFX_HIP(hipGetLastError());
     return 0;
 }
 
