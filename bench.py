@@ -47,23 +47,28 @@ def kernel_bytes(C, B, F, dim, flux_bytes=8):
     """Algorithmic bytes per launch of the NS residual kernels: every array the
     kernel must read or write counted once (FP64 values, int32 indices; the
     face fluxes are FP32 in the FP32-flux variant, flux_bytes = 4).  In 2-D the
-    w component and z coordinates are neither read nor written."""
+    w component and z coordinates are neither read nor written.  Static face
+    terms stored at mesh upload count as what the kernels read in place of the
+    metrics they are formed from: Green-Gauss weights and area-weighted normal
+    (4 doubles per face, 3-D + 1), face centre relative to both cell centres
+    (4 doubles, 3-D + 2), the viscous STD-gradient vector (4 doubles)."""
     T = C + B
     nnz = 2 * F - B                  # entries of the cell -> face lists
     nq = 4 if dim == 2 else 5        # primitive components carried
     D = dim
-    idx = 4 * (C + 1) + 4 * nnz      # c2f offsets + entries
+    idx = 4 * (C + 1) + 4 * nnz      # face-list counts + entries
+    gg = 8 * (4 + (D == 3)) * F      # Green-Gauss weights + area-weighted normal
+    rec = 8 * (4 + 2 * (D == 3)) * F  # face centre - left / right cell centre
     return {
-        # q, centroids, volume, c2f + c2n, face centre/normal/area -> grad of nq+T, min/max
-        "grad": 8 * nq * T + 8 * D * T + 8 * C + idx + 4 * nnz + 8 * (2 * D + 1) * F
-                + 8 * (nq + 1) * D * C + 8 * 2 * nq * C,
-        # grad, min/max, centroids, c2f, face centres -> limiter
-        "limiter": 8 * nq * D * C + 8 * 2 * nq * C + 8 * D * C + idx + 8 * D * F + 8 * nq * C,
-        # lc/rc, face normal/area/centre, q, centroids, grads (nq + T), limiter,
-        # laminar + eddy viscosity -> inviscid and viscous face fluxes
-        "face": 8 * F + 8 * (D + 1) * F + 8 * D * F + 8 * nq * T + 8 * D * T + 8 * (nq + 1) * D * T
+        # q, volume, face lists (face + neighbour), weights -> grad of nq+T, min/max
+        "grad": 8 * nq * T + 8 * C + idx + 4 * nnz + gg + 8 * (nq + 1) * D * C + 8 * 2 * nq * C,
+        # grad, min/max, face lists, centre offsets -> limiter
+        "limiter": 8 * nq * D * C + 8 * 2 * nq * C + idx + rec + 8 * nq * C,
+        # lc/rc, face normal/area, centre offsets, viscous vector, q, grads (nq + T),
+        # limiter, laminar + eddy viscosity -> inviscid and viscous face fluxes
+        "face": 8 * F + 8 * (D + 1) * F + rec + 32 * F + 8 * nq * T + 8 * (nq + 1) * D * T
                 + 8 * nq * T + 16 * T + 2 * flux_bytes * nq * F,
-        # c2f, both face fluxes -> residual
+        # face lists, both face fluxes -> residual
         "gather": idx + 2 * flux_bytes * nq * F + 8 * nq * C,
     }
 

@@ -46,7 +46,27 @@ struct DevMesh {
     double *xfc = nullptr, *yfc = nullptr, *zfc = nullptr;
     double *xcc = nullptr, *ycc = nullptr, *zcc = nullptr, *vol = nullptr;
     double *dist = nullptr;
+    // Static per-face terms, computed once from the metrics above by the same
+    // expressions the kernels would evaluate (so bit for bit those values):
+    //   fgg  = {cl, cr, nx A, ny A}: Green-Gauss inverse-distance weights of
+    //          the left / right cell and the area-weighted normal;
+    //   frec = {xf - xl, yf - yl, xf - xr, yf - yr}: face centre relative to
+    //          the left and right cell centres (reconstruction, limiter);
+    //   fz   = {nz A, zf - zl, zf - zr, 0} (3-D only).
+    double4 *fgg = nullptr, *frec = nullptr, *fz = nullptr;
+    //   fv0  = {dx, dy, dz, 1/|d|}: unit centre-to-centre vector of the STD
+    //          viscous face gradient (VisGeom, method 0);
+    //   fv2  = 5 per face: {d1, d2, angle1, angle2}, {dxnl, dynl, dznl, dxnr},
+    //          {dynr, dznr, t1x, t1y}, {t1z, t2x, t2y, t2z}, {fw1, fw2, 0, 0}
+    //          (method 2, TEST).
+    double4 *fv0 = nullptr, *fv2 = nullptr;
     int *lc = nullptr, *rc = nullptr, *bct = nullptr;
+    // The c2f / c2n lists again, cell-major with a fixed width ellw =
+    // maxFacesPerCell: entry j of cell i at j * C + i (0 / i past the cell's
+    // count).  A thread's loads of its cell's entries then depend on i only
+    // and a wave's loads of entry j are contiguous.
+    int ellw = 0;
+    int *ellf = nullptr, *elln = nullptr;
     // cell -> face CSR in ascending face id; entry = f if the cell is lc[f], ~f if rc[f]
     int *c2fOff = nullptr, *c2f = nullptr;
     int *c2n = nullptr;  // neighbour cell of each c2f entry (ghost C+f on boundary faces)
@@ -202,6 +222,18 @@ __host__ __device__ __forceinline__ int xcd_remap_seg(int b, int nblk, int per, 
     }
     return b;  // not reached for a consistent launch
 }
+// Entries 0..MAXF-1 of cell i's face list (cell-major copy; entries past the
+// list hold 0 / i and must be masked with the cell's count).
+template <int MAXF>
+__device__ __forceinline__ void cell_faces(const DevMesh& m, int i, int* ent, int* nb) {
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        bool in = j < m.ellw;
+        ent[j] = in ? m.ellf[(size_t)j * m.C + i] : 0;
+        nb[j] = in ? m.elln[(size_t)j * m.C + i] : i;
+    }
+}
+
 // Logical block of a launch over the cells (faces), `per` items per block.
 __device__ __forceinline__ int cell_block(const DevMesh& m, int per) {
     return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segc, m.nsegc);

@@ -7,6 +7,7 @@
 // The cell -> face lists and the LU-SGS level schedule are integer
 // preprocessing done once on the host.
 #include "ofx_common.h"
+#include "ofx_physics.h"
 
 #include <algorithm>
 #include <numeric>
@@ -139,6 +140,45 @@ __global__ void k_cell_metrics_3d(DevMesh m) {
 }
 
 // ghost mirror (codes/geometry/src/UnsGrid.cpp:668-690)
+// Static per-face terms (DevMesh::fgg / frec / fz).  The expressions are
+// those of the gradient (CalcGradGGCellWeight, codes/usolver/src/UGrad.cpp:88-149)
+// and reconstruction (LimField::CalcFaceValue, codes/usolver/src/ULimiter.cpp:181-214)
+// kernels, with z terms 0 in 2-D as there.
+template <int DIM>
+__global__ void k_face_static(DevMesh m) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    int l = m.lc[f], r = (f < m.B) ? m.C + f : m.rc[f];
+    double fx = m.xfc[f], fy = m.yfc[f], fz = (DIM == 3) ? m.zfc[f] : 0.0;
+    double xl = m.xcc[l], yl = m.ycc[l], zl = (DIM == 3) ? m.zcc[l] : 0.0;
+    double xr = m.xcc[r], yr = m.ycc[r], zr = (DIM == 3) ? m.zcc[r] : 0.0;
+    double delt1 = sqrt(sqr3(fx - xl, fy - yl, fz - zl));
+    double delt2 = sqrt(sqr3(fx - xr, fy - yr, fz - zr));
+    double delta = 1.0 / (delt1 + delt2 + SMALL);
+    double cl = delt2 * delta, cr = delt1 * delta;
+    double ar = m.area[f];
+    m.fgg[f] = make_double4(cl, cr, m.xfn[f] * ar, m.yfn[f] * ar);
+    m.frec[f] = make_double4(fx - xl, fy - yl, fx - xr, fy - yr);
+    if (DIM == 3) m.fz[f] = make_double4(m.zfn[f] * ar, fz - zl, fz - zr, 0.0);
+}
+
+// Viscous face-gradient geometry records (DevMesh::fv0 / fv2) from vis_geom.
+__global__ void k_face_vis_static(DevMesh m) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    int l = m.lc[f], r = (f < m.B) ? m.C + f : m.rc[f];
+    VisGeom G;
+    vis_geom(m, f, l, r, 0, G);
+    m.fv0[f] = make_double4(G.dx, G.dy, G.dz, G.ods);
+    vis_geom(m, f, l, r, 2, G);
+    double4* o = m.fv2 + 5 * (size_t)f;
+    o[0] = make_double4(G.d1, G.d2, G.angle1, G.angle2);
+    o[1] = make_double4(G.dxnl, G.dynl, G.dznl, G.dxnr);
+    o[2] = make_double4(G.dynr, G.dznr, G.t1[0], G.t1[1]);
+    o[3] = make_double4(G.t1[2], G.t2[0], G.t2[1], G.t2[2]);
+    o[4] = make_double4(G.fw1, G.fw2, 0.0, 0.0);
+}
+
 __global__ void k_ghost_centres(DevMesh m) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.B) return;
@@ -440,12 +480,27 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     double** ca[] = {&m.xcc, &m.ycc, &m.zcc, &m.vol};
     for (double** p : ca) OFX_TRY(A((void**)p, dT));
     OFX_TRY(A((void**)&m.dist, sizeof(double) * C));
+    OFX_TRY(A((void**)&m.fgg, sizeof(double4) * F));
+    OFX_TRY(A((void**)&m.frec, sizeof(double4) * F));
+    OFX_TRY(A((void**)&m.fz, sizeof(double4) * (m.dim == 3 ? F : 1)));
+    OFX_TRY(A((void**)&m.fv0, sizeof(double4) * F));
+    OFX_TRY(A((void**)&m.fv2, sizeof(double4) * 5 * (size_t)F));
     OFX_TRY(A((void**)&m.lc, sizeof(int) * F));
     OFX_TRY(A((void**)&m.rc, sizeof(int) * F));
     OFX_TRY(A((void**)&m.bct, sizeof(int) * std::max(1, B)));
     OFX_TRY(A((void**)&m.c2fOff, sizeof(int) * (T + 1)));
     OFX_TRY(A((void**)&m.c2f, sizeof(int) * std::max(1, off[T])));
     OFX_TRY(A((void**)&m.c2n, sizeof(int) * std::max(1, off[T])));
+    m.ellw = maxf;
+    std::vector<int> ellf((size_t)maxf * C, 0), elln((size_t)maxf * C, 0);
+    for (int i = 0; i < C; ++i)
+        for (int j = 0; j < maxf; ++j) {
+            bool in = j < off[i + 1] - off[i];
+            ellf[(size_t)j * C + i] = in ? c2f[off[i] + j] : 0;
+            elln[(size_t)j * C + i] = in ? c2n[off[i] + j] : i;
+        }
+    OFX_TRY(A((void**)&m.ellf, sizeof(int) * std::max<size_t>(1, ellf.size())));
+    OFX_TRY(A((void**)&m.elln, sizeof(int) * std::max<size_t>(1, elln.size())));
     OFX_TRY(A((void**)&m.loOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.upOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.loOff, sizeof(int) * (nlo + 1)));
@@ -462,6 +517,8 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(upload(c, m.c2fOff, off.data(), sizeof(int) * (T + 1)));
     OFX_TRY(upload(c, m.c2f, c2f.data(), sizeof(int) * off[T]));
     OFX_TRY(upload(c, m.c2n, c2n.data(), sizeof(int) * off[T]));
+    OFX_TRY(upload(c, m.ellf, ellf.data(), sizeof(int) * ellf.size()));
+    OFX_TRY(upload(c, m.elln, elln.data(), sizeof(int) * elln.size()));
     OFX_TRY(upload(c, m.loOrder, loOrd.data(), sizeof(int) * C));
     OFX_TRY(upload(c, m.upOrder, upOrd.data(), sizeof(int) * C));
     OFX_TRY(upload(c, m.loOff, c->sched.loOff.data(), sizeof(int) * (nlo + 1)));
@@ -477,6 +534,11 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     else k_cell_metrics_3d<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());
     if (B) k_ghost_centres<<<(B + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    OFX_HIP(hipGetLastError());
+    if (m.dim == 2) k_face_static<2><<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    else k_face_static<3><<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
+    OFX_HIP(hipGetLastError());
+    k_face_vis_static<<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());
     k_wall_distance<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());

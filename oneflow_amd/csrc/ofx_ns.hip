@@ -138,13 +138,9 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
         // every load of the cell's faces is issued before the sums; each face
         // is then evaluated with its own left / right roles, in face order.
         // In 2-D w and the z components are zero and left out.
-        int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+        const int n = m.c2fOff[i + 1] - m.c2fOff[i];
         int ent[MAXF], nb[MAXF];
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            ent[j] = (j < n) ? m.c2f[b + j] : 0;
-            nb[j] = (j < n) ? m.c2n[b + j] : i;
-        }
+        cell_faces<MAXF>(m, i, ent, nb);
         double qi[5], vli = 0.0, vti = 0.0;
         for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
         if (visc) { vli = d.visl[i]; vti = d.vist[i]; }
@@ -299,26 +295,19 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
     double mn[5], mx[5];
     bool live = i < m.C;
     if (live) {
-        int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+        const int n = m.c2fOff[i + 1] - m.c2fOff[i];
         int ent[MAXF], nb[MAXF];
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            ent[j] = (j < n) ? m.c2f[b + j] : 0;
-            nb[j] = (j < n) ? m.c2n[b + j] : i;
-        }
+        cell_faces<MAXF>(m, i, ent, nb);
         double qi[5];
         for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
-        double xi = m.xcc[i], yi = m.ycc[i], zi = (DIM == 3) ? m.zcc[i] : 0.0;
-        double fx[MAXF], fy[MAXF], fz[MAXF], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF];
-        double cx[MAXF], cy[MAXF], cz[MAXF], qn[MAXF][5];
+        double4 gw[MAXF];
+        double gz[MAXF], qn[MAXF][5];
 #pragma unroll
         for (int j = 0; j < MAXF; ++j) {
             if (j < n) {
                 int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-                fx[j] = m.xfc[f]; fy[j] = m.yfc[f]; fz[j] = (DIM == 3) ? m.zfc[f] : 0.0;
-                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
-                ar[j] = m.area[f];
-                cx[j] = m.xcc[c]; cy[j] = m.ycc[c]; cz[j] = (DIM == 3) ? m.zcc[c] : 0.0;
+                gw[j] = m.fgg[f];
+                gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
                 for (int e2 = 0; e2 < 5; ++e2) qn[j][e2] = (DIM == 2 && e2 == 3) ? 0.0 : d.q[e2 * T + c];
             }
         }
@@ -330,13 +319,8 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
         for (int j = 0; j < MAXF; ++j) {
             if (j < n) {
                 bool own = ent[j] >= 0;  // this cell is the face's left cell
-                double xl = own ? xi : cx[j], yl = own ? yi : cy[j], zl = own ? zi : cz[j];
-                double xr = own ? cx[j] : xi, yr = own ? cy[j] : yi, zr = own ? cz[j] : zi;
-                double delt1 = sqrt(sqr3(fx[j] - xl, fy[j] - yl, fz[j] - zl));
-                double delt2 = sqrt(sqr3(fx[j] - xr, fy[j] - yr, fz[j] - zr));
-                double delta = 1.0 / (delt1 + delt2 + SMALL);
-                double cl = delt2 * delta, cr = delt1 * delta;
-                double fnxa = nx[j] * ar[j], fnya = ny[j] * ar[j], fnza = nz[j] * ar[j];
+                double cl = gw[j].x, cr = gw[j].y;
+                double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
                 double tn = temperature(qn[j][0], qn[j][4], k.statecoef);
                 for (int s = 0; s < 6; ++s) {
                     double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
@@ -392,25 +376,18 @@ __global__ void __launch_bounds__(384) k_ns_grad_q(DevMesh m, DevFields d, Const
     const Consts k = zoned(k0, m, zone);
     double mn = 0.0, mx = 0.0;
     if (live) {
-        int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+        const int n = m.c2fOff[i + 1] - m.c2fOff[i];
         int ent[MAXF], nb[MAXF];
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            ent[j] = (j < n) ? m.c2f[b + j] : 0;
-            nb[j] = (j < n) ? m.c2n[b + j] : i;
-        }
+        cell_faces<MAXF>(m, i, ent, nb);
         double vi = (s < 5) ? d.q[s * T + i] : temperature(d.q[i], d.q[4 * T + i], k.statecoef);
-        double xi = m.xcc[i], yi = m.ycc[i], zi = (DIM == 3) ? m.zcc[i] : 0.0;
-        double fx[MAXF], fy[MAXF], fz[MAXF], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF];
-        double cx[MAXF], cy[MAXF], cz[MAXF], vn[MAXF];
+        double4 gw[MAXF];
+        double gz[MAXF], vn[MAXF];
 #pragma unroll
         for (int j = 0; j < MAXF; ++j) {
             if (j < n) {
                 int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-                fx[j] = m.xfc[f]; fy[j] = m.yfc[f]; fz[j] = (DIM == 3) ? m.zfc[f] : 0.0;
-                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
-                ar[j] = m.area[f];
-                cx[j] = m.xcc[c]; cy[j] = m.ycc[c]; cz[j] = (DIM == 3) ? m.zcc[c] : 0.0;
+                gw[j] = m.fgg[f];
+                gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
                 vn[j] = (s < 5) ? d.q[s * T + c] : temperature(d.q[c], d.q[4 * T + c], k.statecoef);
             }
         }
@@ -420,13 +397,8 @@ __global__ void __launch_bounds__(384) k_ns_grad_q(DevMesh m, DevFields d, Const
         for (int j = 0; j < MAXF; ++j) {
             if (j < n) {
                 bool own = ent[j] >= 0;
-                double xl = own ? xi : cx[j], yl = own ? yi : cy[j], zl = own ? zi : cz[j];
-                double xr = own ? cx[j] : xi, yr = own ? cy[j] : yi, zr = own ? cz[j] : zi;
-                double delt1 = sqrt(sqr3(fx[j] - xl, fy[j] - yl, fz[j] - zl));
-                double delt2 = sqrt(sqr3(fx[j] - xr, fy[j] - yr, fz[j] - zr));
-                double delta = 1.0 / (delt1 + delt2 + SMALL);
-                double cl = delt2 * delta, cr = delt1 * delta;
-                double fnxa = nx[j] * ar[j], fnya = ny[j] * ar[j], fnza = nz[j] * ar[j];
+                double cl = gw[j].x, cr = gw[j].y;
+                double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
                 double value = own ? (cl * vi + cr * vn[j]) : (cl * vn[j] + cr * vi);
                 if (own) { acc[0] += fnxa * value; acc[1] += fnya * value; acc[2] += fnza * value; }
                 else { acc[0] -= fnxa * value; acc[1] -= fnya * value; acc[2] -= fnza * value; }
@@ -476,16 +448,19 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
     int gid = cell_block(m, blockDim.x / LPC) * blockDim.x + threadIdx.x;
     int i = gid / LPC, j = gid % LPC;
     bool live = i < m.C;
-    int b = 0, n = 0;
-    if (live) { b = m.c2fOff[i]; n = m.c2fOff[i + 1] - b; }
+    int n = 0;
+    if (live) n = m.c2fOff[i + 1] - m.c2fOff[i];
+    // entry j of the cell's face list, loaded without waiting for the count
+    const int ej = (live && j < m.ellw) ? m.ellf[(size_t)j * m.C + i] : 0;
     bool has = live && j < n;
     double dx = 0, dy = 0, dz = 0, dot = 0;
     if (has) {
-        int e = m.c2f[b + j];
+        int e = ej;
         int f = e >= 0 ? e : ~e;
-        dx = m.xfc[f] - m.xcc[i];
-        dy = m.yfc[f] - m.ycc[i];
-        dz = (DIM == 3) ? m.zfc[f] - m.zcc[i] : 0.0;
+        double4 rc4 = m.frec[f];  // face centre relative to the left / right cell centre
+        dx = e >= 0 ? rc4.x : rc4.z;
+        dy = e >= 0 ? rc4.y : rc4.w;
+        dz = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
         if (LIM == 1) {
             double ds = sqrt(sqr3(dx, dy, dz));
             double nz = (DIM == 3) ? m.zfn[f] : 0.0;
@@ -559,10 +534,9 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
     } else if (LIM == -1) {
         for (int e = 0; e < 5; ++e) { p1[e] = ql[e]; p2[e] = qr[e]; }
     } else {
-        double xfc = m.xfc[f], yfc = m.yfc[f], zfc = (DIM == 3) ? m.zfc[f] : 0.0;
-        double zl = (DIM == 3) ? m.zcc[l] : 0.0, zr = (DIM == 3) ? m.zcc[r] : 0.0;
-        double dxl = xfc - m.xcc[l], dyl = yfc - m.ycc[l], dzl = zfc - zl;
-        double dxr = xfc - m.xcc[r], dyr = yfc - m.ycc[r], dzr = zfc - zr;
+        double4 rc4 = m.frec[f];
+        double dxl = rc4.x, dyl = rc4.y, dxr = rc4.z, dyr = rc4.w, dzl = 0.0, dzr = 0.0;
+        if (DIM == 3) { double4 z4 = m.fz[f]; dzl = z4.y; dzr = z4.z; }
         for (int e = 0; e < 5; ++e) {
             if (DIM == 2 && e == 3) {
                 // w and its gradient are zero in 2-D: the reconstruction leaves w = 0
@@ -603,7 +577,7 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
 
     if (!VISC) return;
     VisGeom G;
-    vis_geom(m, f, l, r, k.ivischeme, G);
+    vis_geom_pre(m, f, k.ivischeme, G);
     double tl = temperature(ql[0], ql[4], k.statecoef);
     double tr = temperature(qr[0], qr[4], k.statecoef);
     double gq[6][3], qfv[6], dqdn[6];
@@ -665,12 +639,14 @@ __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
     int gid = cell_block(m, blockDim.x / LPC) * blockDim.x + threadIdx.x;
     int i = gid / LPC, j = gid % LPC;
     bool live = i < m.C;
-    int b = 0, n = 0;
-    if (live) { b = m.c2fOff[i]; n = m.c2fOff[i + 1] - b; }
+    int n = 0;
+    if (live) n = m.c2fOff[i + 1] - m.c2fOff[i];
+    // entry j of the cell's face list, loaded without waiting for the count
+    const int ej = (live && j < m.ellw) ? m.ellf[(size_t)j * m.C + i] : 0;
     bool has = live && j < n;
     double fi[5] = {0, 0, 0, 0, 0}, fv[5] = {0, 0, 0, 0, 0};
     if (has) {
-        int e = m.c2f[b + j];
+        int e = ej;
         int f = e >= 0 ? e : ~e;
         const float* inv32 = reinterpret_cast<const float*>(d.invflux);
         const float* vis32 = reinterpret_cast<const float*>(d.visflux);
@@ -882,13 +858,12 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const double mindt = zone_mindt(m, d, i, mindt1);
     const bool visc = k.vismodel != 0, sr1 = k.visSRModel == 1;
-    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
+    cell_faces<MAXF>(m, i, ent, nb);
     bool act[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        ent[j] = (j < n) ? m.c2f[b + j] : 0;
-        nb[j] = (j < n) ? m.c2n[b + j] : i;
         int f = ent[j] >= 0 ? ent[j] : ~ent[j];
         act[j] = j < n && f >= m.B && (UPPER ? nb[j] > i : nb[j] < i);
     }

@@ -207,6 +207,25 @@ __device__ __forceinline__ void vis_geom(const DevMesh& m, int f, int l, int r, 
     }
 }
 
+// The same geometry read from the per-face records the mesh upload stores
+// (DevMesh::fv0, fv2), which vis_geom itself computed.
+__device__ __forceinline__ void vis_geom_pre(const DevMesh& m, int f, int method, VisGeom& G) {
+    G.xfn = m.xfn[f]; G.yfn = m.yfn[f]; G.zfn = m.zfn[f];
+    G.bc_nonint = f < m.B;
+    if (method == 0) {
+        double4 a = m.fv0[f];
+        G.dx = a.x; G.dy = a.y; G.dz = a.z; G.ods = a.w;
+    } else if (method == 2) {
+        const double4* r = m.fv2 + 5 * (size_t)f;
+        double4 a = r[0], b = r[1], c = r[2], e = r[3], w = r[4];
+        G.fw1 = w.x; G.fw2 = w.y;
+        G.d1 = a.x; G.d2 = a.y; G.angle1 = a.z; G.angle2 = a.w;
+        G.dxnl = b.x; G.dynl = b.y; G.dznl = b.z; G.dxnr = b.w;
+        G.dynr = c.x; G.dznr = c.y; G.t1[0] = c.z; G.t1[1] = c.w;
+        G.t1[2] = e.x; G.t2[0] = e.y; G.t2[1] = e.z; G.t2[2] = e.w;
+    }
+}
+
 // One scalar through VisGrad (AverGrad, then STD / AVER / TEST method;
 // codes/usolver/src/VisGrad.cpp:130-365).  gf = face gradient, dqdn = normal derivative.
 __device__ __forceinline__ void vis_grad(int method, const VisGeom& G, double q1, double q2, const double* g1,

@@ -84,24 +84,18 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
     if (i >= m.C) return;
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const int T = m.T;
-    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
-#pragma unroll
-    for (int j = 0; j < MAXF; ++j) {
-        ent[j] = (j < n) ? m.c2f[b + j] : 0;
-        nb[j] = (j < n) ? m.c2n[b + j] : i;
-    }
+    cell_faces<MAXF>(m, i, ent, nb);
     double nui = d.turbq[i];
-    double xi = m.xcc[i], yi = m.ycc[i], zi = (DIM == 3) ? m.zcc[i] : 0.0;
-    double fx[MAXF], fy[MAXF], fz[MAXF], nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF], cx[MAXF], cy[MAXF], cz[MAXF], vn[MAXF];
+    double4 gw[MAXF];
+    double gz[MAXF], vn[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         if (j < n) {
             int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-            fx[j] = m.xfc[f]; fy[j] = m.yfc[f]; fz[j] = (DIM == 3) ? m.zfc[f] : 0.0;
-            nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
-            ar[j] = m.area[f];
-            cx[j] = m.xcc[c]; cy[j] = m.ycc[c]; cz[j] = (DIM == 3) ? m.zcc[c] : 0.0;
+            gw[j] = m.fgg[f];
+            gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
             if (f < m.B) {
                 vn[j] = sa_bc_value(m, d, k, f, i, nui);
                 d.turbq[c] = vn[j];
@@ -115,14 +109,9 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
     for (int j = 0; j < MAXF; ++j) {
         if (j < n) {
             bool own = ent[j] >= 0;
-            double xl = own ? xi : cx[j], yl = own ? yi : cy[j], zl = own ? zi : cz[j];
-            double xr = own ? cx[j] : xi, yr = own ? cy[j] : yi, zr = own ? cz[j] : zi;
-            double delt1 = sqrt(sqr3(fx[j] - xl, fy[j] - yl, fz[j] - zl));
-            double delt2 = sqrt(sqr3(fx[j] - xr, fy[j] - yr, fz[j] - zr));
-            double delta = 1.0 / (delt1 + delt2 + SMALL);
-            double cl = delt2 * delta, cr = delt1 * delta;
+            double cl = gw[j].x, cr = gw[j].y;
             double value = own ? (cl * nui + cr * vn[j]) : (cl * vn[j] + cr * nui);
-            double fnxa = nx[j] * ar[j], fnya = ny[j] * ar[j], fnza = nz[j] * ar[j];
+            double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
             if (own) { ax += fnxa * value; ay += fnya * value; az += fnza * value; }
             else { ax -= fnxa * value; ay -= fnya * value; az -= fnza * value; }
         }
@@ -195,7 +184,7 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     double conv = farea * (vnl_p * 1.0 * cnl + vnr_n * 1.0 * cnr);
     // diffusion
     VisGeom G;
-    vis_geom(m, f, l, r, k.iturb_visflux, G);
+    vis_geom_pre(m, f, k.iturb_visflux, G);
     double g1[3] = {d.tgrad[l], d.tgrad[T + l], (DIM == 3) ? d.tgrad[2 * T + l] : 0.0};
     double g2[3] = {d.tgrad[rg], d.tgrad[T + rg], (DIM == 3) ? d.tgrad[2 * T + rg] : 0.0};
     double qf, gf[3], dqdn;
@@ -305,10 +294,10 @@ __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts 
     mindt = zone_mindt(m, d, i, mindt);
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const int F = m.F;
-    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF];
 #pragma unroll
-    for (int j = 0; j < MAXF; ++j) ent[j] = (j < n) ? m.c2f[b + j] : 0;
+    for (int j = 0; j < MAXF; ++j) ent[j] = (j < m.ellw) ? m.ellf[(size_t)j * m.C + i] : 0;
     double rad[MAXF], cv[MAXF], df[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
@@ -377,13 +366,12 @@ int sa_residual(ofx_ctx* c) {
 template <bool UPPER, int MAXF>
 __device__ __forceinline__ void sa_lusgs_cell(const DevMesh& m, const DevFields& d, int i) {
     const int F = m.F;
-    int b = m.c2fOff[i], n = m.c2fOff[i + 1] - b;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
+    cell_faces<MAXF>(m, i, ent, nb);
     bool act[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        ent[j] = (j < n) ? m.c2f[b + j] : 0;
-        nb[j] = (j < n) ? m.c2n[b + j] : i;
         int f = ent[j] >= 0 ? ent[j] : ~ent[j];
         act[j] = j < n && f >= m.B && (UPPER ? nb[j] > i : nb[j] < i);
     }
