@@ -192,6 +192,13 @@ int ofx_iterate(ofx_ctx* ctx, int first_iter, int n, ofx_norms* norms_out);
  * (codes/force/src/AeroForceTask.cpp:186-380). */
 int ofx_aero(ofx_ctx* ctx, double* out4);
 
+/* DUMP_PRESSURE_COEFF (the idump_pres rows of CalcAeroForce,
+ * codes/force/src/AeroForceTask.cpp:242-262,426-440, written to wallaeroFile):
+ * the number of wall faces, and per wall face in face order x, y, z of the
+ * face centre, -cp and cf -> out[5 k .. 5 k + 4]. */
+int ofx_wall_count(ofx_ctx* ctx, int* n);
+int ofx_wall_data(ofx_ctx* ctx, double* out);
+
 /* Kernel timing (no counterpart in the reference; measurement only):
  * launches one part of the residual `iters` times back to back on the
  * context's stream between two HIP events, after one warm-up launch, and

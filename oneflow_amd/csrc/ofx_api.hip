@@ -174,6 +174,8 @@ ZoneK zone_consts(const ofx_config& g) {
     z.inflow[2] = 1.0 * std::sin(aoa) * std::cos(aos);
     z.inflow[3] = 1.0 * std::sin(aos);
     z.inflow[4] = z.statecoef * 1.0 * 1.0 / 1.0;
+    z.cosa = std::cos(aoa);
+    z.sina = std::sin(aoa);
     return z;
 }
 
@@ -404,6 +406,20 @@ int ofx_iterate(ofx_ctx* c, int first_iter, int n, ofx_norms* norms_out) {
     }
     OFX_HIP(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int ofx_wall_count(ofx_ctx* c, int* n) {
+    CHECK_CTX(c);
+    if (!c->has_mesh) { set_error("no mesh"); return OFX_ESTATE; }
+    if (!n) { set_error("null out"); return OFX_EINVAL; }
+    *n = c->m.nWall;
+    return 0;
+}
+
+int ofx_wall_data(ofx_ctx* c, double* out) {
+    CHECK_READY(c);
+    if (!out) { set_error("null out"); return OFX_EINVAL; }
+    return wall_data(c, out);
 }
 
 int ofx_aero(ofx_ctx* c, double* out4) {

@@ -93,11 +93,15 @@ struct DevMesh {
 struct ZoneK {
     double statecoef, reynolds, oreynolds, const_cp;
     double inflow[5];
+    double cosa, sina;  // of the angle of attack (wall friction sign)
 };
 
 // Host copies of the level schedule sizes for launch decisions.
 struct HostSched {
     std::vector<int> loOff, upOff;
+    // per level: first cell when the level's cells are one ascending run of
+    // consecutive cells (a colouring stored colour by colour), else -1
+    std::vector<int> loBase, upBase;
 };
 
 struct Consts {
@@ -450,6 +454,7 @@ int norms(ofx_ctx* c, ofx_norms* out);
 int wall_distance(ofx_ctx* c);
 int norms_range(ofx_ctx* c, int c0, int c1, ofx_norms* out);
 int aero_zone(ofx_ctx* c, int z, double* out4);
+int wall_data(ofx_ctx* c, double* host_out);
 int fill_ghost_grad(ofx_ctx* c);
 int materialize_timestep(ofx_ctx* c);
 int ns_update_fused(ofx_ctx* c);

@@ -440,6 +440,17 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     std::vector<int> loOrd, upOrd;
     order_by(lev, nlo, loOrd, c->sched.loOff);
     order_by(levU, nup, upOrd, c->sched.upOff);
+    auto runs = [](const std::vector<int>& ord, const std::vector<int>& lo, std::vector<int>& base) {
+        base.assign(lo.size() - 1, -1);
+        for (size_t L = 0; L + 1 < lo.size(); ++L) {
+            int b = lo[L], e = lo[L + 1];
+            bool run = e > b;
+            for (int k = b + 1; k < e && run; ++k) run = ord[k] == ord[b] + (k - b);
+            if (run) base[L] = ord[b];
+        }
+    };
+    runs(loOrd, c->sched.loOff, c->sched.loBase);
+    runs(upOrd, c->sched.upOff, c->sched.upBase);
     m.nLevLo = nlo; m.nLevUp = nup;
     // Segments for XCD placement: a numbering whose cells are sorted by
     // lower-sweep level with at most 8 levels is a colouring stored colour by

@@ -926,12 +926,15 @@ __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, C
 }
 
 // One level per launch (wide levels on large meshes).
+// base >= 0: the level's cells are base, base + 1, ... (no order lookup).
 template <bool UPPER, int MAXF>
-__global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int par) {
+__global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int base,
+                                                       int par) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= e) return;
-    ns_lusgs_cell<UPPER, MAXF>(m, d, k, (UPPER ? m.upOrder : m.loOrder)[idx], mindt);
+    int i = base >= 0 ? base + (idx - b) : (UPPER ? m.upOrder : m.loOrder)[idx];
+    ns_lusgs_cell<UPPER, MAXF>(m, d, k, i, mindt);
 }
 
 // Launch policy: few wide levels (multicolour numbering) -> one grid-wide
@@ -952,11 +955,14 @@ static void launch_lusgs(ofx_ctx* c) {
             if (up) k_ns_sweep_1wg<true, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
             else k_ns_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
         } else {
+            const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
             for (size_t L = 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
-                if (up) k_ns_sweep_level<true, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
-                else k_ns_sweep_level<false, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, c->par_dt);
+                if (up)
+                    k_ns_sweep_level<true, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L], c->par_dt);
+                else
+                    k_ns_sweep_level<false, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L], c->par_dt);
             }
         }
     }
@@ -1146,50 +1152,96 @@ int norms(ofx_ctx* c, ofx_norms* out) { return norms_range(c, 0, c->m.C, out); }
 
 // CalcAeroForce per wall face (codes/force/src/AeroForceTask.cpp:186-440), the
 // sum in face order on one thread as the reference loops.
+// Pressure and viscous force of wall face f and its pressure coefficient
+// (CalcAeroForce's face loop, codes/force/src/AeroForceTask.cpp:365-424, with
+// Stress::CalcForce, codes/solver/src/Stress.cpp).  bcdq* = owner gradient.
+__device__ __forceinline__ void wall_face_force(const DevMesh& m, const DevFields& d, const Consts& k, int f,
+                                                double* pf, double* vf, double& cp) {
+    const int T = m.T;
+    int l = m.lc[f], r = m.C + f;
+    double area = m.area[f], fnx = m.xfn[f], fny = m.yfn[f], fnz = m.zfn[f];
+    double fanx = fnx * area, fany = fny * area, fanz = fnz * area;
+    double dx = m.xfc[f] - m.xcc[l], dy = m.yfc[f] - m.ycc[l], dz = m.zfc[f] - m.zcc[l];
+    double gp[3] = {d.grad[(0 * 6 + 4) * T + l], d.grad[(1 * 6 + 4) * T + l], d.grad[(2 * 6 + 4) * T + l]};
+    double wp = d.q[4 * T + l] + gp[0] * dx + gp[1] * dy + gp[2] * dz;
+    if (wp < 0.0) wp = d.q[4 * T + l];
+    cp = 2.0 * (wp - k.inflow[4]);
+    pf[0] = fanx * cp; pf[1] = fany * cp; pf[2] = fanz * cp;
+    vf[0] = vf[1] = vf[2] = 0.0;
+    if (k.vismodel > 0) {
+        double g[3][3];
+        for (int s = 0; s < 3; ++s)
+            for (int a = 0; a < 3; ++a) g[s][a] = d.grad[(a * 6 + 1 + s) * T + l];
+        double ex = m.xcc[r] - m.xcc[l], ey = m.ycc[r] - m.ycc[l], ez = m.zcc[r] - m.zcc[l];
+        double ods = 1.0 / sqrt(sqr3(ex, ey, ez));
+        ex *= ods; ey *= ods; ez *= ods;
+        for (int s = 0; s < 3; ++s) {
+            double fOld = g[s][0] * ex + g[s][1] * ey + g[s][2] * ez;
+            double fnew = (d.q[(1 + s) * T + r] - d.q[(1 + s) * T + l]) * ods;
+            double df = fnew - fOld;
+            g[s][0] += df * ex; g[s][1] += df * ey; g[s][2] += df * ez;
+        }
+        double vis = d.visl[l];
+        double dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
+        double dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
+        double dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
+        double divv2p3 = (2.0 / 3.0) * (dudx + dvdy + dwdz);
+        double txx = vis * (2.0 * dudx - divv2p3), tyy = vis * (2.0 * dvdy - divv2p3), tzz = vis * (2.0 * dwdz - divv2p3);
+        double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
+        double coef = -2.0 * area * k.oreynolds;
+        vf[0] = coef * (fnx * txx + fny * txy + fnz * txz);
+        vf[1] = coef * (fnx * txy + fny * tyy + fnz * tyz);
+        vf[2] = coef * (fnx * txz + fny * tyz + fnz * tzz);
+    }
+}
+
+// Force sums over the walls of boundary faces [f0, f1), in face order (one thread).
 __global__ void k_aero(DevMesh m, DevFields d, Consts k, double* out, int f0, int f1) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int T = m.T;
     double tot[3] = {0, 0, 0}, pres[3] = {0, 0, 0};
     for (int f = f0; f < f1; ++f) {
         if (m.bct[f] != BC_SOLID_SURFACE) continue;
-        int l = m.lc[f], r = m.C + f;
-        double area = m.area[f], fnx = m.xfn[f], fny = m.yfn[f], fnz = m.zfn[f];
-        double fanx = fnx * area, fany = fny * area, fanz = fnz * area;
-        double dx = m.xfc[f] - m.xcc[l], dy = m.yfc[f] - m.ycc[l], dz = m.zfc[f] - m.zcc[l];
-        double gp[3] = {d.grad[(0 * 6 + 4) * T + l], d.grad[(1 * 6 + 4) * T + l], d.grad[(2 * 6 + 4) * T + l]};
-        double wp = d.q[4 * T + l] + gp[0] * dx + gp[1] * dy + gp[2] * dz;
-        if (wp < 0.0) wp = d.q[4 * T + l];
-        double cp = 2.0 * (wp - k.inflow[4]);
-        double pf[3] = {fanx * cp, fany * cp, fanz * cp};
-        double vf[3] = {0, 0, 0};
-        if (k.vismodel > 0) {
-            double g[3][3];
-            for (int s = 0; s < 3; ++s)
-                for (int a = 0; a < 3; ++a) g[s][a] = d.grad[(a * 6 + 1 + s) * T + l];
-            double ex = m.xcc[r] - m.xcc[l], ey = m.ycc[r] - m.ycc[l], ez = m.zcc[r] - m.zcc[l];
-            double ods = 1.0 / sqrt(sqr3(ex, ey, ez));
-            ex *= ods; ey *= ods; ez *= ods;
-            for (int s = 0; s < 3; ++s) {
-                double fOld = g[s][0] * ex + g[s][1] * ey + g[s][2] * ez;
-                double fnew = (d.q[(1 + s) * T + r] - d.q[(1 + s) * T + l]) * ods;
-                double df = fnew - fOld;
-                g[s][0] += df * ex; g[s][1] += df * ey; g[s][2] += df * ez;
-            }
-            double vis = d.visl[l];
-            double dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
-            double dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
-            double dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
-            double divv2p3 = (2.0 / 3.0) * (dudx + dvdy + dwdz);
-            double txx = vis * (2.0 * dudx - divv2p3), tyy = vis * (2.0 * dvdy - divv2p3), tzz = vis * (2.0 * dwdz - divv2p3);
-            double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
-            double coef = -2.0 * area * k.oreynolds;
-            vf[0] = coef * (fnx * txx + fny * txy + fnz * txz);
-            vf[1] = coef * (fnx * txy + fny * tyy + fnz * tyz);
-            vf[2] = coef * (fnx * txz + fny * tyz + fnz * tzz);
-        }
+        double pf[3], vf[3], cp;
+        wall_face_force(m, d, k, f, pf, vf, cp);
         for (int a = 0; a < 3; ++a) { tot[a] += pf[a] + vf[a]; pres[a] += pf[a]; }
     }
     for (int a = 0; a < 3; ++a) { out[a] = tot[a]; out[3 + a] = pres[a]; }
+}
+
+// DUMP_PRESSURE_COEFF rows (idump_pres branch of CalcAeroForce,
+// codes/force/src/AeroForceTask.cpp:426-440; AeroCom::CalcCF,
+// codes/force/src/Force.cpp:160-165): wall k (face order) -> x, y, z of the
+// face centre, -cp, cf at out[5k..5k+4].  One thread per wall face; zone
+// constants and angles by the face's zone (single zone: k, cosa, sina).
+__global__ void k_wall_data(DevMesh m, DevFields d, Consts k0, double cosa0, double sina0, double* out) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= m.nWall) return;
+    int f = m.wallFaces[w];
+    int zone = face_zone(m, f);
+    const Consts k = zoned(k0, m, zone);
+    double cosa = m.nz > 1 ? m.zk[zone].cosa : cosa0, sina = m.nz > 1 ? m.zk[zone].sina : sina0;
+    double pf[3], vf[3], cp;
+    wall_face_force(m, d, k, f, pf, vf, cp);
+    double dir = cosa * vf[0] + sina * vf[1];
+    double sgn = 1.0 * dabs(dir) / (dir + SMALL);
+    double cf = sqrt(sqr3(vf[0], vf[1], vf[2])) / m.area[f] * sgn;
+    double* o = out + 5 * (size_t)w;
+    o[0] = m.xfc[f]; o[1] = m.yfc[f]; o[2] = m.zfc[f]; o[3] = -cp; o[4] = cf;
+}
+
+int wall_data(ofx_ctx* c, double* host_out) {
+    const int n = c->m.nWall;
+    if (n == 0) return 0;
+    double* buf = nullptr;
+    OFX_HIP(hipMalloc((void**)&buf, sizeof(double) * 5 * (size_t)n));
+    double aoa = c->cfg.aoa_degree * PI / 180.0;
+    k_wall_data<<<(n + 255) / 256, 256, 0, c->stream>>>(c->m, c->d, c->k, std::cos(aoa), std::sin(aoa), buf);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, buf, sizeof(double) * 5 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return OFX_EHIP; }
+    return 0;
 }
 
 // Forces of zone z (its boundary faces [f0, f1), its constants and angles).

@@ -406,10 +406,11 @@ __global__ void __launch_bounds__(1024) k_sa_sweep_1wg(DevMesh m, DevFields d) {
 }
 
 template <bool UPPER, int MAXF>
-__global__ void __launch_bounds__(TBS) k_sa_sweep_level(DevMesh m, DevFields d, int b, int e) {
+__global__ void __launch_bounds__(TBS) k_sa_sweep_level(DevMesh m, DevFields d, int b, int e, int base) {
     int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= e) return;
-    sa_lusgs_cell<UPPER, MAXF>(m, d, (UPPER ? m.upOrder : m.loOrder)[idx]);
+    int i = base >= 0 ? base + (idx - b) : (UPPER ? m.upOrder : m.loOrder)[idx];
+    sa_lusgs_cell<UPPER, MAXF>(m, d, i);
 }
 
 template <int MAXF>
@@ -422,11 +423,12 @@ static void launch_sa_lusgs(ofx_ctx* c) {
             if (up) k_sa_sweep_1wg<true, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d);
             else k_sa_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d);
         } else {
+            const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
             for (size_t L = 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
-                if (up) k_sa_sweep_level<true, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
-                else k_sa_sweep_level<false, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e);
+                if (up) k_sa_sweep_level<true, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e, base[L]);
+                else k_sa_sweep_level<false, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e, base[L]);
             }
         }
     }

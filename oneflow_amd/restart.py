@@ -13,7 +13,10 @@ A steady restart takes q = q1 (Restart::ReadUnsteady) and continues from the
 stored step count.  res.dat / turbres.dat rows are written as
 ResidualTask::DumpResidual does (codes/residual/src/ResidualTask.cpp:140-188):
 "iter sub-iter " then every norm left-aligned in 13 columns, scientific with 5
-digits, each followed by a space.
+digits, each followed by a space.  wallaero.dat (DUMP_PRESSURE_COEFF, the
+idump_pres branch of CalcAeroForce, codes/force/src/AeroForceTask.cpp:242-260,
+353-362): a title block, "Zone  i = N ", then per wall face x y z -cp cf
+left-aligned in 20 columns, scientific with 10 digits.
 """
 from __future__ import annotations
 
@@ -72,3 +75,14 @@ def residual_lines(first_step: int, norms, n_var: int) -> list[str]:
 
 def write_residuals(path, first_step: int, norms, n_var: int) -> None:
     Path(path).write_text("\n".join(residual_lines(first_step, norms, n_var)) + "\n")
+
+
+def wallaero_lines(rows) -> list[str]:
+    """wallaero.dat text for wall rows [n][5] (x, y, z, -cp, cf)."""
+    head = ['title="THE FLOW FIELD OF ONEFLOW"', "variables=", '"x"', '"y"', '"z"', '"-cp"', '"cf"',
+            f"Zone  i = {len(rows)} "]
+    return head + ["".join(f"{float(v):.10e}".ljust(20) for v in r) for r in np.asarray(rows)]
+
+
+def write_wallaero(path, rows) -> None:
+    Path(path).write_text("\n".join(wallaero_lines(rows)) + "\n")

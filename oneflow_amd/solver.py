@@ -68,6 +68,8 @@ def load_library():
     L.ofx_zones_norms.argtypes = [P, ctypes.POINTER(OfxNorms)]
     L.ofx_zones_aero.argtypes = [P, dp]
     L.ofx_set_face_tile.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.ofx_wall_count.argtypes = [P, ip]
+    L.ofx_wall_data.argtypes = [P, dp]
     L.ofx_set_kernel_variant.argtypes = [P, ctypes.c_int, ctypes.c_int]
     L.ofx_autotune.argtypes = [P, ctypes.c_int, ip]
     L.ofx_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, ctypes.c_int]
@@ -192,6 +194,15 @@ class Solver:
         return a
 
     KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5}
+
+    def wall_data(self):               # DUMP_PRESSURE_COEFF
+        """[n_wall][5]: x, y, z of each wall face centre, -cp, cf (face order)."""
+        n = np.zeros(1, np.int32)
+        self._check(self._L.ofx_wall_count(self.h, _ip(n)))
+        a = np.zeros((int(n[0]), 5))
+        if n[0]:
+            self._check(self._L.ofx_wall_data(self.h, _dp(a)))
+        return a
 
     def time_kernel(self, which, iters=100):
         """Average ms of one residual part over `iters` back-to-back launches."""

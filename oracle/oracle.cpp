@@ -1741,48 +1741,56 @@ int ofo_iterate(ofo_case* c, int first_iter, int n, ofx_norms* norms_out) {
     return 0;
 }
 
+// Pressure and viscous force of wall face f and its pressure coefficient
+// (CalcAeroForce's face loop, codes/force/src/AeroForceTask.cpp:365-424, with
+// Stress::CalcForce, codes/solver/src/Stress.cpp).
+static void wall_face(const ofo_case* c, int f, double* pf, double* vf, double& cp) {
+    double orey = 1.0 / c->cfg.reynolds;
+    int l = c->lc[f], r = c->rc[f];
+    double area = c->area[f], fnx = c->xfn[f], fny = c->yfn[f], fnz = c->zfn[f];
+    double fanx = fnx * area, fany = fny * area, fanz = fnz * area;
+    double dx = c->xfc[f] - c->xcc[l], dy = c->yfc[f] - c->ycc[l], dz = c->zfc[f] - c->zcc[l];
+    double wp = c->q[4][l] + c->bcdqdx[4][f] * dx + c->bcdqdy[4][f] * dy + c->bcdqdz[4][f] * dz;
+    if (wp < 0.0) wp = c->q[4][l];
+    cp = 2.0 * (wp - c->inflow[4]);
+    pf[0] = fanx * cp; pf[1] = fany * cp; pf[2] = fanz * cp;
+    vf[0] = vf[1] = vf[2] = 0.0;
+    if (c->cfg.vismodel > 0) {
+        double g[3][3];
+        for (int k = 0; k < 3; ++k) { g[k][0] = c->bcdqdx[1 + k][f]; g[k][1] = c->bcdqdy[1 + k][f]; g[k][2] = c->bcdqdz[1 + k][f]; }
+        double ex = c->xcc[r] - c->xcc[l], ey = c->ycc[r] - c->ycc[l], ez = c->zcc[r] - c->zcc[l];
+        double ods = 1.0 / DIST(ex, ey, ez);
+        ex *= ods; ey *= ods; ez *= ods;
+        for (int k = 0; k < 3; ++k) {
+            double fOld = g[k][0] * ex + g[k][1] * ey + g[k][2] * ez;
+            double fnew = (c->q[1 + k][r] - c->q[1 + k][l]) * ods;
+            double df = fnew - fOld;
+            g[k][0] += df * ex; g[k][1] += df * ey; g[k][2] += df * ez;
+        }
+        double vis = c->visl[l];
+        double dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
+        double dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
+        double dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
+        double divv2p3 = two3rd * (dudx + dvdy + dwdz);
+        double txx = vis * (2.0 * dudx - divv2p3), tyy = vis * (2.0 * dvdy - divv2p3), tzz = vis * (2.0 * dwdz - divv2p3);
+        double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
+        double coef = -2.0 * area * orey;
+        vf[0] = coef * (fnx * txx + fny * txy + fnz * txz);
+        vf[1] = coef * (fnx * txy + fny * tyy + fnz * tyz);
+        vf[2] = coef * (fnx * txz + fny * tyz + fnz * tzz);
+    }
+}
+
 // CalcAeroForce + AeroForceInfo::CalcCoef (codes/force/src/AeroForceTask.cpp:186-440,
 // codes/force/src/Force.cpp:148-270, codes/solver/src/Stress.cpp), aref = 1.
 int ofo_aero(ofo_case* c, double* out4) {
     double aoa = c->cfg.aoa_degree * PI / 180.0, aos = c->cfg.sideslip_degree * PI / 180.0;
     double sina = std::sin(aoa), cosa = std::cos(aoa), sinb = std::sin(aos), cosb = std::cos(aos);
-    double orey = 1.0 / c->cfg.reynolds;
     double tot[3] = {0, 0, 0}, pres[3] = {0, 0, 0};
     for (int f = 0; f < c->nBFaces; ++f) {
         if (c->bcType[f] != BC_SOLID_SURFACE) continue;
-        int l = c->lc[f], r = c->rc[f];
-        double area = c->area[f], fnx = c->xfn[f], fny = c->yfn[f], fnz = c->zfn[f];
-        double fanx = fnx * area, fany = fny * area, fanz = fnz * area;
-        double dx = c->xfc[f] - c->xcc[l], dy = c->yfc[f] - c->ycc[l], dz = c->zfc[f] - c->zcc[l];
-        double wp = c->q[4][l] + c->bcdqdx[4][f] * dx + c->bcdqdy[4][f] * dy + c->bcdqdz[4][f] * dz;
-        if (wp < 0.0) wp = c->q[4][l];
-        double cp = 2.0 * (wp - c->inflow[4]);
-        double pf[3] = {fanx * cp, fany * cp, fanz * cp};
-        double vf[3] = {0, 0, 0};
-        if (c->cfg.vismodel > 0) {
-            double g[3][3];
-            for (int k = 0; k < 3; ++k) { g[k][0] = c->bcdqdx[1 + k][f]; g[k][1] = c->bcdqdy[1 + k][f]; g[k][2] = c->bcdqdz[1 + k][f]; }
-            double ex = c->xcc[r] - c->xcc[l], ey = c->ycc[r] - c->ycc[l], ez = c->zcc[r] - c->zcc[l];
-            double ods = 1.0 / DIST(ex, ey, ez);
-            ex *= ods; ey *= ods; ez *= ods;
-            for (int k = 0; k < 3; ++k) {
-                double fOld = g[k][0] * ex + g[k][1] * ey + g[k][2] * ez;
-                double fnew = (c->q[1 + k][r] - c->q[1 + k][l]) * ods;
-                double df = fnew - fOld;
-                g[k][0] += df * ex; g[k][1] += df * ey; g[k][2] += df * ez;
-            }
-            double vis = c->visl[l];
-            double dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
-            double dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
-            double dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
-            double divv2p3 = two3rd * (dudx + dvdy + dwdz);
-            double txx = vis * (2.0 * dudx - divv2p3), tyy = vis * (2.0 * dvdy - divv2p3), tzz = vis * (2.0 * dwdz - divv2p3);
-            double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
-            double coef = -2.0 * area * orey;
-            vf[0] = coef * (fnx * txx + fny * txy + fnz * txz);
-            vf[1] = coef * (fnx * txy + fny * tyy + fnz * tyz);
-            vf[2] = coef * (fnx * txz + fny * tyz + fnz * tzz);
-        }
+        double pf[3], vf[3], cp;
+        wall_face(c, f, pf, vf, cp);
         for (int k = 0; k < 3; ++k) { tot[k] += pf[k] + vf[k]; pres[k] += pf[k]; }
     }
     double cl = -sina * tot[0] + cosa * tot[1];
@@ -1790,6 +1798,32 @@ int ofo_aero(ofo_case* c, double* out4) {
     double cdp = cosa * cosb * pres[0] + sina * cosb * pres[1] + sinb * pres[2];
     out4[0] = cl; out4[1] = cd; out4[2] = cdp; out4[3] = cd - cdp;
     return 0;
+}
+
+// DUMP_PRESSURE_COEFF rows (the idump_pres branch of CalcAeroForce,
+// codes/force/src/AeroForceTask.cpp:426-440, AeroCom::CalcCF,
+// codes/force/src/Force.cpp:160-165): per wall face in face order
+// x, y, z of the face centre, -cp, cf -> out[5 * k .. 5 * k + 4].
+int ofo_wall_data(ofo_case* c, double* out) {
+    double aoa = c->cfg.aoa_degree * PI / 180.0;
+    double sina = std::sin(aoa), cosa = std::cos(aoa);
+    int k = 0;
+    for (int f = 0; f < c->nBFaces; ++f) {
+        if (c->bcType[f] != BC_SOLID_SURFACE) continue;
+        double pf[3], vf[3], cp;
+        wall_face(c, f, pf, vf, cp);
+        double dir = cosa * vf[0] + sina * vf[1];
+        double cf = DIST(vf[0], vf[1], vf[2]) / c->area[f] * SIGN(1.0, dir);
+        double* o = out + 5 * (size_t)k++;
+        o[0] = c->xfc[f]; o[1] = c->yfc[f]; o[2] = c->zfc[f]; o[3] = -cp; o[4] = cf;
+    }
+    return 0;
+}
+
+int ofo_wall_count(ofo_case* c) {
+    int n = 0;
+    for (int f = 0; f < c->nBFaces; ++f) n += c->bcType[f] == BC_SOLID_SURFACE;
+    return n;
 }
 
 int ofo_face_flux(ofo_case* c, double* inv, double* vis) {

@@ -45,6 +45,10 @@ int  ofo_update_sa(ofo_case* c);
 int  ofo_norms_get(ofo_case* c, ofx_norms* out);
 int  ofo_iterate(ofo_case* c, int first_iter, int n, ofx_norms* norms_out);
 int  ofo_aero(ofo_case* c, double* out4);
+/* Wall faces (face order): count, and x, y, z of the face centre, -cp, cf
+ * per face -> out[5 * k .. 5 * k + 4]. */
+int  ofo_wall_count(ofo_case* c);
+int  ofo_wall_data(ofo_case* c, double* out);
 /* Face flux arrays of the last ofo_residual_ns: inv[5][F], vis[5][F]. */
 int  ofo_face_flux(ofo_case* c, double* inv, double* vis);
 

@@ -58,6 +58,8 @@ def lib():
         L.ofo_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
         L.ofo_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
         L.ofo_aero.argtypes = [P, dp]
+        L.ofo_wall_count.argtypes = [P]
+        L.ofo_wall_data.argtypes = [P, dp]
         L.ofo_face_flux.argtypes = [P, dp, dp]
         _lib = L
     return _lib
@@ -160,6 +162,13 @@ class OracleCase:
     def aero(self):
         a = np.zeros(4)
         self._check(lib().ofo_aero(self.h, _dp(a)))
+        return a
+
+    def wall_data(self):
+        """[n_wall][5]: x, y, z of each wall face centre, -cp, cf (face order)."""
+        n = lib().ofo_wall_count(self.h)
+        a = np.zeros((n, 5))
+        self._check(lib().ofo_wall_data(self.h, _dp(a)))
         return a
 
     def face_flux(self):

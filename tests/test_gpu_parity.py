@@ -12,6 +12,7 @@ import pytest
 from oneflow_amd.config import REFERENCE_CASES
 from oneflow_amd.mesh import read_ofl
 from oracle.pyoracle import OracleCase
+from oneflow_amd.restart import wallaero_lines
 from tests.golden_util import GOLDEN, harness_equal, load_res
 from tests.parity_util import report
 
@@ -150,6 +151,16 @@ def test_fifty_steps_against_goldens(case):
     for it, row in enumerate(rows):
         for mine, ref in zip(aero[it][:2], row[3:5]):
             assert harness_equal("%.4e" % mine, "%.4e" % float(ref)), (case, it + 1, aero[it], row[3:5])
+    # wallaero.dat after step 50: per wall face -cp and cf at 11 digits
+    wall = gpu.wall_data()
+    ew = report("wall", wall, ora.wall_data())
+    assert ew <= 1.0e-10, ew
+    mine = wallaero_lines(wall)
+    gold_w = (GOLDEN / case / "wallaero.dat").read_text().splitlines()
+    assert len(mine) == len(gold_w)
+    for a, b in zip(mine[8:], gold_w[8:]):
+        for x, y in zip(a.split(), b.split()):
+            assert harness_equal(x, y), (case, a, b)
 
 
 @pytest.mark.parametrize("case", ["syn_natural", "syn_mc"])

@@ -76,3 +76,6 @@ def test_zones_equal_single_cases(which, variant):
                 assert np.array_equal(a, b), (which, chunk, z, f, float(np.max(np.abs(a - b))))
             assert np.array_equal(zn[z], s.norms()), (which, z, zn[z], s.norms())
             assert np.array_equal(za[z], s.aero()), (which, z, za[z], s.aero())
+        # wall rows: the zones' walls in face order, zone after zone
+        walls = np.concatenate([s.wall_data() for s in single])
+        assert np.array_equal(multi.wall_data(), walls), which

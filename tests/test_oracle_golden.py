@@ -11,10 +11,12 @@ import pytest
 
 from oneflow_amd.config import REFERENCE_CASES
 from oneflow_amd.mesh import read_ofl
+from oneflow_amd.restart import wallaero_lines
 from oracle.pyoracle import OracleCase
 from tests.golden_util import GOLDEN, load_res, harness_equal
 
 CASES = ["plateuns2dslau2", "turbplateuns2droe_sa", "rae2822_roe_sa"]
+WALL = {}  # wall rows of each case after step 50
 
 
 @pytest.fixture(scope="module", params=CASES)
@@ -27,6 +29,7 @@ def run(request):
     for it in range(1, 51):
         norms.append(c.iterate(it, 1)[0])
         aero.append(c.aero())
+    WALL[case] = c.wall_data()
     return case, np.array(norms), np.array(aero)
 
 
@@ -57,3 +60,22 @@ def test_aero_dat(run):
         cl, cd, cdp, cdf = (float(v) for v in row[3:7])
         for mine, ref in zip(aero[it], (cl, cd, cdp, cdf)):
             assert harness_equal("%.4e" % mine, "%.4e" % ref), (case, it + 1, aero[it], row[3:7])
+
+
+def test_wallaero_dat(run):
+    """Per wall face x, y, z, -cp and cf after step 50 at 11 significant
+    digits: a field-level pin of the wall state, its gradients and the
+    viscous stress."""
+    case, _, _ = run
+    gold = (GOLDEN / case / "wallaero.dat").read_text().splitlines()
+    mine = wallaero_lines(WALL[case])
+    assert len(mine) == len(gold), (case, len(mine), len(gold))
+    assert mine[:8] == [g.rstrip("\r") for g in gold[:8]]
+    worst = 0.0
+    for k, (a, b) in enumerate(zip(mine[8:], gold[8:])):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb) == 5, (case, k)
+        for x, y in zip(ta, tb):
+            worst = max(worst, abs(float(x) - float(y)))
+            assert harness_equal(x, y), (case, k, a, b)
+    print(f"[wallaero] {case}: {len(mine) - 8} wall faces, worst |diff| {worst:.2e}")
