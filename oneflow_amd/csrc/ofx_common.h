@@ -36,6 +36,11 @@ void set_error(const std::string& msg);
 // arrays of length T = C + B (ghost of boundary face f is cell C + f).
 constexpr int MAXSEG = 9;  // cell segments <= 8 (colours); faces: boundary + 8
 
+// Four doubles as a native vector: loads and stores of one are single
+// 128-bit memory operations pairs kept in registers (HIP's double4 struct is
+// copied through private memory when the compiler cannot split it).
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
 struct DevMesh {
     int dim = 2, nNodes = 0, C = 0, F = 0, B = 0, T = 0;
     int maxFacesPerCell = 0;
@@ -53,13 +58,13 @@ struct DevMesh {
     //   frec = {xf - xl, yf - yl, xf - xr, yf - yr}: face centre relative to
     //          the left and right cell centres (reconstruction, limiter);
     //   fz   = {nz A, zf - zl, zf - zr, 0} (3-D only).
-    double4 *fgg = nullptr, *frec = nullptr, *fz = nullptr;
+    dvec4 *fgg = nullptr, *frec = nullptr, *fz = nullptr;
     //   fv0  = {dx, dy, dz, 1/|d|}: unit centre-to-centre vector of the STD
     //          viscous face gradient (VisGeom, method 0);
     //   fv2  = 5 per face: {d1, d2, angle1, angle2}, {dxnl, dynl, dznl, dxnr},
     //          {dynr, dznr, t1x, t1y}, {t1z, t2x, t2y, t2z}, {fw1, fw2, 0, 0}
     //          (method 2, TEST).
-    double4 *fv0 = nullptr, *fv2 = nullptr;
+    dvec4 *fv0 = nullptr, *fv2 = nullptr;
     int *lc = nullptr, *rc = nullptr, *bct = nullptr;
     // The c2f / c2n lists again, cell-major with a fixed width ellw =
     // maxFacesPerCell: entry j of cell i at j * C + i (0 / i past the cell's

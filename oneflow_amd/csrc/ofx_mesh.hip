@@ -157,9 +157,9 @@ __global__ void k_face_static(DevMesh m) {
     double delta = 1.0 / (delt1 + delt2 + SMALL);
     double cl = delt2 * delta, cr = delt1 * delta;
     double ar = m.area[f];
-    m.fgg[f] = make_double4(cl, cr, m.xfn[f] * ar, m.yfn[f] * ar);
-    m.frec[f] = make_double4(fx - xl, fy - yl, fx - xr, fy - yr);
-    if (DIM == 3) m.fz[f] = make_double4(m.zfn[f] * ar, fz - zl, fz - zr, 0.0);
+    m.fgg[f] = dvec4{cl, cr, m.xfn[f] * ar, m.yfn[f] * ar};
+    m.frec[f] = dvec4{fx - xl, fy - yl, fx - xr, fy - yr};
+    if (DIM == 3) m.fz[f] = dvec4{m.zfn[f] * ar, fz - zl, fz - zr, 0.0};
 }
 
 // Viscous face-gradient geometry records (DevMesh::fv0 / fv2) from vis_geom.
@@ -169,14 +169,14 @@ __global__ void k_face_vis_static(DevMesh m) {
     int l = m.lc[f], r = (f < m.B) ? m.C + f : m.rc[f];
     VisGeom G;
     vis_geom(m, f, l, r, 0, G);
-    m.fv0[f] = make_double4(G.dx, G.dy, G.dz, G.ods);
+    m.fv0[f] = dvec4{G.dx, G.dy, G.dz, G.ods};
     vis_geom(m, f, l, r, 2, G);
-    double4* o = m.fv2 + 5 * (size_t)f;
-    o[0] = make_double4(G.d1, G.d2, G.angle1, G.angle2);
-    o[1] = make_double4(G.dxnl, G.dynl, G.dznl, G.dxnr);
-    o[2] = make_double4(G.dynr, G.dznr, G.t1[0], G.t1[1]);
-    o[3] = make_double4(G.t1[2], G.t2[0], G.t2[1], G.t2[2]);
-    o[4] = make_double4(G.fw1, G.fw2, 0.0, 0.0);
+    dvec4* o = m.fv2 + 5 * (size_t)f;
+    o[0] = dvec4{G.d1, G.d2, G.angle1, G.angle2};
+    o[1] = dvec4{G.dxnl, G.dynl, G.dznl, G.dxnr};
+    o[2] = dvec4{G.dynr, G.dznr, G.t1[0], G.t1[1]};
+    o[3] = dvec4{G.t1[2], G.t2[0], G.t2[1], G.t2[2]};
+    o[4] = dvec4{G.fw1, G.fw2, 0.0, 0.0};
 }
 
 __global__ void k_ghost_centres(DevMesh m) {
@@ -491,11 +491,11 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     double** ca[] = {&m.xcc, &m.ycc, &m.zcc, &m.vol};
     for (double** p : ca) OFX_TRY(A((void**)p, dT));
     OFX_TRY(A((void**)&m.dist, sizeof(double) * C));
-    OFX_TRY(A((void**)&m.fgg, sizeof(double4) * F));
-    OFX_TRY(A((void**)&m.frec, sizeof(double4) * F));
-    OFX_TRY(A((void**)&m.fz, sizeof(double4) * (m.dim == 3 ? F : 1)));
-    OFX_TRY(A((void**)&m.fv0, sizeof(double4) * F));
-    OFX_TRY(A((void**)&m.fv2, sizeof(double4) * 5 * (size_t)F));
+    OFX_TRY(A((void**)&m.fgg, sizeof(dvec4) * F));
+    OFX_TRY(A((void**)&m.frec, sizeof(dvec4) * F));
+    OFX_TRY(A((void**)&m.fz, sizeof(dvec4) * (m.dim == 3 ? F : 1)));
+    OFX_TRY(A((void**)&m.fv0, sizeof(dvec4) * F));
+    OFX_TRY(A((void**)&m.fv2, sizeof(dvec4) * 5 * (size_t)F));
     OFX_TRY(A((void**)&m.lc, sizeof(int) * F));
     OFX_TRY(A((void**)&m.rc, sizeof(int) * F));
     OFX_TRY(A((void**)&m.bct, sizeof(int) * std::max(1, B)));

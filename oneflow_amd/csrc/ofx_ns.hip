@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
         cell_faces<MAXF>(m, i, ent, nb);
         double qi[5];
         for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
-        double4 gw[MAXF];
+        dvec4 gw[MAXF];
         double gz[MAXF], qn[MAXF][5];
 #pragma unroll
         for (int j = 0; j < MAXF; ++j) {
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(384) k_ns_grad_q(DevMesh m, DevFields d, Const
         int ent[MAXF], nb[MAXF];
         cell_faces<MAXF>(m, i, ent, nb);
         double vi = (s < 5) ? d.q[s * T + i] : temperature(d.q[i], d.q[4 * T + i], k.statecoef);
-        double4 gw[MAXF];
+        dvec4 gw[MAXF];
         double gz[MAXF], vn[MAXF];
 #pragma unroll
         for (int j = 0; j < MAXF; ++j) {
@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
     if (has) {
         int e = ej;
         int f = e >= 0 ? e : ~e;
-        double4 rc4 = m.frec[f];  // face centre relative to the left / right cell centre
+        dvec4 rc4 = m.frec[f];  // face centre relative to the left / right cell centre
         dx = e >= 0 ? rc4.x : rc4.z;
         dy = e >= 0 ? rc4.y : rc4.w;
         dz = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
@@ -494,6 +494,69 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
         }
         lim = group_min<LPC>(lim);
         if (live && j == 0) d.limiter[s * T + i] = dmin(1.0, lim);
+    }
+}
+
+// Viscous face flux (UNsVisFlux::CalcFlux, codes/uns/src/UNsVisFlux.cpp:127-440)
+// for face-gradient method VM: face gradients of u, v, (w,) T, Newtonian
+// stress and heat flux, stored as visflux[5][F].
+template <int DIM, bool F32, int VM>
+__device__ __forceinline__ void ns_face_visc(const DevMesh& m, const DevFields& d, const Consts& k, int f, int l,
+                                             int r, int rg, const double* ql, const double* qr, double xfn,
+                                             double yfn, double zfn, double area) {
+    const int T = m.T;
+    VisGeom G;
+    vis_geom_pre<VM>(m, f, G);
+    double tl = temperature(ql[0], ql[4], k.statecoef);
+    double tr = temperature(qr[0], qr[4], k.statecoef);
+    double gq[6][3], qfv[6], dqdn[6];
+#pragma unroll
+    for (int s = 1; s < 6; ++s) {
+        if (s == 4) continue;  // pressure does not enter the viscous flux
+        if (DIM == 2 && s == 3) {
+            gq[3][0] = gq[3][1] = gq[3][2] = 0.0; qfv[3] = 0.0; dqdn[3] = 0.0;
+            continue;
+        }
+        double g1[3] = {d.grad[(0 * 6 + s) * T + l], d.grad[(1 * 6 + s) * T + l], (DIM == 3) ? d.grad[(2 * 6 + s) * T + l] : 0.0};
+        double g2[3] = {d.grad[(0 * 6 + s) * T + rg], d.grad[(1 * 6 + s) * T + rg], (DIM == 3) ? d.grad[(2 * 6 + s) * T + rg] : 0.0};
+        double v1 = s < 5 ? ql[s] : tl, v2 = s < 5 ? qr[s] : tr;
+        vis_grad<VM>(G, v1, v2, g1, g2, qfv[s], gq[s], dqdn[s]);
+    }
+    double visl = 0.5 * (sutherland(tl, k.ts_suth) + sutherland(tr, k.ts_suth));
+    double vist = 0.5 * (d.vist[l] + d.vist[r]);
+    double vis = visl + vist;
+    double dudx = gq[1][0], dudy = gq[1][1], dudz = gq[1][2];
+    double dvdx = gq[2][0], dvdy = gq[2][1], dvdz = gq[2][2];
+    double dwdx = gq[3][0], dwdy = gq[3][1], dwdz = gq[3][2];
+    double qNormal = 0.0;
+    double kcp = (visl * k.oprl + vist * k.oprt) * k.const_cp;
+    qNormal += xfn * 0.0 + yfn * 0.0 + zfn * 0.0;
+    qNormal += kcp * dqdn[5];
+    double divv2p3 = (2.0 / 3.0) * (dudx + dvdy + dwdz);
+    double txx = vis * (2.0 * dudx - divv2p3);
+    double tyy = vis * (2.0 * dvdy - divv2p3);
+    double tzz = vis * (2.0 * dwdz - divv2p3);
+    double txy = vis * (dudy + dvdx);
+    double txz = vis * (dudz + dwdx);
+    double tyz = vis * (dvdz + dwdy);
+    double fv1 = xfn * txx + yfn * txy + zfn * txz;
+    double fv2 = xfn * txy + yfn * tyy + zfn * tyz;
+    double fv3 = xfn * txz + yfn * tyz + zfn * tzz;
+    double fv4 = qfv[1] * fv1 + qfv[2] * fv2 + qfv[3] * fv3 + qNormal;
+    double coeff = -k.oreynolds * area;
+    if (F32) {
+        float* out = reinterpret_cast<float*>(d.visflux);
+        out[0 * m.F + f] = (float)(coeff * 0.0);
+        out[1 * m.F + f] = (float)(coeff * fv1);
+        out[2 * m.F + f] = (float)(coeff * fv2);
+        out[3 * m.F + f] = (float)(coeff * fv3);
+        out[4 * m.F + f] = (float)(coeff * fv4);
+    } else {
+        d.visflux[0 * m.F + f] = coeff * 0.0;
+        d.visflux[1 * m.F + f] = coeff * fv1;
+        d.visflux[2 * m.F + f] = coeff * fv2;
+        d.visflux[3 * m.F + f] = coeff * fv3;
+        d.visflux[4 * m.F + f] = coeff * fv4;
     }
 }
 
@@ -534,9 +597,9 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
     } else if (LIM == -1) {
         for (int e = 0; e < 5; ++e) { p1[e] = ql[e]; p2[e] = qr[e]; }
     } else {
-        double4 rc4 = m.frec[f];
+        dvec4 rc4 = m.frec[f];
         double dxl = rc4.x, dyl = rc4.y, dxr = rc4.z, dyr = rc4.w, dzl = 0.0, dzr = 0.0;
-        if (DIM == 3) { double4 z4 = m.fz[f]; dzl = z4.y; dzr = z4.z; }
+        if (DIM == 3) { dvec4 z4 = m.fz[f]; dzl = z4.y; dzr = z4.z; }
         for (int e = 0; e < 5; ++e) {
             if (DIM == 2 && e == 3) {
                 // w and its gradient are zero in 2-D: the reconstruction leaves w = 0
@@ -576,58 +639,11 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
     }
 
     if (!VISC) return;
-    VisGeom G;
-    vis_geom_pre(m, f, k.ivischeme, G);
-    double tl = temperature(ql[0], ql[4], k.statecoef);
-    double tr = temperature(qr[0], qr[4], k.statecoef);
-    double gq[6][3], qfv[6], dqdn[6];
-    for (int s = 1; s < 6; ++s) {
-        if (s == 4) continue;  // pressure does not enter the viscous flux
-        if (DIM == 2 && s == 3) {
-            gq[3][0] = gq[3][1] = gq[3][2] = 0.0; qfv[3] = 0.0; dqdn[3] = 0.0;
-            continue;
-        }
-        double g1[3] = {d.grad[(0 * 6 + s) * T + l], d.grad[(1 * 6 + s) * T + l], (DIM == 3) ? d.grad[(2 * 6 + s) * T + l] : 0.0};
-        double g2[3] = {d.grad[(0 * 6 + s) * T + rg], d.grad[(1 * 6 + s) * T + rg], (DIM == 3) ? d.grad[(2 * 6 + s) * T + rg] : 0.0};
-        double v1 = s < 5 ? ql[s] : tl, v2 = s < 5 ? qr[s] : tr;
-        vis_grad(k.ivischeme, G, v1, v2, g1, g2, qfv[s], gq[s], dqdn[s]);
-    }
-    double visl = 0.5 * (sutherland(tl, k.ts_suth) + sutherland(tr, k.ts_suth));
-    double vist = 0.5 * (d.vist[l] + d.vist[r]);
-    double vis = visl + vist;
-    double dudx = gq[1][0], dudy = gq[1][1], dudz = gq[1][2];
-    double dvdx = gq[2][0], dvdy = gq[2][1], dvdz = gq[2][2];
-    double dwdx = gq[3][0], dwdy = gq[3][1], dwdz = gq[3][2];
-    double qNormal = 0.0;
-    double kcp = (visl * k.oprl + vist * k.oprt) * k.const_cp;
-    qNormal += xfn * 0.0 + yfn * 0.0 + zfn * 0.0;
-    qNormal += kcp * dqdn[5];
-    double divv2p3 = (2.0 / 3.0) * (dudx + dvdy + dwdz);
-    double txx = vis * (2.0 * dudx - divv2p3);
-    double tyy = vis * (2.0 * dvdy - divv2p3);
-    double tzz = vis * (2.0 * dwdz - divv2p3);
-    double txy = vis * (dudy + dvdx);
-    double txz = vis * (dudz + dwdx);
-    double tyz = vis * (dvdz + dwdy);
-    double fv1 = xfn * txx + yfn * txy + zfn * txz;
-    double fv2 = xfn * txy + yfn * tyy + zfn * tyz;
-    double fv3 = xfn * txz + yfn * tyz + zfn * tzz;
-    double fv4 = qfv[1] * fv1 + qfv[2] * fv2 + qfv[3] * fv3 + qNormal;
-    double coeff = -k.oreynolds * area;
-    if (F32) {
-        float* out = reinterpret_cast<float*>(d.visflux);
-        out[0 * m.F + f] = (float)(coeff * 0.0);
-        out[1 * m.F + f] = (float)(coeff * fv1);
-        out[2 * m.F + f] = (float)(coeff * fv2);
-        out[3 * m.F + f] = (float)(coeff * fv3);
-        out[4 * m.F + f] = (float)(coeff * fv4);
-    } else {
-        d.visflux[0 * m.F + f] = coeff * 0.0;
-        d.visflux[1 * m.F + f] = coeff * fv1;
-        d.visflux[2 * m.F + f] = coeff * fv2;
-        d.visflux[3 * m.F + f] = coeff * fv3;
-        d.visflux[4 * m.F + f] = coeff * fv4;
-    }
+    // the face-gradient method is uniform: one dispatch, each branch a
+    // separate instantiation of the viscous flux
+    if (k.ivischeme == 0) ns_face_visc<DIM, F32, 0>(m, d, k, f, l, r, rg, ql, qr, xfn, yfn, zfn, area);
+    else if (k.ivischeme == 1) ns_face_visc<DIM, F32, 1>(m, d, k, f, l, r, rg, ql, qr, xfn, yfn, zfn, area);
+    else ns_face_visc<DIM, F32, 2>(m, d, k, f, l, r, rg, ql, qr, xfn, yfn, zfn, area);
 }
 
 // Residual gather: res = -sum(inviscid) then -sum(viscous), each over the
@@ -698,11 +714,14 @@ static void launch_face_dim(ofx_ctx* c, bool visc) {
 
 // NS residual kernels; `parts` selects them (bit 0 gradient + min/max, bit 1
 // limiter, bit 2 face fluxes, bit 3 gather) so each can be timed alone.
+// MAXF bounds the faces per cell (unrolled loops); the lanes-per-cell kernels
+// use LPC, the next power of two (their group reductions are butterflies).
 template <int DIM, int MAXF>
 static void launch_cells(ofx_ctx* c, unsigned parts) {
+    constexpr int LPC = MAXF <= 4 ? 4 : (MAXF <= 8 ? 8 : 16);
     const Consts& k = c->k;
     const int C1 = c->m.C;         // threads: one per cell
-    const int C = c->m.C * MAXF;   // threads: MAXF lanes per cell
+    const int C = c->m.C * LPC;    // threads: LPC lanes per cell
     bool visc = k.vismodel != 0;
     bool lim = k.ilim == 1 || k.ilim == 2;
     if (parts & 1u) {
@@ -722,17 +741,17 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
         }
     }
     if ((parts & 2u) && lim) {
-        if (k.ilim == 2) k_ns_limiter<DIM, MAXF, 2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-        else k_ns_limiter<DIM, MAXF, 1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        if (k.ilim == 2) k_ns_limiter<DIM, LPC, 2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        else k_ns_limiter<DIM, LPC, 1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
     }
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
     if (parts & 8u) {
         if (k.flux_fp32) {
-            if (visc) k_ns_gather<MAXF, true, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
-            else k_ns_gather<MAXF, false, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+            if (visc) k_ns_gather<LPC, true, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather<LPC, false, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
         } else {
-            if (visc) k_ns_gather<MAXF, true, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
-            else k_ns_gather<MAXF, false, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+            if (visc) k_ns_gather<LPC, true, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather<LPC, false, false><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
         }
     }
 }
@@ -744,7 +763,8 @@ int ns_residual_parts(ofx_ctx* c, unsigned parts) {
         else if (mf <= 8) launch_cells<2, 8>(c, parts);
         else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
     } else {
-        if (mf <= 8) launch_cells<3, 8>(c, parts);
+        if (mf <= 6) launch_cells<3, 6>(c, parts);
+        else if (mf <= 8) launch_cells<3, 8>(c, parts);
         else if (mf <= 16) launch_cells<3, 16>(c, parts);
         else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
     }

@@ -208,16 +208,18 @@ __device__ __forceinline__ void vis_geom(const DevMesh& m, int f, int l, int r, 
 }
 
 // The same geometry read from the per-face records the mesh upload stores
-// (DevMesh::fv0, fv2), which vis_geom itself computed.
-__device__ __forceinline__ void vis_geom_pre(const DevMesh& m, int f, int method, VisGeom& G) {
+// (DevMesh::fv0, fv2), which vis_geom itself computed.  METHOD is the
+// caller's compile-time choice, so no member is merged across branches.
+template <int METHOD>
+__device__ __forceinline__ void vis_geom_pre(const DevMesh& m, int f, VisGeom& G) {
     G.xfn = m.xfn[f]; G.yfn = m.yfn[f]; G.zfn = m.zfn[f];
     G.bc_nonint = f < m.B;
-    if (method == 0) {
-        double4 a = m.fv0[f];
+    if (METHOD == 0) {
+        dvec4 a = m.fv0[f];
         G.dx = a.x; G.dy = a.y; G.dz = a.z; G.ods = a.w;
-    } else if (method == 2) {
-        const double4* r = m.fv2 + 5 * (size_t)f;
-        double4 a = r[0], b = r[1], c = r[2], e = r[3], w = r[4];
+    } else if (METHOD == 2) {
+        const dvec4* r = m.fv2 + 5 * (size_t)f;
+        dvec4 a = r[0], b = r[1], c = r[2], e = r[3], w = r[4];
         G.fw1 = w.x; G.fw2 = w.y;
         G.d1 = a.x; G.d2 = a.y; G.angle1 = a.z; G.angle2 = a.w;
         G.dxnl = b.x; G.dynl = b.y; G.dznl = b.z; G.dxnr = b.w;
@@ -228,18 +230,19 @@ __device__ __forceinline__ void vis_geom_pre(const DevMesh& m, int f, int method
 
 // One scalar through VisGrad (AverGrad, then STD / AVER / TEST method;
 // codes/usolver/src/VisGrad.cpp:130-365).  gf = face gradient, dqdn = normal derivative.
-__device__ __forceinline__ void vis_grad(int method, const VisGeom& G, double q1, double q2, const double* g1,
+template <int METHOD>
+__device__ __forceinline__ void vis_grad(const VisGeom& G, double q1, double q2, const double* g1,
                                          const double* g2, double& qf, double* gf, double& dqdn) {
     gf[0] = 0.5 * (g1[0] + g2[0]); gf[1] = 0.5 * (g1[1] + g2[1]); gf[2] = 0.5 * (g1[2] + g2[2]);
     dqdn = 0.0;
     qf = 0.5 * (q1 + q2);
-    if (method == 0) {
+    if (METHOD == 0) {
         double fOld = gf[0] * G.dx + gf[1] * G.dy + gf[2] * G.dz;
         double fnew = (q2 - q1) * G.ods;
         double df = fnew - fOld;
         gf[0] += df * G.dx; gf[1] += df * G.dy; gf[2] += df * G.dz;
         dqdn = G.xfn * gf[0] + G.yfn * gf[1] + G.zfn * gf[2];
-    } else if (method == 1) {
+    } else if (METHOD == 1) {
         dqdn = G.xfn * gf[0] + G.yfn * gf[1] + G.zfn * gf[2];
     } else {
         double q11 = q1, q22 = q2;

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
     double nui = d.turbq[i];
-    double4 gw[MAXF];
+    dvec4 gw[MAXF];
     double gz[MAXF], vn[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
@@ -120,6 +120,18 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
     d.tgrad[i] = ax * ovol;
     d.tgrad[T + i] = ay * ovol;
     d.tgrad[2 * T + i] = az * ovol;
+}
+
+// Normal derivative of nu_tilde at face f by face-gradient method VM
+// (UTurbVisFlux::CalcFaceVisFlux1Equ via VisGrad, codes/uturb/src/UTurbVisFlux.cpp:296-369).
+template <int VM>
+__device__ __forceinline__ double sa_face_dqdn(const DevMesh& m, int f, double nl, double nr, const double* g1,
+                                               const double* g2) {
+    VisGeom G;
+    vis_geom_pre<VM>(m, f, G);
+    double qf, gf[3], dqdn;
+    vis_grad<VM>(G, nl, nr, g1, g2, qf, gf, dqdn);
+    return dqdn;
 }
 
 // Per face: SA spectral radius and off-diagonal face matrices
@@ -182,13 +194,13 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     double vnl_p = 0.5 * (cvnl + dabs(cvnl));
     double vnr_n = 0.5 * (cvnr - dabs(cvnr));
     double conv = farea * (vnl_p * 1.0 * cnl + vnr_n * 1.0 * cnr);
-    // diffusion
-    VisGeom G;
-    vis_geom_pre(m, f, k.iturb_visflux, G);
+    // diffusion (face-gradient method uniform: one dispatch, separate instantiations)
     double g1[3] = {d.tgrad[l], d.tgrad[T + l], (DIM == 3) ? d.tgrad[2 * T + l] : 0.0};
     double g2[3] = {d.tgrad[rg], d.tgrad[T + rg], (DIM == 3) ? d.tgrad[2 * T + rg] : 0.0};
-    double qf, gf[3], dqdn;
-    vis_grad(k.iturb_visflux, G, nl, nr, g1, g2, qf, gf, dqdn);
+    double dqdn;
+    if (k.iturb_visflux == 0) dqdn = sa_face_dqdn<0>(m, f, nl, nr, g1, g2);
+    else if (k.iturb_visflux == 1) dqdn = sa_face_dqdn<1>(m, f, nl, nr, g1, g2);
+    else dqdn = sa_face_dqdn<2>(m, f, nl, nr, g1, g2);
     double visl_nv = 0.5 * (visll * orl + vislr * orr);
     double vist_nv = 0.5 * (nl + nr);
     double comVis = (visl_nv + vist_nv) * k.osigma;
