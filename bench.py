@@ -448,10 +448,13 @@ def residual_bench(s, grid, args, flux_bytes=8):
     s.timestep(args.pre_iters + 1)
     # launch configuration: the gradient variants timed here for the record,
     # then the autotuner picks variant and face tiles on this state
-    grad_us = {}
-    for v in (0, 1):
-        s.set_grad_variant(v)
-        grad_us[v] = 1e3 * s.time_kernel("grad", iters=args.kernel_iters)
+    variant_us = {}
+    for kernel, key in ((0, "grad"), (1, "limiter"), (2, "gather")):
+        t = {}
+        for v in (0, 1):
+            s.set_kernel_variant(kernel, v)
+            t[v] = 1e3 * s.time_kernel(key, iters=args.kernel_iters)
+        variant_us[key] = t
     tiles = s.autotune(iters=20)
     T, F = s.n_tcell, s.n_faces
     for _ in range(args.warmup):
@@ -490,7 +493,7 @@ def residual_bench(s, grid, args, flux_bytes=8):
     return {"value": T * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline, "kernels": kernels,
             "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
             "copy_kernel_gbs": copy_gbs,
-            "launch": {"grad_variant_us": grad_us, "face_tiles": list(tiles)}}
+            "launch": {"variant_us": variant_us, "face_tiles": list(tiles)}}
 
 
 def c2_config():

@@ -242,13 +242,16 @@ int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
 
 /* Kernel variant (no counterpart in the reference; results do not depend on
  * it).  kernel 0, the NS gradient: variant 0 (default) one thread per cell,
- * variant 1 one wave per quantity (more waves in flight on small grids). */
+ * variant 1 one wave per quantity (more waves in flight on small grids).
+ * kernel 1, the limiter, and kernel 2, the residual gather: variant 0 one
+ * lane per face of a cell, variant 1 (default) one thread per cell. */
 int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
 
 /* Autotuner (no counterpart in the reference): times the NS face kernel for
- * each face tile, the NS gradient for each variant and the SA residual for
- * each SA face tile on the current state over `iters` launches, keeps the
- * fastest, and reports the face tiles in chosen2[0..1] (may be NULL).
+ * each face tile, the NS gradient, limiter and gather for each variant and
+ * the SA residual for each SA face tile on the current state over `iters`
+ * launches, keeps the fastest, and reports the face tiles in chosen2[0..1]
+ * (may be NULL).
  * Leaves res / turbres as one evaluation of the current state. */
 int ofx_autotune(ofx_ctx* ctx, int iters, int* chosen2);
 

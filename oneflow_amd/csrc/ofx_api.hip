@@ -469,11 +469,11 @@ int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
 
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel != 0 || variant < 0 || variant > 1) {
-        set_error("kernel variant: kernel 0 (NS gradient) takes variant 0 or 1");
+    if (kernel < 0 || kernel > 2 || variant < 0 || variant > 1) {
+        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter) or 2 (gather), variant 0 or 1");
         return OFX_EINVAL;
     }
-    c->grad_variant = variant;
+    (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant) = variant;
     ++c->gen;
     return 0;
 }
@@ -490,14 +490,19 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
         if (ms < best) { best = ms; pick = t; }
     }
     c->face_tb = pick;
-    best = 1e300;
-    pick = c->grad_variant;
-    for (int v = 0; v < 2; ++v) {
-        c->grad_variant = v;
-        OFX_TRY(ofx_time_kernel(c, 0, iters, &ms));
-        if (ms < best) { best = ms; pick = v; }
+    // NS gradient, limiter and gather variants (kernel parts 0, 1, 3)
+    int* var[3] = {&c->grad_variant, &c->lim_variant, &c->gather_variant};
+    const int part[3] = {0, 1, 3};
+    for (int q = 0; q < 3; ++q) {
+        best = 1e300;
+        pick = *var[q];
+        for (int v = 0; v < 2; ++v) {
+            *var[q] = v;
+            OFX_TRY(ofx_time_kernel(c, part[q], iters, &ms));
+            if (ms < best) { best = ms; pick = v; }
+        }
+        *var[q] = pick;
     }
-    c->grad_variant = pick;
     if (c->k.vismodel == 3) {
         best = 1e300;
         pick = c->saface_tb;

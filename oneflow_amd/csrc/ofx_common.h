@@ -165,6 +165,8 @@ struct ofx_ctx {
     int par_dt = 0, par_lim = 0, par_fix = 0;  // parity of the reduction buffers in use
     int face_tb = 256, saface_tb = 256;        // face-kernel tile (threads per block)
     int grad_variant = 0;  // NS gradient: 0 one thread per cell, 1 one wave per quantity
+    int lim_variant = 1;     // limiter: 0 lanes per cell, 1 one thread per cell
+    int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
     // several zones: cell / boundary-face starts (nz + 1 each), host copies of
     // the per-zone constants and configurations

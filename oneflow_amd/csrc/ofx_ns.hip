@@ -497,6 +497,123 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
     }
 }
 
+// The same limiter with one thread per cell: the cell's gradients and
+// neighbour extents are loaded once and its faces walked in a loop (the
+// minimum is exact in any order).
+template <int DIM, int MAXF, int LIM>
+__global__ void __launch_bounds__(TB) k_ns_limiter_c(DevMesh m, DevFields d, Consts k, int par) {
+    __shared__ double sext[10];
+    {
+        const unsigned long long* mine = d.slots + (size_t)par * m.nz * RQ_N * NSLOT;
+        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        double x[3];
+        for (int r = 0; r < 3; ++r) {
+            int q = w + 4 * r;
+            x[r] = (q < 10) ? __longlong_as_double((long long)mine[(RQ_QMIN + q) * NSLOT + lane]) : 0.0;
+        }
+        for (int r = 0; r < 3; ++r) {
+            int q = w + 4 * r;
+            double y = x[r];
+            for (int o = 32; o > 0; o >>= 1) { double z = __shfl_xor(y, o); y = (y > z) ? y : z; }
+            if (q < 10 && lane == 0) sext[q] = y;
+        }
+        __syncthreads();
+    }
+    const int T = m.T;
+    const int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
+    int ent[MAXF], nb[MAXF];
+    cell_faces<MAXF>(m, i, ent, nb);
+    double dx[MAXF], dy[MAXF], dz[MAXF], dot[MAXF];
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        if (j < n) {
+            int e = ent[j], f = e >= 0 ? e : ~e;
+            dvec4 rc4 = m.frec[f];
+            dx[j] = e >= 0 ? rc4.x : rc4.z;
+            dy[j] = e >= 0 ? rc4.y : rc4.w;
+            dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
+            if (LIM == 1) {
+                double ds = sqrt(sqr3(dx[j], dy[j], dz[j]));
+                double nz = (DIM == 3) ? m.zfn[f] : 0.0;
+                dot[j] = (m.xfn[f] * dx[j] + m.yfn[f] * dy[j] + nz * dz[j]) / (ds + SMALL);
+                if (e < 0) dot[j] = -dot[j];
+            }
+        }
+    }
+    for (int s = 0; s < 5; ++s) {
+        double lim = 1.0;
+        bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
+        if (!skip) {
+            double gx = d.grad[(0 * 6 + s) * T + i];
+            double gy = d.grad[(1 * 6 + s) * T + i];
+            double gz = (DIM == 3) ? d.grad[(2 * 6 + s) * T + i] : 0.0;
+            double mn = d.minmax[s * m.C + i];
+            double mx = d.minmax[(5 + s) * m.C + i];
+            double eps1 = 0.0;
+            if (LIM == 2) {
+                double lo = sext[s], hi = sext[5 + s];
+                if (m.nz > 1) {
+                    const double* zr = d.zred + (size_t)m.zcell[i] * RQ_N;
+                    lo = zr[RQ_QMIN + s];
+                    hi = zr[RQ_QMAX + s];
+                }
+                double eps = k.vencat_coef * dmax(hi - (-lo), 1.0);
+                eps1 = eps * eps + SMALL;
+            }
+#pragma unroll
+            for (int j = 0; j < MAXF; ++j) {
+                if (j < n) {
+                    double dq = gx * dx[j] + gy * dy[j] + gz * dz[j];
+                    double phi = (LIM == 2) ? vencat_fn(dq, mn, mx, eps1) : barth_fn(dq, mn, mx, dot[j]);
+                    lim = dmin(lim, phi);
+                }
+            }
+        }
+        d.limiter[s * T + i] = dmin(1.0, lim);
+    }
+}
+
+// The residual gather with one thread per cell (face loads of the cell all
+// issued first, then the ordered sums: inviscid chain, then viscous).
+template <int MAXF, bool VISC, bool F32>
+__global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d) {
+    const int T = m.T, F = m.F;
+    const int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
+    int ent[MAXF], nb[MAXF];
+    cell_faces<MAXF>(m, i, ent, nb);
+    const float* inv32 = reinterpret_cast<const float*>(d.invflux);
+    const float* vis32 = reinterpret_cast<const float*>(d.visflux);
+    for (int s = 0; s < 5; ++s) {
+        double fi[MAXF], fv[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                int e = ent[j], f = e >= 0 ? e : ~e;
+                double x = F32 ? (double)inv32[s * F + f] : d.invflux[s * F + f];
+                fi[j] = (e >= 0) ? -x : x;
+                if (VISC) {
+                    double y = F32 ? (double)vis32[s * F + f] : d.visflux[s * F + f];
+                    fv[j] = (e >= 0) ? -y : y;
+                }
+            }
+        }
+        double sum = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j)
+            if (j < n) sum = sum + fi[j];
+        if (VISC) {
+#pragma unroll
+            for (int j = 0; j < MAXF; ++j)
+                if (j < n) sum = sum + fv[j];
+        }
+        d.res[s * T + i] = sum;
+    }
+}
+
 // Viscous face flux (UNsVisFlux::CalcFlux, codes/uns/src/UNsVisFlux.cpp:127-440)
 // for face-gradient method VM: face gradients of u, v, (w,) T, Newtonian
 // stress and heat flux, stored as visflux[5][F].
@@ -741,11 +858,24 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
         }
     }
     if ((parts & 2u) && lim) {
-        if (k.ilim == 2) k_ns_limiter<DIM, LPC, 2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-        else k_ns_limiter<DIM, LPC, 1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        if (c->lim_variant == 1) {
+            if (k.ilim == 2) k_ns_limiter_c<DIM, MAXF, 2><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            else k_ns_limiter_c<DIM, MAXF, 1><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        } else {
+            if (k.ilim == 2) k_ns_limiter<DIM, LPC, 2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            else k_ns_limiter<DIM, LPC, 1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        }
     }
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
-    if (parts & 8u) {
+    if ((parts & 8u) && c->gather_variant == 1) {
+        if (k.flux_fp32) {
+            if (visc) k_ns_gather_c<MAXF, true, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather_c<MAXF, false, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+        } else {
+            if (visc) k_ns_gather_c<MAXF, true, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather_c<MAXF, false, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+        }
+    } else if (parts & 8u) {
         if (k.flux_fp32) {
             if (visc) k_ns_gather<LPC, true, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);
             else k_ns_gather<LPC, false, true><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d);

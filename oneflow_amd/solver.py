@@ -218,6 +218,10 @@ class Solver:
         """NS gradient kernel: 0 one thread per cell, 1 one wave per quantity."""
         self._check(self._L.ofx_set_kernel_variant(self.h, 0, variant))
 
+    def set_kernel_variant(self, kernel, variant):
+        """kernel 0 gradient, 1 limiter, 2 gather; variant 0 or 1 (results identical)."""
+        self._check(self._L.ofx_set_kernel_variant(self.h, kernel, variant))
+
     def autotune(self, iters=20):
         """Pick the fastest face tiles and gradient variant on the current state; returns (ns_tile, sa_tile)."""
         a = np.zeros(2, np.int32)

@@ -1,7 +1,8 @@
 """Launch variants give identical bits.
 
-The NS gradient has two kernels (one thread per cell; one wave per quantity)
-and the face kernels take any face tile.  Every combination must give the
+The NS gradient (one thread per cell; one wave per quantity), the limiter
+and the residual gather (one lane per face; one thread per cell) have two
+kernels each, and the face kernels take any face tile.  Every combination must give the
 same state, gradients, limiter and residual bit for bit, here after twelve
 pseudo-time steps from freestream (through the CFL ramp) on 2-D and 3-D grids
 and several option sets.  Variant 0 with the default tile is checked against
@@ -23,9 +24,12 @@ def test_grad_variant_and_tiles_bitwise(grid, opt):
     from oneflow_amd.solver import Solver
     base, _, g = make(grid, opt)
     runs = [base]
-    for variant, tiles in ((1, (256, 256)), (1, (64, 128)), (0, (512, 64))):
+    # (gradient, limiter, gather) variants and face tiles
+    for variants, tiles in (((1, 0, 0), (256, 256)), ((1, 1, 1), (64, 128)), ((0, 1, 0), (512, 64)),
+                            ((0, 0, 1), (128, 512))):
         s = Solver(g, base.cfg)
-        s.set_grad_variant(variant)
+        for kernel, v in enumerate(variants):
+            s.set_kernel_variant(kernel, v)
         s.set_face_tile(*tiles)
         runs.append(s)
     for s in runs:
