@@ -475,7 +475,11 @@ def residual_bench(s, grid, args, flux_bytes=8):
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
     # each NS residual kernel timed with HIP events on the solver's stream over
     # back-to-back launches on the state the timed steps left
-    names = {"grad": "k_ns_grad", "limiter": "k_ns_limiter", "face": "k_ns_face", "gather": "k_ns_gather"}
+    # kernel names of the launch configuration the autotuner chose
+    names = {"grad": ("k_ns_grad", "k_ns_grad_q")[s.kernel_variant(0)],
+             "limiter": ("k_ns_limiter", "k_ns_limiter_c")[s.kernel_variant(1)],
+             "face": "k_ns_face",
+             "gather": ("k_ns_gather", "k_ns_gather_c")[s.kernel_variant(2)]}
     prof = {key: s.time_kernel(key, iters=args.kernel_iters) for key in names}
     s.residual_ns()  # leave a consistent residual behind
     s.sync()

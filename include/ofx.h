@@ -246,6 +246,7 @@ int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
  * kernel 1, the limiter, and kernel 2, the residual gather: variant 0 one
  * lane per face of a cell, variant 1 (default) one thread per cell. */
 int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
+int ofx_get_kernel_variant(ofx_ctx* ctx, int kernel, int* variant);
 
 /* Autotuner (no counterpart in the reference): times the NS face kernel for
  * each face tile, the NS gradient, limiter and gather for each variant and

@@ -478,6 +478,13 @@ int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     return 0;
 }
 
+int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
+    CHECK_CTX(c);
+    if (!variant || kernel < 0 || kernel > 2) { set_error("kernel variant: kernel 0, 1 or 2 and non-null out"); return OFX_EINVAL; }
+    *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant;
+    return 0;
+}
+
 int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     CHECK_READY(c);
     if (iters < 1) { set_error("autotune: iters >= 1"); return OFX_EINVAL; }

@@ -71,6 +71,7 @@ def load_library():
     L.ofx_wall_count.argtypes = [P, ip]
     L.ofx_wall_data.argtypes = [P, dp]
     L.ofx_set_kernel_variant.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.ofx_get_kernel_variant.argtypes = [P, ctypes.c_int, ip]
     L.ofx_autotune.argtypes = [P, ctypes.c_int, ip]
     L.ofx_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, ctypes.c_int]
     L.ofx_batch_destroy.argtypes = [P]
@@ -217,6 +218,11 @@ class Solver:
     def set_grad_variant(self, variant):
         """NS gradient kernel: 0 one thread per cell, 1 one wave per quantity."""
         self._check(self._L.ofx_set_kernel_variant(self.h, 0, variant))
+
+    def kernel_variant(self, kernel):
+        v = np.zeros(1, np.int32)
+        self._check(self._L.ofx_get_kernel_variant(self.h, kernel, _ip(v)))
+        return int(v[0])
 
     def set_kernel_variant(self, kernel, variant):
         """kernel 0 gradient, 1 limiter, 2 gather; variant 0 or 1 (results identical)."""
