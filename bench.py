@@ -228,72 +228,135 @@ def warmstart_main(args):
     trained first on fields the solver converges itself (a separate seeded
     training sweep, also solved as zones).  Spatial order 1 (ilim = -1) -- the
     reference's second-order Venkat setting does not converge to 1e-8 on these
-    grids."""
-    from concurrent.futures import ProcessPoolExecutor
+    grids.
+
+    --phase all runs everything in one process.  For sweeps whose plain
+    case-after-case baseline does not fit one run, the phases run separately
+    and meet in --ws-dir: train (fields + network -> net.npy, train.json),
+    seq --seq-range a:b (the plain solver on test cases a..b-1 ->
+    seq_a_b.json; the baseline is a sum of independent per-case times), zones
+    (plain and learned zones -> zones.json), combine (the JSON line).  Every
+    phase regenerates the same seeded cases."""
+    from pathlib import Path
 
     from oneflow_amd import warmstart as ws
     from oneflow_amd.cases import airfoil_sweep
     from oneflow_amd.mlp import Mlp
     from oneflow_amd.solver import Solver, ZoneSolver
 
+    log = lambda m: print(f"[warmstart] {m}", file=sys.stderr, flush=True)
+    phase = args.phase
+    wsdir = Path(args.ws_dir)
+    wsdir.mkdir(parents=True, exist_ok=True)
     train = airfoil_sweep(args.n_train, seed=args.seed + 1, min_cells=20_000, max_cells=50_000)
     test = airfoil_sweep(args.n_test, seed=args.seed, min_cells=20_000, max_cells=60_000)
-    t0 = time.perf_counter()
-    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        grids = list(ex.map(_sweep_grid_o1, train + test))
-    gtrain, gtest = grids[:len(train)], grids[len(train):]
-    log = lambda m: print(f"[warmstart] {m}", file=sys.stderr, flush=True)
-    log(f"grids: train {sum(g.n_cells for g in gtrain)} cells, test {sum(g.n_cells for g in gtest)} cells, "
-        f"{time.perf_counter() - t0:.1f} s")
     cfg_of = lambda c: c.config(ilim=-1, cfled=args.cfl)
+    if phase == "combine":
+        return _warmstart_combine(args, wsdir, train, test)
 
-    # 1. training fields: the training cases as zones of one context, a fixed number of steps
-    zt = ZoneSolver(gtrain, [cfg_of(c) for c in train])
-    zt.init_freestream()
-    t0 = time.perf_counter()
-    done = 0
-    while done < args.train_steps:
-        k = min(5000, args.train_steps - done)
-        zt.iterate(1 + done, k, with_norms=False)
-        zt.sync()
-        done += k
-        log(f"training fields: {done} steps, {time.perf_counter() - t0:.1f} s")
-    t_gen = time.perf_counter() - t0
-    views = [ws.ZoneView(zt, z) for z in range(zt.n_zones)]
-    X = np.concatenate([ws.cell_features(v, c) for v, c in zip(views, train)])
-    Y = np.concatenate([ws.cell_targets(v, c, v.cfg) for v, c in zip(views, train)])
-    zt.close()
-    log(f"training fields: {X.shape[0]} samples in {t_gen:.1f} s")
+    def grids_of(cases):
+        from concurrent.futures import ProcessPoolExecutor
+        t0 = time.perf_counter()
+        with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            g = list(ex.map(_sweep_grid_o1, cases))
+        log(f"{len(g)} grids, {sum(x.n_cells for x in g)} cells, {time.perf_counter() - t0:.1f} s")
+        return g
 
-    # 2. train the network on the GPU
-    net = Mlp(seed=args.seed)
-    t0 = time.perf_counter()
-    losses = net.train(X, Y, batch=8192, steps=args.mlp_steps, lr=1e-3)
-    t_train = time.perf_counter() - t0
-    flops_step = 3 * 2 * 8192 * (16 * 256 + 4 * 256 * 256 + 256 * 64)
-    log(f"trained {args.mlp_steps} steps in {t_train:.1f} s, loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
+    info = {}
+    if phase in ("all", "train"):
+        gtrain = grids_of(train)
+        # 1. training fields: the training cases as zones of one context, a fixed number of steps
+        zt = ZoneSolver(gtrain, [cfg_of(c) for c in train])
+        zt.init_freestream()
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.train_steps:
+            k = min(5000, args.train_steps - done)
+            zt.iterate(1 + done, k, with_norms=False)
+            zt.sync()
+            done += k
+            log(f"training fields: {done} steps, {time.perf_counter() - t0:.1f} s")
+        t_gen = time.perf_counter() - t0
+        views = [ws.ZoneView(zt, z) for z in range(zt.n_zones)]
+        X = np.concatenate([ws.cell_features(v, c) for v, c in zip(views, train)])
+        Y = np.concatenate([ws.cell_targets(v, c, v.cfg) for v, c in zip(views, train)])
+        zt.close()
+        del gtrain
+        log(f"training fields: {X.shape[0]} samples in {t_gen:.1f} s")
+        # 2. train the network on the GPU
+        net = Mlp(seed=args.seed)
+        t0 = time.perf_counter()
+        losses = net.train(X, Y, batch=8192, steps=args.mlp_steps, lr=1e-3)
+        t_train = time.perf_counter() - t0
+        flops_step = 3 * 2 * 8192 * (16 * 256 + 4 * 256 * 256 + 256 * 64)
+        log(f"trained {args.mlp_steps} steps in {t_train:.1f} s, loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
+        xt = np.random.default_rng(0).normal(0, 1, (1 << 20, 9)).astype(np.float32)
+        net.forward(xt)  # sizes the activation buffers
+        t0 = time.perf_counter()
+        net.forward(xt)
+        t_fwd = time.perf_counter() - t0
+        info = {"train_cases": len(train), "train_samples": int(X.shape[0]), "train_steps": args.train_steps,
+                "training_fields_s": t_gen, "training_s": t_train,
+                "final_loss": float(losses[-50:].mean()), "first_loss": float(losses[:10].mean()),
+                "train_tflops": flops_step * args.mlp_steps / t_train / 1e12, "forward_1M_ms": 1e3 * t_fwd,
+                "forward_tflops": 2.0 * (1 << 20) * (16 * 256 + 4 * 256 * 256 + 256 * 64) / t_fwd / 1e12}
+        del X, Y
+        if phase == "train":
+            np.save(wsdir / "net.npy", net.params())
+            (wsdir / "train.json").write_text(json.dumps(info))
+            log(f"network and training record written to {wsdir}")
+            return
 
     # 3. unseen cases: reference levels (untimed pre-pass of the plain run)
-    def fresh():
-        out = []
-        for g, c in zip(gtest, test):
-            s = Solver(g, cfg_of(c))
-            s.init_freestream()
-            out.append(s)
-        return out
-    pre = fresh()
-    peaks = np.array([s.iterate(1, 1000).max(axis=0) for s in pre])  # [case][6] peak norm per equation
-    refs = [float(v) for v in peaks[:, 0]]
-    for s in pre:
-        s.close()
+    lo, hi = 0, len(test)
+    if phase == "seq" and args.seq_range:
+        lo, hi = (int(v) for v in args.seq_range.split(":"))
+    cases = test[lo:hi] if phase == "seq" else test
+    gtest = grids_of(cases)
 
-    seq = fresh()
-    prog = lambda i, n: log(f"  case {i} done at step {n}")
-    t_seq, st_seq = ws.solve_to_tolerance(seq, refs, tol=args.tol, batched=False, progress=prog)
-    aero_seq = np.array([s.aero() for s in seq])
-    for s in seq:
+    def fresh(i):
+        s = Solver(gtest[i], cfg_of(cases[i]))
+        s.init_freestream()
+        return s
+    peaks = []
+    for i in range(len(cases)):
+        s = fresh(i)
+        peaks.append(s.iterate(1, 1000).max(axis=0))  # peak norm per equation over the first 1000 steps
         s.close()
-    log(f"plain, case after case: {t_seq:.1f} s")
+    peaks = np.array(peaks)
+    refs = [float(v) for v in peaks[:, 0]]
+    prog = lambda i, n: log(f"  case {lo + i} done at step {n}")
+    seq = {}
+    if phase in ("all", "seq"):
+        # plain solver, case after case: one context at a time, each timed alone
+        t_case, st_seq, aero_seq = [], [], []
+        for i in range(len(cases)):
+            s = fresh(i)
+            t, st = ws.solve_to_tolerance([s], [refs[i]], tol=args.tol, batched=False)
+            t_case.append(t)
+            st_seq.append(int(st[0]))
+            aero_seq.append(s.aero().tolist())
+            s.close()
+            prog(i, st[0])
+        seq = {"range": [lo, hi], "seconds_per_case": t_case, "steps": st_seq, "aero": aero_seq}
+        log(f"plain, case after case: {sum(t_case):.1f} s")
+        if phase == "seq":
+            (wsdir / f"seq_{lo}_{hi}.json").write_text(json.dumps(seq))
+            return
+
+    if phase == "zones":
+        net = Mlp(seed=args.seed)
+        net.set_params(np.load(wsdir / "net.npy"))
+    zres = _warmstart_zones(args, test, gtest, cfg_of, refs, peaks, ws, ZoneSolver, log, net)
+    if phase == "zones":
+        (wsdir / "zones.json").write_text(json.dumps(zres))
+        return
+    print(json.dumps(_warmstart_result(args, train, test, info, [seq], zres)))
+
+
+def _warmstart_zones(args, test, gtest, cfg_of, refs, peaks, ws, ZoneSolver, log, net):
+    """Plain and learned zones to the tolerance (and optionally past it)."""
+    prog = lambda i, n: log(f"  zone {i} done at step {n}")
 
     def zones():
         z = ZoneSolver(gtest, [cfg_of(c) for c in test])
@@ -312,30 +375,36 @@ def warmstart_main(args):
     t_zl, st_zl = ws.solve_zones_to_tolerance(zl, refs, tol=args.tol, progress=prog)
     aero_zl = zl.zone_aero()
     log(f"learned, zones: {t_inf:.2f} s inference + {t_zl:.1f} s")
-
-    def rel(a, b):
-        return [float(np.max(np.abs(a[:, k] - b[:, k]) / np.maximum(np.abs(b[:, k]), 1e-12))) for k in (0, 1)]
-    d_seq = rel(aero_zl, aero_seq)
-    # both zone runs continued (untimed) until every equation's norm (SA
-    # included) is 100x below tol times its peak: the forces both starts reach
-    it_p, it_l = max(st_zp) + 1, max(st_zl) + 1
-    lim = args.tol * 1e-2 * np.where(peaks > 0, peaks, np.inf)
-    for zs, it in ((zp, it_p), (zl, it_l)):
-        while np.any(zs.zone_norms() > lim) and it < 400_000:
-            zs.iterate(it, 2000, with_norms=False)
-            it += 2000
-        log(f"continued to step {it - 1}, worst norm / peak {float(np.max(zs.zone_norms() / np.where(peaks > 0, peaks, np.inf))):.2e}")
-    aero_tp, aero_tl = zp.zone_aero(), zl.zone_aero()
-    d_tight = rel(aero_tl, aero_tp)
+    out = {"plain_zones_s": t_zp, "learned_zones_s": t_zl, "inference_s": t_inf,
+           "plain_zones_steps": [int(v) for v in st_zp], "learned_zones_steps": [int(v) for v in st_zl],
+           "aero_plain_zones": aero_zp.tolist(), "aero_learned_zones": aero_zl.tolist()}
+    if args.tight:
+        # both zone runs continued (untimed) until every equation's norm (SA
+        # included) is 100x below tol times its peak: the forces both starts reach
+        lim = args.tol * 1e-2 * np.where(peaks > 0, peaks, np.inf)
+        for zs, it in ((zp, max(st_zp) + 1), (zl, max(st_zl) + 1)):
+            while np.any(zs.zone_norms() > lim) and it < 400_000:
+                zs.iterate(it, 2000, with_norms=False)
+                it += 2000
+            log(f"continued to step {it - 1}")
+        out["aero_plain_tight"] = zp.zone_aero().tolist()
+        out["aero_learned_tight"] = zl.zone_aero().tolist()
     zp.close()
     zl.close()
-    log(f"aero learned vs plain: at tol {d_seq}, at tol/100 {d_tight}")
-    # network throughput (forward of 2^20 samples, MFMA)
-    xt = np.random.default_rng(0).normal(0, 1, (1 << 20, 9)).astype(np.float32)
-    net.forward(xt)  # sizes the activation buffers
-    t0 = time.perf_counter()
-    net.forward(xt)
-    t_fwd = time.perf_counter() - t0
+    return out
+
+
+def _warmstart_result(args, train, test, info, seqs, z):
+    t_case = [t for s in seqs for t in s["seconds_per_case"]]
+    st_seq = [v for s in seqs for v in s["steps"]]
+    aero_seq = np.array([a for s in seqs for a in s["aero"]])
+    t_seq = float(sum(t_case))
+    t_zp, t_zl, t_inf = z["plain_zones_s"], z["learned_zones_s"], z["inference_s"]
+    aero_zl = np.array(z["aero_learned_zones"])
+
+    def rel(a, b, k):
+        return np.abs(a[:, k] - b[:, k]) / np.maximum(np.abs(b[:, k]), 1e-12)
+    dcl, dcd = rel(aero_zl, aero_seq, 0), rel(aero_zl, aero_seq, 1)
     out = {
         "metric": "time to a 1e-8 residual drop, learned vs plain",
         "value": t_seq / (t_inf + t_zl),
@@ -345,23 +414,30 @@ def warmstart_main(args):
         "dtype": "f64 solver, bf16 network",
         "data": "synthetic",
         "config": {"workload": f"{len(test)} unseen NACA 4-digit airfoils (20k-60k cells), Roe + SA, first order",
-                   "train_cases": len(train), "train_samples": int(X.shape[0]), "tol": args.tol, "cfl": args.cfl},
-        "seconds": {"plain_sequential": t_seq, "plain_zones": t_zp, "learned_zones": t_zl,
-                    "inference": t_inf, "training_fields": t_gen, "training": t_train},
-        "steps": {"plain_per_case": [int(v) for v in st_seq], "plain_zones_per_case": [int(v) for v in st_zp],
-                  "learned_zones_per_case": [int(v) for v in st_zl]},
+                   "train_cases": len(train), "tol": args.tol, "cfl": args.cfl},
+        "seconds": {"plain_sequential": t_seq, "plain_zones": t_zp, "learned_zones": t_zl, "inference": t_inf},
         "speedup": {"zones": t_seq / t_zp, "warm_start": t_zp / (t_inf + t_zl), "total": t_seq / (t_inf + t_zl)},
-        "aero_max_rel_diff": {"learned_vs_plain_at_tol": {"CL": d_seq[0], "CD": d_seq[1]},
-                              "learned_vs_plain_at_tol_over_100": {"CL": d_tight[0], "CD": d_tight[1]}},
-        "per_case": {"mach": [c.mach for c in test], "aoa": [c.aoa for c in test],
-                     "cl_cd_plain_at_tol": aero_seq[:, :2].tolist(), "cl_cd_learned_at_tol": aero_zl[:, :2].tolist(),
-                     "cl_cd_plain_tight": aero_tp[:, :2].tolist(), "cl_cd_learned_tight": aero_tl[:, :2].tolist()},
-        "mlp": {"final_loss": float(losses[-50:].mean()), "first_loss": float(losses[:10].mean()),
-                "train_tflops": flops_step * args.mlp_steps / t_train / 1e12,
-                "forward_1M_ms": 1e3 * t_fwd,
-                "forward_tflops": 2.0 * (1 << 20) * (16 * 256 + 4 * 256 * 256 + 256 * 64) / t_fwd / 1e12},
+        "aero_rel_diff_learned_vs_plain": {
+            "max_CL": float(dcl.max()), "max_CD": float(dcd.max()),
+            "cases_within_1e-6": int(np.sum((dcl <= 1e-6) & (dcd <= 1e-6))), "cases": len(test)},
+        "steps": {"plain_per_case": st_seq, "learned_zones_per_case": z["learned_zones_steps"]},
+        "training": info,
     }
-    print(json.dumps(out))
+    if "aero_plain_tight" in z:
+        tp, tl = np.array(z["aero_plain_tight"]), np.array(z["aero_learned_tight"])
+        out["aero_rel_diff_learned_vs_plain_tight"] = {"max_CL": float(rel(tl, tp, 0).max()),
+                                                       "max_CD": float(rel(tl, tp, 1).max())}
+    return out
+
+
+def _warmstart_combine(args, wsdir, train, test):
+    info = json.loads((wsdir / "train.json").read_text())
+    seqs = sorted((json.loads(p.read_text()) for p in wsdir.glob("seq_*.json")), key=lambda s: s["range"][0])
+    covered = [i for s in seqs for i in range(*s["range"])]
+    if covered != list(range(len(test))):
+        raise SystemExit(f"seq phases cover {covered[:3]}..{covered[-3:]} of {len(test)} cases; run the missing ranges")
+    z = json.loads((wsdir / "zones.json").read_text())
+    print(json.dumps(_warmstart_result(args, train, test, info, seqs, z)))
 
 
 def main():
@@ -385,6 +461,11 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=3000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cfl", type=float, default=10.0, help="final CFL of the warm-start cases")
+    ap.add_argument("--phase", default="all", choices=["all", "train", "seq", "zones", "combine"],
+                    help="warmstart: run everything, or one phase of a split run (see warmstart_main)")
+    ap.add_argument("--ws-dir", default="warmstart_run", help="warmstart: where split phases meet")
+    ap.add_argument("--seq-range", default="", help="warmstart seq phase: test cases a:b")
+    ap.add_argument("--tight", action="store_true", help="warmstart: continue both zone runs 100x past tol")
     ap.add_argument("--cases", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--graph-steps", type=int, default=10, help="steps recorded per graph replay (batch)")

@@ -137,6 +137,7 @@ static void set_consts(ofx_ctx* c) {
 }
 
 int init_freestream(ofx_ctx* c) {
+    OFX_TRY(ensure_dist(c));
     const long T = c->m.T;
     if (c->m.nz > 1) {
         // each zone's cells and ghosts take that zone's freestream
@@ -308,6 +309,7 @@ static int field_view(ofx_ctx* c, int field, double** p, size_t* n) {
 
 int ofx_get_field(ofx_ctx* c, int field, double* host) {
     CHECK_READY(c);
+    if (field == OFX_F_DIST) OFX_TRY(ensure_dist(c));
     if (!host) { set_error("null host pointer"); return OFX_EINVAL; }
     const size_t T = c->m.T, F = c->m.F;
     const DevMesh& m = c->m;
@@ -399,6 +401,7 @@ extern "C" {
 
 int ofx_iterate(ofx_ctx* c, int first_iter, int n, ofx_norms* norms_out) {
     CHECK_READY(c);
+    OFX_TRY(ensure_dist(c));
     if (n < 0) { set_error("n must be >= 0"); return OFX_EINVAL; }
     for (int s = 0; s < n; ++s) {
         OFX_TRY(step_launch(c, first_iter + s));
@@ -602,7 +605,7 @@ int ofx_zones_set(ofx_ctx* c, int n, const int* cellStart, const int* bfaceStart
     c->zc.assign(cellStart, cellStart + n + 1);
     c->zb.assign(bfaceStart, bfaceStart + n + 1);
     c->zcfg.assign(cfgs, cfgs + n);
-    OFX_TRY(wall_distance(c));
+    c->dist_valid = false;  // per-zone walls from now on, computed on first use
     OFX_HIP(hipStreamSynchronize(c->stream));
     ++c->gen;
     return 0;

@@ -97,6 +97,9 @@ int ofx_batch_create(ofx_batch** out, ofx_ctx* const* ctxs, int n, int steps_per
         ofx_ctx* c = ctxs[i];
         if (!c || !c->has_mesh || !c->has_cfg) { set_error("batch_create: every context needs a mesh and a config"); return OFX_ESTATE; }
         if (c->device != dev) { set_error("batch_create: all contexts must be on one device"); return OFX_EINVAL; }
+        // lazily computed state must exist before any step is recorded into a graph
+        OFX_TRY(ensure_dist(c));
+        OFX_HIP(hipStreamSynchronize(c->stream));
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == c) { set_error("batch_create: a context appears twice"); return OFX_EINVAL; }
         ramp = std::max(ramp, c->cfg.ncfl);

@@ -168,6 +168,7 @@ struct ofx_ctx {
     int lim_variant = 1;     // limiter: 0 lanes per cell, 1 one thread per cell
     int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
+    bool dist_valid = false;  // wall distance computed for the current mesh / zones
     // several zones: cell / boundary-face starts (nz + 1 each), host copies of
     // the per-zone constants and configurations
     std::vector<int> zc, zb;
@@ -459,6 +460,7 @@ int ns_update(ofx_ctx* c);
 int sa_update(ofx_ctx* c);
 int norms(ofx_ctx* c, ofx_norms* out);
 int wall_distance(ofx_ctx* c);
+inline int ensure_dist(ofx_ctx* c) { return c->dist_valid ? 0 : wall_distance(c); }
 int norms_range(ofx_ctx* c, int c0, int c1, ofx_norms* out);
 int aero_zone(ofx_ctx* c, int z, double* out4);
 int wall_data(ofx_ctx* c, double* host_out);

@@ -551,8 +551,9 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_HIP(hipGetLastError());
     k_face_vis_static<<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());
-    k_wall_distance<<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
-    OFX_HIP(hipGetLastError());
+    // the wall distance is computed on first use (ensure_dist): a mesh that
+    // is declared as several zones next searches each zone's walls only
+    c->dist_valid = false;
     OFX_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -561,6 +562,7 @@ int wall_distance(ofx_ctx* c) {
     const int TB = 256;
     if (c->m.C) k_wall_distance<<<(c->m.C + TB - 1) / TB, TB, 0, c->stream>>>(c->m);
     OFX_HIP(hipGetLastError());
+    c->dist_valid = true;
     return 0;
 }
 

@@ -1106,7 +1106,9 @@ static void launch_lusgs(ofx_ctx* c) {
             else k_ns_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
         } else {
             const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
-            for (size_t L = 0; L + 1 < off.size(); ++L) {
+            // upper level 0 holds the cells without a higher neighbour: their
+            // increment is dq += (-0)/D, which leaves dq unchanged bit for bit
+            for (size_t L = up ? 1 : 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
                 if (up)

@@ -356,6 +356,7 @@ static void launch_sa(ofx_ctx* c) {
 }
 
 int sa_residual(ofx_ctx* c) {
+    OFX_TRY(ensure_dist(c));
     if (c->k.vismodel != 3) return 0;
     int mf = c->m.maxFacesPerCell;
     if (c->m.dim == 2) {
@@ -436,7 +437,8 @@ static void launch_sa_lusgs(ofx_ctx* c) {
             else k_sa_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d);
         } else {
             const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
-            for (size_t L = 0; L + 1 < off.size(); ++L) {
+            // upper level 0: no higher neighbour, turbdq += (-0)/D changes nothing
+            for (size_t L = up ? 1 : 0; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
                 if (up) k_sa_sweep_level<true, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e, base[L]);
