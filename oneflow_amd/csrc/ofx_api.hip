@@ -385,12 +385,22 @@ namespace ofx {
 // it can also be recorded into a HIP graph.
 int step_launch(ofx_ctx* c, int iter) {
     OFX_TRY(ns_timestep(c, iter));
-    OFX_TRY(ns_residual(c));
-    OFX_TRY(ns_lusgs(c));
+    // the first lower-sweep level is computed by the residual gathers when the
+    // schedule allows it (fusable_l0), and the sweeps start at level 1
+    const int L0 = fusable_l0(c);
+    const int ns_l0 = c->gather_variant == 1 ? L0 : 0;
+    c->fuse_l0 = ns_l0;
+    int rc = ns_residual(c);
+    c->fuse_l0 = 0;
+    OFX_TRY(rc);
+    OFX_TRY(ns_lusgs(c, ns_l0 > 0 ? 1 : 0));
     OFX_TRY(ns_update_fused(c));
     if (c->k.vismodel == 3) {
-        OFX_TRY(sa_residual(c));
-        OFX_TRY(sa_lusgs(c));
+        c->fuse_l0 = L0;
+        rc = sa_residual(c);
+        c->fuse_l0 = 0;
+        OFX_TRY(rc);
+        OFX_TRY(sa_lusgs(c, L0 > 0 ? 1 : 0));
         OFX_TRY(sa_update_fused(c));
     }
     return 0;

@@ -169,6 +169,7 @@ struct ofx_ctx {
     int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
     bool dist_valid = false;  // wall distance computed for the current mesh / zones
+    int fuse_l0 = 0;  // > 0 while a full step fuses the first lower-sweep level into the gathers
     // several zones: cell / boundary-face starts (nz + 1 each), host copies of
     // the per-zone constants and configurations
     std::vector<int> zc, zb;
@@ -454,8 +455,9 @@ int ns_residual(ofx_ctx* c);
 int ns_residual_parts(ofx_ctx* c, unsigned parts);
 int step_launch(ofx_ctx* c, int iter);
 int sa_residual(ofx_ctx* c);
-int ns_lusgs(ofx_ctx* c);
-int sa_lusgs(ofx_ctx* c);
+int ns_lusgs(ofx_ctx* c, int lo_first = 0);
+int fusable_l0(const ofx_ctx* c);
+int sa_lusgs(ofx_ctx* c, int lo_first = 0);
 int ns_update(ofx_ctx* c);
 int sa_update(ofx_ctx* c);
 int norms(ofx_ctx* c, ofx_norms* out);

@@ -577,11 +577,21 @@ __global__ void __launch_bounds__(TB) k_ns_limiter_c(DevMesh m, DevFields d, Con
 
 // The residual gather with one thread per cell (face loads of the cell all
 // issued first, then the ordered sums: inviscid chain, then viscous).
+// L0 (> 0 inside a full pseudo-time step): cells [0, L0) are the first level
+// of the lower LU-SGS sweep (no lower-numbered neighbour), whose increment is
+// dq = (res - 0) / D; the gather writes it with the residual and the sweep
+// starts at its second level.
 template <int MAXF, bool VISC, bool F32>
-__global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d) {
+__global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d, Consts k0, int par, int L0) {
     const int T = m.T, F = m.F;
+    const double mindt1 = (L0 > 0) ? slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN) : 0.0;
     const int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
+    double rad = 1.0;
+    if (i < L0) {
+        const Consts k = zoned(k0, m, cell_zone(m, i));
+        rad = ns_impsr(m, d, k, i, zone_mindt(m, d, i, mindt1));
+    }
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
@@ -611,6 +621,7 @@ __global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d) {
                 if (j < n) sum = sum + fv[j];
         }
         d.res[s * T + i] = sum;
+        if (i < L0) { double t = sum - 0.0; t /= rad; d.dq[s * T + i] = t; }
     }
 }
 
@@ -868,12 +879,13 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
     }
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
     if ((parts & 8u) && c->gather_variant == 1) {
+        const int L0 = c->fuse_l0;
         if (k.flux_fp32) {
-            if (visc) k_ns_gather_c<MAXF, true, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
-            else k_ns_gather_c<MAXF, false, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+            if (visc) k_ns_gather_c<MAXF, true, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+            else k_ns_gather_c<MAXF, false, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
         } else {
-            if (visc) k_ns_gather_c<MAXF, true, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
-            else k_ns_gather_c<MAXF, false, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+            if (visc) k_ns_gather_c<MAXF, true, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+            else k_ns_gather_c<MAXF, false, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
         }
     } else if (parts & 8u) {
         if (k.flux_fp32) {
@@ -1096,7 +1108,7 @@ bool use_level_launches(const std::vector<int>& off) {
 }
 
 template <int MAXF>
-static void launch_lusgs(ofx_ctx* c) {
+static void launch_lusgs(ofx_ctx* c, int lo_first) {
     const DevMesh& m = c->m;
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
@@ -1108,7 +1120,7 @@ static void launch_lusgs(ofx_ctx* c) {
             const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
             // upper level 0 holds the cells without a higher neighbour: their
             // increment is dq += (-0)/D, which leaves dq unchanged bit for bit
-            for (size_t L = up ? 1 : 0; L + 1 < off.size(); ++L) {
+            for (size_t L = up ? 1 : (size_t)lo_first; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
                 if (up)
@@ -1120,12 +1132,21 @@ static void launch_lusgs(ofx_ctx* c) {
     }
 }
 
-int ns_lusgs(ofx_ctx* c) {
+// The first level of the lower sweep that may be fused into the residual
+// gathers: its cells must be the run 0..loOff[1]-1 and the sweeps launched
+// per level.  Returns loOff[1], or 0.
+int fusable_l0(const ofx_ctx* c) {
+    const HostSched& h = c->sched;
+    if (!use_level_launches(h.loOff) || h.loBase.empty() || h.loBase[0] != 0) return 0;
+    return h.loOff[1];
+}
+
+int ns_lusgs(ofx_ctx* c, int lo_first) {
     int mf = c->m.maxFacesPerCell;
-    if (mf <= 4) launch_lusgs<4>(c);
-    else if (mf <= 6) launch_lusgs<6>(c);
-    else if (mf <= 8) launch_lusgs<8>(c);
-    else if (mf <= 16) launch_lusgs<16>(c);
+    if (mf <= 4) launch_lusgs<4>(c, lo_first);
+    else if (mf <= 6) launch_lusgs<6>(c, lo_first);
+    else if (mf <= 8) launch_lusgs<8>(c, lo_first);
+    else if (mf <= 16) launch_lusgs<16>(c, lo_first);
     else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
     OFX_HIP(hipGetLastError());
     return 0;

@@ -299,7 +299,7 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
 // gathered in ascending face order) -- TurbCalcRHS order
 // (codes/turb/src/TurbRhs.cpp:57-71).
 template <int MAXF>
-__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k0, int par) {
+__global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k0, int par, int L0) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
@@ -345,6 +345,8 @@ __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts 
         if (j < n) { if (ent[j] >= 0) res -= df[j]; else res += df[j]; }
     d.turbimpsr[i] = imp;
     d.turbres[i] = res;
+    // cells [0, L0): the first level of the lower sweep, turbdq = (res - 0) / D
+    if (i < L0) { double t = res - 0.0; t /= imp; d.turbdq[i] = t; }
 }
 
 template <int DIM, int MAXF>
@@ -352,7 +354,7 @@ static void launch_sa(ofx_ctx* c) {
     const DevMesh& m = c->m;
     k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
     k_sa_face<DIM><<<(m.F + c->saface_tb - 1) / c->saface_tb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k);
-    k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+    k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt, c->fuse_l0);
 }
 
 int sa_residual(ofx_ctx* c) {
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(TBS) k_sa_sweep_level(DevMesh m, DevFields d, 
 }
 
 template <int MAXF>
-static void launch_sa_lusgs(ofx_ctx* c) {
+static void launch_sa_lusgs(ofx_ctx* c, int lo_first) {
     const DevMesh& m = c->m;
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
@@ -438,7 +440,7 @@ static void launch_sa_lusgs(ofx_ctx* c) {
         } else {
             const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
             // upper level 0: no higher neighbour, turbdq += (-0)/D changes nothing
-            for (size_t L = up ? 1 : 0; L + 1 < off.size(); ++L) {
+            for (size_t L = up ? 1 : (size_t)lo_first; L + 1 < off.size(); ++L) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
                 if (up) k_sa_sweep_level<true, MAXF><<<nblks(e - b), TBS, 0, c->stream>>>(m, c->d, b, e, base[L]);
@@ -448,13 +450,13 @@ static void launch_sa_lusgs(ofx_ctx* c) {
     }
 }
 
-int sa_lusgs(ofx_ctx* c) {
+int sa_lusgs(ofx_ctx* c, int lo_first) {
     if (c->k.vismodel != 3) return 0;
     int mf = c->m.maxFacesPerCell;
-    if (mf <= 4) launch_sa_lusgs<4>(c);
-    else if (mf <= 6) launch_sa_lusgs<6>(c);
-    else if (mf <= 8) launch_sa_lusgs<8>(c);
-    else if (mf <= 16) launch_sa_lusgs<16>(c);
+    if (mf <= 4) launch_sa_lusgs<4>(c, lo_first);
+    else if (mf <= 6) launch_sa_lusgs<6>(c, lo_first);
+    else if (mf <= 8) launch_sa_lusgs<8>(c, lo_first);
+    else if (mf <= 16) launch_sa_lusgs<16>(c, lo_first);
     else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
     OFX_HIP(hipGetLastError());
     return 0;
