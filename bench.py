@@ -362,23 +362,29 @@ def _warmstart_zones(args, test, gtest, cfg_of, refs, peaks, ws, ZoneSolver, log
         z = ZoneSolver(gtest, [cfg_of(c) for c in test])
         z.init_freestream()
         return z
+    # converged cases leave: the unconverged ones move into a smaller context
+    rebuild = None
+    if args.compact > 0:
+        rebuild = lambda ids: ZoneSolver([gtest[i] for i in ids], [cfg_of(test[i]) for i in ids])
     zp = zones()
-    t_zp, st_zp = ws.solve_zones_to_tolerance(zp, refs, tol=args.tol, progress=prog)
-    aero_zp = zp.zone_aero()
+    t_zp, st_zp, aero_zp = ws.solve_zones_to_tolerance(zp, refs, tol=args.tol, progress=prog, rebuild=rebuild,
+                                                       compact=args.compact)
+    aero_zp = np.array(aero_zp)
     log(f"plain, zones: {t_zp:.1f} s")
     zl = zones()
     zl.sync()
     t0 = time.perf_counter()
-    ws.apply_predictions_zones(zl, test, net)
+    ws.apply_predictions_zones(zl, test, net, mach_gate=args.mach_gate if args.mach_gate > 0 else None)
     zl.sync()
     t_inf = time.perf_counter() - t0
-    t_zl, st_zl = ws.solve_zones_to_tolerance(zl, refs, tol=args.tol, progress=prog)
-    aero_zl = zl.zone_aero()
+    t_zl, st_zl, aero_zl = ws.solve_zones_to_tolerance(zl, refs, tol=args.tol, progress=prog, rebuild=rebuild,
+                                                       compact=args.compact)
+    aero_zl = np.array(aero_zl)
     log(f"learned, zones: {t_inf:.2f} s inference + {t_zl:.1f} s")
     out = {"plain_zones_s": t_zp, "learned_zones_s": t_zl, "inference_s": t_inf,
            "plain_zones_steps": [int(v) for v in st_zp], "learned_zones_steps": [int(v) for v in st_zl],
            "aero_plain_zones": aero_zp.tolist(), "aero_learned_zones": aero_zl.tolist()}
-    if args.tight:
+    if args.tight and args.compact <= 0:
         # both zone runs continued (untimed) until every equation's norm (SA
         # included) is 100x below tol times its peak: the forces both starts reach
         lim = args.tol * 1e-2 * np.where(peaks > 0, peaks, np.inf)
@@ -414,7 +420,8 @@ def _warmstart_result(args, train, test, info, seqs, z):
         "dtype": "f64 solver, bf16 network",
         "data": "synthetic",
         "config": {"workload": f"{len(test)} unseen NACA 4-digit airfoils (20k-60k cells), Roe + SA, first order",
-                   "train_cases": len(train), "tol": args.tol, "cfl": args.cfl},
+                   "train_cases": len(train), "tol": args.tol, "cfl": args.cfl, "mach_gate": args.mach_gate,
+                   "compact": args.compact},
         "seconds": {"plain_sequential": t_seq, "plain_zones": t_zp, "learned_zones": t_zl, "inference": t_inf},
         "speedup": {"zones": t_seq / t_zp, "warm_start": t_zp / (t_inf + t_zl), "total": t_seq / (t_inf + t_zl)},
         "aero_rel_diff_learned_vs_plain": {
@@ -466,6 +473,11 @@ def main():
     ap.add_argument("--ws-dir", default="warmstart_run", help="warmstart: where split phases meet")
     ap.add_argument("--seq-range", default="", help="warmstart seq phase: test cases a:b")
     ap.add_argument("--tight", action="store_true", help="warmstart: continue both zone runs 100x past tol")
+    ap.add_argument("--mach-gate", type=float, default=0.0,
+                    help="warmstart: cases above this Mach keep the freestream start (0: no gate)")
+    ap.add_argument("--compact", type=float, default=0.5,
+                    help="warmstart: move unconverged cases into a smaller context once they are at most this "
+                         "fraction of the zones (0: never)")
     ap.add_argument("--cases", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--graph-steps", type=int, default=10, help="steps recorded per graph replay (batch)")

@@ -581,14 +581,15 @@ __global__ void __launch_bounds__(TB) k_ns_limiter_c(DevMesh m, DevFields d, Con
 // of the lower LU-SGS sweep (no lower-numbered neighbour), whose increment is
 // dq = (res - 0) / D; the gather writes it with the residual and the sweep
 // starts at its second level.
-template <int MAXF, bool VISC, bool F32>
-__global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d, Consts k0, int par, int L0) {
+template <int MAXF, bool VISC, bool F32, bool FUSE>
+__device__ __forceinline__ void ns_gather_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int par,
+                                               int L0) {
     const int T = m.T, F = m.F;
-    const double mindt1 = (L0 > 0) ? slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN) : 0.0;
+    const double mindt1 = FUSE ? slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN) : 0.0;
     const int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     double rad = 1.0;
-    if (i < L0) {
+    if (FUSE && i < L0) {
         const Consts k = zoned(k0, m, cell_zone(m, i));
         rad = ns_impsr(m, d, k, i, zone_mindt(m, d, i, mindt1));
     }
@@ -621,8 +622,18 @@ __global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d, Cons
                 if (j < n) sum = sum + fv[j];
         }
         d.res[s * T + i] = sum;
-        if (i < L0) { double t = sum - 0.0; t /= rad; d.dq[s * T + i] = t; }
+        if (FUSE && i < L0) { double t = sum - 0.0; t /= rad; d.dq[s * T + i] = t; }
     }
+}
+
+template <int MAXF, bool VISC, bool F32>
+__global__ void __launch_bounds__(TB) k_ns_gather_c(DevMesh m, DevFields d) {
+    ns_gather_cell<MAXF, VISC, F32, false>(m, d, Consts{}, 0, 0);
+}
+
+template <int MAXF, bool VISC, bool F32>
+__global__ void __launch_bounds__(TB) k_ns_gather_cf(DevMesh m, DevFields d, Consts k0, int par, int L0) {
+    ns_gather_cell<MAXF, VISC, F32, true>(m, d, k0, par, L0);
 }
 
 // Viscous face flux (UNsVisFlux::CalcFlux, codes/uns/src/UNsVisFlux.cpp:127-440)
@@ -880,12 +891,20 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
     if ((parts & 8u) && c->gather_variant == 1) {
         const int L0 = c->fuse_l0;
-        if (k.flux_fp32) {
-            if (visc) k_ns_gather_c<MAXF, true, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
-            else k_ns_gather_c<MAXF, false, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+        if (L0 > 0) {
+            if (k.flux_fp32) {
+                if (visc) k_ns_gather_cf<MAXF, true, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+                else k_ns_gather_cf<MAXF, false, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+            } else {
+                if (visc) k_ns_gather_cf<MAXF, true, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+                else k_ns_gather_cf<MAXF, false, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+            }
+        } else if (k.flux_fp32) {
+            if (visc) k_ns_gather_c<MAXF, true, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather_c<MAXF, false, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
         } else {
-            if (visc) k_ns_gather_c<MAXF, true, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
-            else k_ns_gather_c<MAXF, false, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_dt, L0);
+            if (visc) k_ns_gather_c<MAXF, true, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
+            else k_ns_gather_c<MAXF, false, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d);
         }
     } else if (parts & 8u) {
         if (k.flux_fp32) {

@@ -91,22 +91,29 @@ def apply_prediction(s: Solver, case: AirfoilCase, cfg, y: np.ndarray) -> None:
     s.sa_boundary()
 
 
-def apply_predictions_zones(zs, cases, net) -> None:
+def apply_predictions_zones(zs, cases, net, mach_gate=None) -> None:
     """The learned initial state of every zone of a ZoneSolver: the fields are
     read once, one network forward covers all zones' cells, the state is
-    written once and the boundary tasks fill the ghosts of all zones."""
+    written once and the boundary tasks fill the ghosts of all zones.  Cases
+    above `mach_gate` (if given) keep the freestream start the plain solver
+    uses (transonic cases can reach a different steady state from another
+    start)."""
     xcc, dist = zs.get("XCC"), zs.get("DIST")
     q, nu = zs.get("Q"), zs.get("TURBQ")
     feats, spans = [], []
     for z, case in enumerate(cases):
+        if mach_gate is not None and case.mach > mach_gate:
+            continue
         cs, _ = zs.zone_slice(z)
         feats.append(cell_features(_Fields(xcc[:, cs], dist[cs]), case))
-        spans.append(cs)
+        spans.append((z, cs))
+    if not feats:
+        return
     y = net.forward(np.concatenate(feats))
     o = 0
-    for z, (case, cs) in enumerate(zip(cases, spans)):
+    for z, cs in spans:
         n = cs.stop - cs.start
-        _write_prediction(q[:, cs], nu[cs], case, zs.zone_cfgs[z], y[o:o + n])
+        _write_prediction(q[:, cs], nu[cs], cases[z], zs.zone_cfgs[z], y[o:o + n])
         o += n
     zs.set("Q", q)
     zs.set("TURBQ", nu)
@@ -136,30 +143,60 @@ class ZoneView:
         return self.zs.get_zone(name, self.z)
 
 
-def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None):
-    """All cases as zones of one context advance together; the run ends when
-    every zone's density residual norm is below tol * refs[z] (zones that
-    converge earlier keep iterating with the rest).  Returns (wall seconds,
-    step at which each zone first met its tolerance)."""
+def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None, rebuild=None,
+                             compact=0.5):
+    """All cases as zones of one context advance together until every
+    zone's density residual norm is below tol * refs[case].  With `rebuild`
+    (case ids -> a new ZoneSolver for those cases), converged cases leave:
+    once the unconverged cases are at most `compact` of the context's zones,
+    their state moves into a new context holding only them (ghosts and
+    viscosities recomputed by the boundary tasks), so converged cases stop
+    costing work.  Returns (wall seconds including any rebuilds, step at which
+    each case first met its tolerance, forces [CL, CD, CDp, CDf] of each case
+    at that step)."""
     n = zs.n_zones
     steps = [0] * n
+    aero = [None] * n
     zs.sync()
     t0 = time.perf_counter()
     ramp = zs.cfg.ncfl
     it = 1
+    cur, ids = zs, list(range(n))
     while True:
         nstep = chunk + (ramp - it if it < ramp else 0)
-        zs.iterate(it, nstep, with_norms=False)
+        cur.iterate(it, nstep, with_norms=False)
         it += nstep
-        nr = zs.zone_norms()[:, 0]
-        for z in range(n):
-            if not steps[z] and (nr[z] <= tol * refs[z] or it > max_steps):
-                steps[z] = it - 1
+        nr = cur.zone_norms()[:, 0]
+        za = None
+        for z, cid in enumerate(ids):
+            if not steps[cid] and (nr[z] <= tol * refs[cid] or it > max_steps):
+                steps[cid] = it - 1
+                za = cur.zone_aero() if za is None else za
+                aero[cid] = za[z].tolist()
                 if progress:
-                    progress(z, steps[z])
-        if all(steps):
+                    progress(cid, steps[cid])
+        live = [cid for cid in ids if not steps[cid]]
+        if not live:
             break
-    return time.perf_counter() - t0, steps
+        if rebuild is not None and len(live) < len(ids) and len(live) <= compact * len(ids):
+            new = rebuild(live)
+            q, nu = cur.get("Q"), cur.get("TURBQ")
+            qn, nun = new.get("Q"), new.get("TURBQ")
+            for j, cid in enumerate(live):
+                z = ids.index(cid)
+                (cs, gs), (cn, gn) = cur.zone_slice(z), new.zone_slice(j)
+                qn[:, cn], qn[:, gn] = q[:, cs], q[:, gs]
+                nun[cn], nun[gn] = nu[cs], nu[gs]
+            new.set("Q", qn)
+            new.set("TURBQ", nun)
+            new.ns_boundary()
+            new.sa_boundary()
+            if cur is not zs:
+                cur.close()
+            cur, ids = new, live
+    if cur is not zs:
+        cur.close()
+    return time.perf_counter() - t0, steps, aero
 
 
 def solve_to_tolerance(solvers, refs, tol=1e-8, chunk=500, max_steps=150_000, batched=True, steps_per_graph=10,
