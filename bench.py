@@ -70,6 +70,16 @@ def kernel_bytes(C, B, F, dim, flux_bytes=8):
                 + 8 * nq * T + 16 * T + 2 * flux_bytes * nq * F,
         # face lists, both face fluxes -> residual
         "gather": idx + 2 * flux_bytes * nq * F + 8 * nq * C,
+        # SA: nu_tilde, face lists (face + neighbour), weights, volume, wall-face
+        # types -> boundary ghosts of nu_tilde and its gradient
+        "sa_grad": 8 * T + idx + 4 * nnz + gg + 8 * C + 4 * B + 8 * B + 8 * D * C,
+        # lc/rc, normal/area, STD vector, rho + velocity, nu_tilde, mu_l, volume,
+        # grad nu_tilde, wall-face types -> 5 face terms
+        "sa_face": 8 * F + 8 * (D + 1) * F + 32 * F + 8 * (D + 1) * T + 8 * 3 * T + 8 * D * C + 4 * B
+                   + 40 * F,
+        # face lists, 3 face terms, velocity gradient, rho, nu_tilde, mu_l, wall
+        # distance, grad nu_tilde, volume, time step -> diagonal and residual
+        "sa_cell": idx + 24 * F + 8 * D * D * C + 8 * 6 * C + 8 * D * C + 16 * C,
     }
 
 
@@ -572,9 +582,11 @@ def residual_bench(s, grid, args, flux_bytes=8):
     names = {"grad": ("k_ns_grad", "k_ns_grad_q")[s.kernel_variant(0)],
              "limiter": ("k_ns_limiter", "k_ns_limiter_c")[s.kernel_variant(1)],
              "face": "k_ns_face",
-             "gather": ("k_ns_gather", "k_ns_gather_c")[s.kernel_variant(2)]}
+             "gather": ("k_ns_gather", "k_ns_gather_c")[s.kernel_variant(2)],
+             "sa_grad": "k_sa_grad", "sa_face": "k_sa_face", "sa_cell": "k_sa_cell"}
     prof = {key: s.time_kernel(key, iters=args.kernel_iters) for key in names}
     s.residual_ns()  # leave a consistent residual behind
+    s.residual_sa()
     s.sync()
     kernels = {}
     for key, kname in names.items():

@@ -204,7 +204,9 @@ int ofx_wall_data(ofx_ctx* ctx, double* out);
  * context's stream between two HIP events, after one warm-up launch, and
  * returns the average ms per launch.  which: 0 NS gradient + limiter min/max,
  * 1 limiter, 2 NS face fluxes (MUSCL + Roe/SLAU2 + viscous), 3 NS ordered
- * gather, 4 whole NS residual, 5 whole SA residual.  Every part is
+ * gather, 4 whole NS residual, 5 whole SA residual, 6 SA gradient (+ ghost
+ * values), 7 SA face terms, 8 SA cell terms (source + ordered gather).
+ * Every part is
  * idempotent on the current state, so the fields are left as one
  * ofx_residual_ns / ofx_residual_sa call leaves them (which 0 also leaves the
  * limiter stale until the next limiter launch). */

@@ -443,8 +443,8 @@ int ofx_aero(ofx_ctx* c, double* out4) {
 
 int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
     CHECK_READY(c);
-    if (!ms_out || iters < 1 || which < 0 || which > 5) {
-        set_error("time_kernel: which must be 0..5 and iters >= 1");
+    if (!ms_out || iters < 1 || which < 0 || which > 8) {
+        set_error("time_kernel: which must be 0..8 and iters >= 1");
         return OFX_EINVAL;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -454,7 +454,8 @@ int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
     auto once = [&]() -> int {
         if (which <= 3) return ns_residual_parts(c, 1u << which);
         if (which == 4) return ns_residual(c);
-        return sa_residual(c);
+        if (which == 5) return sa_residual(c);
+        return sa_residual_parts(c, 1u << (which - 6));
     };
     rc = once();  // warm-up launch
     if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = OFX_EHIP;

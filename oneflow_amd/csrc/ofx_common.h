@@ -214,19 +214,25 @@ __device__ __forceinline__ int xcd_block(int nblk) { return xcd_remap((int)block
 // (continuing across segments), so every XCD receives exactly the blocks
 // the hardware dispatches to it (physical block b on XCD b % 8).  Bijective on
 // [0, nblk) when nblk = ceil(start[nseg] / per); nseg = 1 is xcd_remap.
-__host__ __device__ __forceinline__ int xcd_remap_seg(int b, int nblk, int per, const int* start, int nseg) {
+// CEIL(v) = ceil(v / per) for the item counts v >= 0 (a division, or a shift
+// when per is a power of two known only at run time).
+template <class CEIL>
+__host__ __device__ __forceinline__ int xcd_remap_seg_by(int b, int nblk, CEIL ceil_div, const int* start, int nseg) {
     const int x = b & 7;
     int slot = b >> 3, rr = 0;
 #pragma unroll
     for (int s = 0; s < MAXSEG; ++s) {
         if (s < nseg) {
-            int b0 = (start[s] + per - 1) / per;
-            int b1 = (s + 1 == nseg) ? nblk : (start[s + 1] + per - 1) / per;
+            int b0 = ceil_div(start[s]);
+            int b1 = (s + 1 == nseg) ? nblk : ceil_div(start[s + 1]);
             int n = b1 - b0, q = n >> 3, r = n & 7;
-            int cnt = q + ((((x - rr) & 7) < r) ? 1 : 0);
+            // XCDs that receive one of the r remainder blocks: the cyclic
+            // window [rr, rr + r) of 0..7, as a bit mask
+            unsigned w = ((1u << r) - 1u) << rr;
+            unsigned mask = (w | (w >> 8)) & 0xFFu;
+            int cnt = q + (int)((mask >> x) & 1u);
             if (slot < cnt) {
-                int before = x * q;
-                for (int y = 0; y < x; ++y) before += (((y - rr) & 7) < r) ? 1 : 0;
+                int before = x * q + __builtin_popcount(mask & ((1u << x) - 1u));
                 return b0 + before + slot;
             }
             slot -= cnt;
@@ -234,6 +240,14 @@ __host__ __device__ __forceinline__ int xcd_remap_seg(int b, int nblk, int per, 
         }
     }
     return b;  // not reached for a consistent launch
+}
+__host__ __device__ __forceinline__ int xcd_remap_seg(int b, int nblk, int per, const int* start, int nseg) {
+    return xcd_remap_seg_by(b, nblk, [per](int v) { return (v + per - 1) / per; }, start, nseg);
+}
+// The same for per = 1 << lg.
+__host__ __device__ __forceinline__ int xcd_remap_seg_lg(int b, int nblk, int lg, const int* start, int nseg) {
+    return xcd_remap_seg_by(
+        b, nblk, [lg](int v) { return (int)(((unsigned)v + (1u << lg) - 1u) >> lg); }, start, nseg);
 }
 // Entries 0..MAXF-1 of cell i's face list (cell-major copy; entries past the
 // list hold 0 / i and must be masked with the cell's count).
@@ -251,9 +265,18 @@ __device__ __forceinline__ void cell_faces(const DevMesh& m, int i, int* ent, in
 __device__ __forceinline__ int cell_block(const DevMesh& m, int per) {
     return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segc, m.nsegc);
 }
+// per: a power of two (the face tiles are 64 .. 512)
 __device__ __forceinline__ int face_block(const DevMesh& m, int per) {
-    return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segf, m.nsegf);
+    return xcd_remap_seg_lg((int)blockIdx.x, (int)gridDim.x, __builtin_ctz((unsigned)per), m.segf, m.nsegf);
 }
+
+// x for a live entry, -0.0 otherwise: sum + live_or_nz(live, x) is bitwise
+// `live ? sum + x : sum` (x + -0.0 == x for every x, signed zeros included),
+// but keeps the load of x unconditional, so the compiler cannot sink it into
+// a divergent branch and wait on it there.
+__device__ __forceinline__ double live_or_nz(bool live, double x) { return live ? x : -0.0; }
+// The same for a subtraction: acc - live_or_pz(live, x) is `live ? acc - x : acc`.
+__device__ __forceinline__ double live_or_pz(bool live, double x) { return live ? x : 0.0; }
 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
@@ -320,6 +343,46 @@ __device__ __forceinline__ double slot_result(const unsigned long long* buf, int
     double out = r[0];
     __syncthreads();
     return out;
+}
+
+// A double from lane l of the wave (l wave-uniform), whatever the exec mask.
+__device__ __forceinline__ double readlane_d(double x, int l) {
+    long long b = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_readlane((int)b, l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Consumer side without LDS or barriers: every wave reduces the NSLOT slots of
+// quantities q0 .. q0+NQ-1 itself, so that no block-wide barrier holds the
+// wave's other loads back.  G lanes share a quantity (lane l takes quantity
+// min(l / G, NQ - 1) and NSLOT / G consecutive slots of it), G - 1 shuffles
+// combine them and out[t] is read from lane t*G.  Exact (a min or max, in any
+// order): the value slot_result returns.  Every lane of the wave must be
+// active, so the callers keep all lanes to this point (lanes past the last
+// item work on the last item and store nothing).
+template <int NQ, bool IS_MAX>
+__device__ __forceinline__ void wave_slot_result(const unsigned long long* buf, int par, int nq_total, int q0,
+                                                 double* out) {
+    constexpr int G = NQ == 1 ? 64 : NQ == 2 ? 32 : NQ <= 4 ? 16 : NQ <= 8 ? 8 : NQ <= 16 ? 4 : 2;
+    constexpr int PER = NSLOT / G;
+    static_assert(NQ <= 32 && NSLOT == 64, "slot layout");
+    const int lane = (int)(threadIdx.x & 63);
+    const int q = (lane / G < NQ) ? lane / G : NQ - 1, part = lane % G;
+    const unsigned long long* src = buf + ((size_t)par * nq_total + q0 + q) * NSLOT + part * PER;
+    double x = __longlong_as_double((long long)src[0]);
+#pragma unroll
+    for (int t = 1; t < PER; ++t) {
+        double y = __longlong_as_double((long long)src[t]);
+        x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+    }
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        double y = __shfl_xor(x, o);
+        x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+    }
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) out[t] = readlane_d(x, t * G);
 }
 
 // Lanes-per-cell helpers: a cell's faces are spread over LPC consecutive lanes
@@ -455,6 +518,7 @@ int ns_residual(ofx_ctx* c);
 int ns_residual_parts(ofx_ctx* c, unsigned parts);
 int step_launch(ofx_ctx* c, int iter);
 int sa_residual(ofx_ctx* c);
+int sa_residual_parts(ofx_ctx* c, unsigned parts);  // bits: gradient, face terms, cell terms
 int ns_lusgs(ofx_ctx* c, int lo_first = 0);
 int fusable_l0(const ofx_ctx* c);
 int sa_lusgs(ofx_ctx* c, int lo_first = 0);

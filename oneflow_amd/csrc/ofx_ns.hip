@@ -128,7 +128,7 @@ int ns_boundary(ofx_ctx* c) {
 // per cell, the face spectral radii are summed in ascending face order.
 template <int DIM, int MAXF>
 __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k0, int par) {
-    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, TB) * TB + threadIdx.x;
     const int T = m.T;
     const int zone = (i < m.C) ? cell_zone(m, i) : -1;
     const Consts k = zoned(k0, m, zone);
@@ -288,7 +288,7 @@ int ns_timestep(ofx_ctx* c, int iter) {
 // the z components and w, which are identically zero in 2-D.
 template <int DIM, int MAXF, bool MINMAX>
 __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k0, int par) {
-    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, TB) * TB + threadIdx.x;
     const int T = m.T;
     const int zone = (i < m.C) ? cell_zone(m, i) : -1;
     const Consts k = zoned(k0, m, zone);
@@ -502,77 +502,75 @@ __global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Const
 // minimum is exact in any order).
 template <int DIM, int MAXF, int LIM>
 __global__ void __launch_bounds__(TB) k_ns_limiter_c(DevMesh m, DevFields d, Consts k, int par) {
-    __shared__ double sext[10];
-    {
-        const unsigned long long* mine = d.slots + (size_t)par * m.nz * RQ_N * NSLOT;
-        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        double x[3];
-        for (int r = 0; r < 3; ++r) {
-            int q = w + 4 * r;
-            x[r] = (q < 10) ? __longlong_as_double((long long)mine[(RQ_QMIN + q) * NSLOT + lane]) : 0.0;
-        }
-        for (int r = 0; r < 3; ++r) {
-            int q = w + 4 * r;
-            double y = x[r];
-            for (int o = 32; o > 0; o >>= 1) { double z = __shfl_xor(y, o); y = (y > z) ? y : z; }
-            if (q < 10 && lane == 0) sext[q] = y;
-        }
-        __syncthreads();
-    }
     const int T = m.T;
-    const int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
+    // Every lane stays active to the end (the extrema are reduced per wave, with
+    // shuffles): lanes past the last cell evaluate the last cell and store nothing.
+    const int i0 = cell_block(m, TB) * TB + threadIdx.x;
+    const bool live = i0 < m.C;
+    const int i = live ? i0 : m.C - 1;
+    double ext[10];
+    if (LIM == 2 && m.nz == 1) wave_slot_result<10, true>(d.slots, par, m.nz * RQ_N, RQ_QMIN, ext);
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
+    // all entries are loaded (padding names face 0) and masked in the minimum
     double dx[MAXF], dy[MAXF], dz[MAXF], dot[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        if (j < n) {
-            int e = ent[j], f = e >= 0 ? e : ~e;
-            dvec4 rc4 = m.frec[f];
-            dx[j] = e >= 0 ? rc4.x : rc4.z;
-            dy[j] = e >= 0 ? rc4.y : rc4.w;
-            dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
-            if (LIM == 1) {
-                double ds = sqrt(sqr3(dx[j], dy[j], dz[j]));
-                double nz = (DIM == 3) ? m.zfn[f] : 0.0;
-                dot[j] = (m.xfn[f] * dx[j] + m.yfn[f] * dy[j] + nz * dz[j]) / (ds + SMALL);
-                if (e < 0) dot[j] = -dot[j];
-            }
+        int e = ent[j], f = e >= 0 ? e : ~e;
+        dvec4 rc4 = m.frec[f];
+        dx[j] = e >= 0 ? rc4.x : rc4.z;
+        dy[j] = e >= 0 ? rc4.y : rc4.w;
+        dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
+        if (LIM == 1) {
+            double ds = sqrt(sqr3(dx[j], dy[j], dz[j]));
+            double nz = (DIM == 3) ? m.zfn[f] : 0.0;
+            dot[j] = (m.xfn[f] * dx[j] + m.yfn[f] * dy[j] + nz * dz[j]) / (ds + SMALL);
+            if (e < 0) dot[j] = -dot[j];
         }
     }
+    double gxs[5], gys[5], gzs[5], mns[5], mxs[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        const bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
+        gxs[s] = skip ? 0.0 : d.grad[(0 * 6 + s) * T + i];
+        gys[s] = skip ? 0.0 : d.grad[(1 * 6 + s) * T + i];
+        gzs[s] = (DIM == 3 && !skip) ? d.grad[(2 * 6 + s) * T + i] : 0.0;
+        mns[s] = skip ? 0.0 : d.minmax[s * m.C + i];
+        mxs[s] = skip ? 0.0 : d.minmax[(5 + s) * m.C + i];
+    }
+    double lims[5];
+#pragma unroll
     for (int s = 0; s < 5; ++s) {
         double lim = 1.0;
-        bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
+        const bool skip = (LIM == 2 && DIM == 2 && s == 3);
         if (!skip) {
-            double gx = d.grad[(0 * 6 + s) * T + i];
-            double gy = d.grad[(1 * 6 + s) * T + i];
-            double gz = (DIM == 3) ? d.grad[(2 * 6 + s) * T + i] : 0.0;
-            double mn = d.minmax[s * m.C + i];
-            double mx = d.minmax[(5 + s) * m.C + i];
+            double gx = gxs[s], gy = gys[s], gz = gzs[s], mn = mns[s], mx = mxs[s];
             double eps1 = 0.0;
             if (LIM == 2) {
-                double lo = sext[s], hi = sext[5 + s];
+                double lo, hi;
                 if (m.nz > 1) {
                     const double* zr = d.zred + (size_t)m.zcell[i] * RQ_N;
                     lo = zr[RQ_QMIN + s];
                     hi = zr[RQ_QMAX + s];
+                } else {
+                    lo = ext[s];
+                    hi = ext[5 + s];
                 }
                 double eps = k.vencat_coef * dmax(hi - (-lo), 1.0);
                 eps1 = eps * eps + SMALL;
             }
 #pragma unroll
             for (int j = 0; j < MAXF; ++j) {
-                if (j < n) {
-                    double dq = gx * dx[j] + gy * dy[j] + gz * dz[j];
-                    double phi = (LIM == 2) ? vencat_fn(dq, mn, mx, eps1) : barth_fn(dq, mn, mx, dot[j]);
-                    lim = dmin(lim, phi);
-                }
+                double dq = gx * dx[j] + gy * dy[j] + gz * dz[j];
+                double phi = (LIM == 2) ? vencat_fn(dq, mn, mx, eps1) : barth_fn(dq, mn, mx, dot[j]);
+                lim = (j < n) ? dmin(lim, phi) : lim;
             }
         }
-        d.limiter[s * T + i] = dmin(1.0, lim);
+        lims[s] = dmin(1.0, lim);
     }
+    if (live)
+        for (int s = 0; s < 5; ++s) d.limiter[s * T + i] = lims[s];
 }
 
 // The residual gather with one thread per cell (face loads of the cell all
@@ -585,9 +583,13 @@ template <int MAXF, bool VISC, bool F32, bool FUSE>
 __device__ __forceinline__ void ns_gather_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int par,
                                                int L0) {
     const int T = m.T, F = m.F;
-    const double mindt1 = FUSE ? slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN) : 0.0;
-    const int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
+    // Every lane stays active to the end (the fused path reduces the time-step
+    // slots per wave): lanes past the last cell evaluate it and store nothing.
+    const int i0 = cell_block(m, TB) * TB + threadIdx.x;
+    const bool live = i0 < m.C;
+    const int i = live ? i0 : m.C - 1;
+    double mindt1 = 0.0;
+    if (FUSE) wave_slot_result<1, false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN, &mindt1);
     double rad = 1.0;
     if (FUSE && i < L0) {
         const Consts k = zoned(k0, m, cell_zone(m, i));
@@ -598,31 +600,37 @@ __device__ __forceinline__ void ns_gather_cell(const DevMesh& m, const DevFields
     cell_faces<MAXF>(m, i, ent, nb);
     const float* inv32 = reinterpret_cast<const float*>(d.invflux);
     const float* vis32 = reinterpret_cast<const float*>(d.visflux);
+    // Every entry is loaded, padding included (it names face 0), so that no
+    // load sits in a divergent branch, and all loads come before the first
+    // store (which the compiler cannot move them across): the first n
+    // entries are summed.
+    double fi[5][MAXF], fv[5][MAXF];
+#pragma unroll
     for (int s = 0; s < 5; ++s) {
-        double fi[MAXF], fv[MAXF];
 #pragma unroll
         for (int j = 0; j < MAXF; ++j) {
-            if (j < n) {
-                int e = ent[j], f = e >= 0 ? e : ~e;
-                double x = F32 ? (double)inv32[s * F + f] : d.invflux[s * F + f];
-                fi[j] = (e >= 0) ? -x : x;
-                if (VISC) {
-                    double y = F32 ? (double)vis32[s * F + f] : d.visflux[s * F + f];
-                    fv[j] = (e >= 0) ? -y : y;
-                }
+            int e = ent[j], f = e >= 0 ? e : ~e;
+            double x = F32 ? (double)inv32[s * F + f] : d.invflux[s * F + f];
+            fi[s][j] = (e >= 0) ? -x : x;
+            if (VISC) {
+                double y = F32 ? (double)vis32[s * F + f] : d.visflux[s * F + f];
+                fv[s][j] = (e >= 0) ? -y : y;
             }
         }
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
         double sum = 0.0;
 #pragma unroll
         for (int j = 0; j < MAXF; ++j)
-            if (j < n) sum = sum + fi[j];
+            sum = sum + live_or_nz(j < n, fi[s][j]);
         if (VISC) {
 #pragma unroll
             for (int j = 0; j < MAXF; ++j)
-                if (j < n) sum = sum + fv[j];
+                sum = sum + live_or_nz(j < n, fv[s][j]);
         }
-        d.res[s * T + i] = sum;
-        if (FUSE && i < L0) { double t = sum - 0.0; t /= rad; d.dq[s * T + i] = t; }
+        if (live) d.res[s * T + i] = sum;
+        if (FUSE && live && i < L0) { double t = sum - 0.0; t /= rad; d.dq[s * T + i] = t; }
     }
 }
 
@@ -709,8 +717,8 @@ __device__ __forceinline__ void ns_face_visc(const DevMesh& m, const DevFields& 
 // the FP64 reconstruction and both face fluxes are stored as FP32 (the
 // gather accumulates them in FP64).
 template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32>
-__global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k0) {
-    int f = face_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k0, int tb) {
+    int f = face_block(m, tb) * tb + threadIdx.x;
     if (f >= m.F) return;
     const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T;
@@ -834,11 +842,11 @@ static void launch_face(ofx_ctx* c, bool visc) {
     const int tb = c->face_tb;  // face tile, chosen by ofx_autotune
     int nb = (c->m.F + tb - 1) / tb;
     if (c->k.flux_fp32) {
-        if (visc) k_ns_face<DIM, LIM, SLAU2, true, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
-        else k_ns_face<DIM, LIM, SLAU2, false, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
+        if (visc) k_ns_face<DIM, LIM, SLAU2, true, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k, tb);
+        else k_ns_face<DIM, LIM, SLAU2, false, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k, tb);
     } else {
-        if (visc) k_ns_face<DIM, LIM, SLAU2, true, false><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
-        else k_ns_face<DIM, LIM, SLAU2, false, false><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k);
+        if (visc) k_ns_face<DIM, LIM, SLAU2, true, false><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k, tb);
+        else k_ns_face<DIM, LIM, SLAU2, false, false><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k, tb);
     }
 }
 

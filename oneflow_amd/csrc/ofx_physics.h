@@ -207,6 +207,37 @@ __device__ __forceinline__ void vis_geom(const DevMesh& m, int f, int l, int r, 
     }
 }
 
+// The raw records of face f for face-gradient method `method` (run-time,
+// wave-uniform): rec[0] the STD record (method 0), rec[0..4] the TEST
+// records (method 2), zeros otherwise.  For kernels that issue these loads
+// early, before any arithmetic; vis_geom_rec unpacks them as vis_geom_pre
+// (below) loads them.
+__device__ __forceinline__ void vis_rec_load(const DevMesh& m, int method, int f, dvec4* rec) {
+    for (int t = 0; t < 5; ++t) rec[t] = dvec4{0.0, 0.0, 0.0, 0.0};
+    if (method == 0) {
+        rec[0] = m.fv0[f];
+    } else if (method == 2) {
+        const dvec4* r = m.fv2 + 5 * (size_t)f;
+        for (int t = 0; t < 5; ++t) rec[t] = r[t];
+    }
+}
+template <int METHOD>
+__device__ __forceinline__ void vis_geom_rec(double xfn, double yfn, double zfn, bool bc_nonint, const dvec4* rec,
+                                             VisGeom& G) {
+    G.xfn = xfn; G.yfn = yfn; G.zfn = zfn;
+    G.bc_nonint = bc_nonint;
+    if (METHOD == 0) {
+        dvec4 a = rec[0];
+        G.dx = a.x; G.dy = a.y; G.dz = a.z; G.ods = a.w;
+    } else if (METHOD == 2) {
+        dvec4 a = rec[0], b = rec[1], c = rec[2], e = rec[3], w = rec[4];
+        G.fw1 = w.x; G.fw2 = w.y;
+        G.d1 = a.x; G.d2 = a.y; G.angle1 = a.z; G.angle2 = a.w;
+        G.dxnl = b.x; G.dynl = b.y; G.dznl = b.z; G.dxnr = b.w;
+        G.dynr = c.x; G.dznr = c.y; G.t1[0] = c.z; G.t1[1] = c.w;
+        G.t1[2] = e.x; G.t2[0] = e.y; G.t2[1] = e.z; G.t2[2] = e.w;
+    }
+}
 // The same geometry read from the per-face records the mesh upload stores
 // (DevMesh::fv0, fv2), which vis_geom itself computed.  METHOD is the
 // caller's compile-time choice, so no member is merged across branches.

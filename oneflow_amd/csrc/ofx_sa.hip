@@ -80,7 +80,7 @@ int sa_boundary(ofx_ctx* c) {
 // boundary faces (they depend only on the owner) and uses them.
 template <int DIM, int MAXF>
 __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts k0) {
-    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+    int i = cell_block(m, TBS) * TBS + threadIdx.x;
     if (i >= m.C) return;
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const int T = m.T;
@@ -88,33 +88,36 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
     double nui = d.turbq[i];
+    // All entries are loaded (padding: face 0, the cell itself) and masked in
+    // the sums, so that no load sits in a divergent branch; only the ghosts
+    // of the cell's own boundary faces are evaluated and written.
     dvec4 gw[MAXF];
     double gz[MAXF], vn[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        if (j < n) {
-            int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-            gw[j] = m.fgg[f];
-            gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
-            if (f < m.B) {
-                vn[j] = sa_bc_value(m, d, k, f, i, nui);
-                d.turbq[c] = vn[j];
-            } else {
-                vn[j] = d.turbq[c];
-            }
+        int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
+        gw[j] = m.fgg[f];
+        gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
+        vn[j] = d.turbq[c];
+    }
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) {
+        int e = ent[j], f = e >= 0 ? e : ~e;
+        if (j < n && f < m.B) {
+            vn[j] = sa_bc_value(m, d, k, f, i, nui);
+            d.turbq[nb[j]] = vn[j];
         }
     }
     double ax = 0, ay = 0, az = 0;
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        if (j < n) {
-            bool own = ent[j] >= 0;
-            double cl = gw[j].x, cr = gw[j].y;
-            double value = own ? (cl * nui + cr * vn[j]) : (cl * vn[j] + cr * nui);
-            double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
-            if (own) { ax += fnxa * value; ay += fnya * value; az += fnza * value; }
-            else { ax -= fnxa * value; ay -= fnya * value; az -= fnza * value; }
-        }
+        const bool lv = j < n;
+        bool own = ent[j] >= 0;
+        double cl = gw[j].x, cr = gw[j].y;
+        double value = own ? (cl * nui + cr * vn[j]) : (cl * vn[j] + cr * nui);
+        double tx = gw[j].z * value, ty = gw[j].w * value, tz = gz[j] * value;
+        if (own) { ax = ax + live_or_nz(lv, tx); ay = ay + live_or_nz(lv, ty); az = az + live_or_nz(lv, tz); }
+        else { ax = ax - live_or_pz(lv, tx); ay = ay - live_or_pz(lv, ty); az = az - live_or_pz(lv, tz); }
     }
     double ovol = 1.0 / m.vol[i];
     d.tgrad[i] = ax * ovol;
@@ -125,10 +128,11 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
 // Normal derivative of nu_tilde at face f by face-gradient method VM
 // (UTurbVisFlux::CalcFaceVisFlux1Equ via VisGrad, codes/uturb/src/UTurbVisFlux.cpp:296-369).
 template <int VM>
-__device__ __forceinline__ double sa_face_dqdn(const DevMesh& m, int f, double nl, double nr, const double* g1,
+__device__ __forceinline__ double sa_face_dqdn(const DevMesh& m, int f, double xfn, double yfn, double zfn,
+                                               const dvec4* rec, double nl, double nr, const double* g1,
                                                const double* g2) {
     VisGeom G;
-    vis_geom_pre<VM>(m, f, G);
+    vis_geom_rec<VM>(xfn, yfn, zfn, f < m.B, rec, G);
     double qf, gf[3], dqdn;
     vis_grad<VM>(G, nl, nr, g1, g2, qf, gf, dqdn);
     return dqdn;
@@ -142,8 +146,8 @@ __device__ __forceinline__ double sa_face_dqdn(const DevMesh& m, int f, double n
 // (UTurbVisFlux::CalcFaceVisFlux1Equ, codes/uturb/src/UTurbVisFlux.cpp:247-369).
 // saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
 template <int DIM>
-__global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k0) {
-    int f = face_block(m, blockDim.x) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k0, int tb) {
+    int f = face_block(m, tb) * tb + threadIdx.x;
     if (f >= m.F) return;
     const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T, F = m.F;
@@ -157,6 +161,19 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     double ur = d.q[T + r], vr = d.q[2 * T + r], wr = (DIM == 3) ? d.q[3 * T + r] : 0.0;
     double nl = d.turbq[l], nr = d.turbq[r];
     double visll = d.visl[l], vislr = d.visl[r];
+    // The diffusion term's loads: in 2-D issued here with the others, before
+    // any arithmetic (measured faster on the 2-D grid); in 3-D where they are
+    // used (the early issue measured slower on the 3-D grid).
+    const double gzf = m.zfn[f];
+    dvec4 rec[5];
+    double g1[3], g2[3];
+    auto diffusion_loads = [&]() {
+        vis_rec_load(m, k0.iturb_visflux, f, rec);
+        g1[0] = d.tgrad[l]; g1[1] = d.tgrad[T + l]; g1[2] = (DIM == 3) ? d.tgrad[2 * T + l] : 0.0;
+        g2[0] = d.tgrad[rg]; g2[1] = d.tgrad[T + rg]; g2[2] = (DIM == 3) ? d.tgrad[2 * T + rg] : 0.0;
+    };
+    if (DIM == 2) diffusion_loads();
+    const int bt = bnd ? m.bct[f] : 0;
     // spectrum
     double vnl = xfn * ul + yfn * vl + zfn * wl - vfn;
     double vnr = xfn * ur + yfn * vr + zfn * wr - vfn;
@@ -182,7 +199,7 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     if (bnd) {
         double tn = 0.5 * (nl + nr);
         cnl = tn; cnr = tn;
-        if (m.bct[f] == BC_SOLID_SURFACE && k.vismodel != 0) {
+        if (bt == BC_SOLID_SURFACE && k.vismodel != 0) {
             cul = cur = 0.0; cvl = cvr = 0.0; cwl = cwr = 0.0;
         } else {
             double tu = 0.5 * (ul + ur), tv = 0.5 * (vl + vr), tw = 0.5 * (wl + wr);
@@ -195,12 +212,11 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     double vnr_n = 0.5 * (cvnr - dabs(cvnr));
     double conv = farea * (vnl_p * 1.0 * cnl + vnr_n * 1.0 * cnr);
     // diffusion (face-gradient method uniform: one dispatch, separate instantiations)
-    double g1[3] = {d.tgrad[l], d.tgrad[T + l], (DIM == 3) ? d.tgrad[2 * T + l] : 0.0};
-    double g2[3] = {d.tgrad[rg], d.tgrad[T + rg], (DIM == 3) ? d.tgrad[2 * T + rg] : 0.0};
+    if (DIM == 3) diffusion_loads();
     double dqdn;
-    if (k.iturb_visflux == 0) dqdn = sa_face_dqdn<0>(m, f, nl, nr, g1, g2);
-    else if (k.iturb_visflux == 1) dqdn = sa_face_dqdn<1>(m, f, nl, nr, g1, g2);
-    else dqdn = sa_face_dqdn<2>(m, f, nl, nr, g1, g2);
+    if (k0.iturb_visflux == 0) dqdn = sa_face_dqdn<0>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
+    else if (k0.iturb_visflux == 1) dqdn = sa_face_dqdn<1>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
+    else dqdn = sa_face_dqdn<2>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
     double visl_nv = 0.5 * (visll * orl + vislr * orr);
     double vist_nv = 0.5 * (nl + nr);
     double comVis = (visl_nv + vist_nv) * k.osigma;
@@ -300,9 +316,13 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
 // (codes/turb/src/TurbRhs.cpp:57-71).
 template <int MAXF>
 __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts k0, int par, int L0) {
-    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
-    int i = cell_block(m, blockDim.x) * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
+    // Every lane stays active to the end (the time-step slots are reduced per
+    // wave): lanes past the last cell evaluate the last cell and store nothing.
+    const int i0 = cell_block(m, TBS) * TBS + threadIdx.x;
+    const bool live = i0 < m.C;
+    const int i = live ? i0 : m.C - 1;
+    double mindt;
+    wave_slot_result<1, false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN, &mindt);
     mindt = zone_mindt(m, d, i, mindt);
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const int F = m.F;
@@ -310,15 +330,14 @@ __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts 
     int ent[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) ent[j] = (j < m.ellw) ? m.ellf[(size_t)j * m.C + i] : 0;
+    // all entries loaded (padding names face 0), masked in the sums
     double rad[MAXF], cv[MAXF], df[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        if (j < n) {
-            int f = ent[j] >= 0 ? ent[j] : ~ent[j];
-            rad[j] = d.saface[f];
-            cv[j] = d.saface[3 * F + f];
-            df[j] = d.saface[4 * F + f];
-        }
+        int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+        rad[j] = d.saface[f];
+        cv[j] = d.saface[3 * F + f];
+        df[j] = d.saface[4 * F + f];
     }
     double vol = m.vol[i];
     double dt = d.timestep[i];
@@ -330,49 +349,58 @@ __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts 
     double imp = 0.0;
     imp += (1.0 / ts) * vol;
 #pragma unroll
-    for (int j = 0; j < MAXF; ++j)
-        if (j < n) imp += rad[j];
+    for (int j = 0; j < MAXF; ++j) imp = imp + live_or_nz(j < n, rad[j]);
     double res_sa, spec_sa;
     sa_source(m, d, k, i, res_sa, spec_sa);
     imp += spec_sa * vol;
     double res = 0.0;
     res += res_sa * vol;
 #pragma unroll
-    for (int j = 0; j < MAXF; ++j)
-        if (j < n) { if (ent[j] >= 0) res -= cv[j]; else res += cv[j]; }
+    for (int j = 0; j < MAXF; ++j) {
+        const bool lv = j < n;
+        res = (ent[j] >= 0) ? res - live_or_pz(lv, cv[j]) : res + live_or_nz(lv, cv[j]);
+    }
 #pragma unroll
-    for (int j = 0; j < MAXF; ++j)
-        if (j < n) { if (ent[j] >= 0) res -= df[j]; else res += df[j]; }
+    for (int j = 0; j < MAXF; ++j) {
+        const bool lv = j < n;
+        res = (ent[j] >= 0) ? res - live_or_pz(lv, df[j]) : res + live_or_nz(lv, df[j]);
+    }
+    if (!live) return;
     d.turbimpsr[i] = imp;
     d.turbres[i] = res;
     // cells [0, L0): the first level of the lower sweep, turbdq = (res - 0) / D
     if (i < L0) { double t = res - 0.0; t /= imp; d.turbdq[i] = t; }
 }
 
+// parts: bit 0 gradient (+ ghosts), bit 1 face terms, bit 2 cell terms.
 template <int DIM, int MAXF>
-static void launch_sa(ofx_ctx* c) {
+static void launch_sa(ofx_ctx* c, unsigned parts) {
     const DevMesh& m = c->m;
-    k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
-    k_sa_face<DIM><<<(m.F + c->saface_tb - 1) / c->saface_tb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k);
-    k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt, c->fuse_l0);
+    if (parts & 1u) k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+    if (parts & 2u)
+        k_sa_face<DIM><<<(m.F + c->saface_tb - 1) / c->saface_tb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k,
+                                                                                                c->saface_tb);
+    if (parts & 4u) k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt, c->fuse_l0);
 }
 
-int sa_residual(ofx_ctx* c) {
+int sa_residual_parts(ofx_ctx* c, unsigned parts) {
     OFX_TRY(ensure_dist(c));
     if (c->k.vismodel != 3) return 0;
     int mf = c->m.maxFacesPerCell;
     if (c->m.dim == 2) {
-        if (mf <= 4) launch_sa<2, 4>(c);
-        else if (mf <= 8) launch_sa<2, 8>(c);
+        if (mf <= 4) launch_sa<2, 4>(c, parts);
+        else if (mf <= 8) launch_sa<2, 8>(c, parts);
         else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
     } else {
-        if (mf <= 6) launch_sa<3, 6>(c);
-        else if (mf <= 12) launch_sa<3, 12>(c);
+        if (mf <= 6) launch_sa<3, 6>(c, parts);
+        else if (mf <= 12) launch_sa<3, 12>(c, parts);
         else { set_error("cells with more than 12 faces are not supported"); return OFX_EINVAL; }
     }
     OFX_HIP(hipGetLastError());
     return 0;
 }
+
+int sa_residual(ofx_ctx* c) { return sa_residual_parts(c, 7u); }
 
 // UTurbLusgs lower / upper cell (codes/uturb/src/UTurbLusgs.cpp:90-304,
 // codes/turb/src/TurbLusgs.cpp:123-254): matrix_l when the cell is the face's

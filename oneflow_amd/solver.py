@@ -194,7 +194,8 @@ class Solver:
         self._check(self._L.ofx_aero(self.h, _dp(a)))
         return a
 
-    KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5}
+    KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5,
+               "sa_grad": 6, "sa_face": 7, "sa_cell": 8}
 
     def wall_data(self):               # DUMP_PRESSURE_COEFF
         """[n_wall][5]: x, y, z of each wall face centre, -cp, cf (face order)."""

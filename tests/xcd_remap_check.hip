@@ -1,7 +1,8 @@
 // Host-side check of the XCD block remaps (ofx_common.h): for random launch
 // shapes and segment cuts, xcd_remap_seg must be a bijection on [0, nblk)
 // whose blocks of segment s stay inside s, and with one segment it must equal
-// xcd_remap.  Prints "ok <blocks>" or the first failure; runs no GPU code.
+// xcd_remap; for power-of-two block sizes the shift form xcd_remap_seg_lg
+// must agree with it block for block.  Prints "ok <blocks>" or the first failure; runs no GPU code.
 #include "../oneflow_amd/csrc/ofx_common.h"
 
 #include <algorithm>
@@ -31,6 +32,10 @@ int main() {
             if (lb < 0 || lb >= nblk) { std::printf("out of range: trial %d block %d -> %d\n", trial, b, lb); return 1; }
             ++seen[lb];
             if (nseg == 1 && lb != xcd_remap(b, nblk)) { std::printf("one segment differs: trial %d\n", trial); return 1; }
+            if ((per & (per - 1)) == 0 && lb != xcd_remap_seg_lg(b, nblk, __builtin_ctz((unsigned)per), st.data(), nseg)) {
+                std::printf("shift form differs: trial %d block %d\n", trial, b);
+                return 1;
+            }
         }
         for (int b = 0; b < nblk; ++b)
             if (seen[b] != 1) { std::printf("not a bijection: trial %d n %d per %d nseg %d\n", trial, n, per, nseg); return 1; }
