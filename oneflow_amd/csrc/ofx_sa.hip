@@ -277,9 +277,14 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
     double ostd = one / (stdv + SMALL);
     double r = rs * ostd;
     r = dmin(r, ten);
-    double r5 = pow(r, 5.0);
+    // The reference's pow(x, n) for the integer powers n = 5, 6, 2 is
+    // evaluated as products (within a few ulps of pow; pow itself is a long
+    // log / exp sequence here, a large share of this kernel's FP64 work).
+    double r2 = r * r;
+    double r5 = r2 * r2 * r;
     double gg = r + k.cw2 * (r5 * r - r);
-    double g6 = pow(gg, 6.0);
+    double gg2 = gg * gg;
+    double g6 = gg2 * gg2 * gg2;
     double or6 = one / six;
     double fw = gg * pow((one + k.cw36) / (g6 + k.cw36), or6);
     const double ft2 = 0.0;
@@ -292,7 +297,7 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
     srcTerm += diff;
     double p2 = fv1 / (xsi + SMALL);
     double p3 = one + xsi * fv1;
-    double dfv2dk = (three * k.cv13 * pow(p2, 2.0) - one) / pow(p3, 2.0);
+    double dfv2dk = (three * k.cv13 * (p2 * p2) - one) / (p3 * p3);
     double dsdnu = k.oreynolds * k.okarm2 * od2 * (fv2 + xsi * dfv2dk);
     double dfwdg = fw / (gg + SMALL) * (one - g6 / (g6 + k.cw36));
     double dgdr = one + k.cw2 * (six * r5 - one);
