@@ -108,6 +108,8 @@ def c1_grid(numbering="multicolor"):
                             m=0.0, p=0.4, t=0.12)
     if numbering == "multicolor":
         g = meshgen.renumber(g, meshgen.multicolor_order(g))
+    elif numbering == "blocked":
+        g = meshgen.renumber(g, meshgen.blocked_multicolor_order(g))
     return g
 
 
@@ -465,7 +467,8 @@ def main():
     ap.add_argument("--pre-iters", type=int, default=20, help="pseudo-time steps before timing")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--numbering", default="multicolor", choices=["natural", "multicolor"])
+    ap.add_argument("--numbering", default=None, choices=["natural", "multicolor", "blocked"],
+                    help="cell numbering (default multicolor)")
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "batch", "warmstart"],
@@ -505,6 +508,8 @@ def main():
     from oneflow_amd.solver import Solver
 
     cfg = c1_config()
+    if args.numbering is None:
+        args.numbering = "multicolor"
     grid = c1_grid(args.numbering)
     print(f"[bench] grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces}", file=sys.stderr, flush=True)
     s = Solver(grid, cfg, device=0)
@@ -611,11 +616,14 @@ def c2_config():
                       reynolds=1.2e7, centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
 
 
-def c2_grid(n_span=34):
+def c2_grid(n_span=34, numbering="multicolor"):
     from oneflow_amd import meshgen
     g = meshgen.swept_wing(n_surf=256, n_quad=25, ratio=1.2, reynolds=1.2e7, n_outer=40, radius=20.0,
                            n_span=n_span, sweep_deg=30.0, taper=0.5, aspect=4.0, t=0.12)
-    return meshgen.renumber(g, meshgen.multicolor_order(g))
+    if numbering == "natural":
+        return g
+    order = meshgen.blocked_multicolor_order(g) if numbering == "blocked" else meshgen.multicolor_order(g)
+    return meshgen.renumber(g, order)
 
 
 def c2_main(args):
@@ -626,7 +634,9 @@ def c2_main(args):
     from oneflow_amd.solver import Solver
     cfg = c2_config()
     t0 = time.perf_counter()
-    grid = c2_grid()
+    if args.numbering is None:
+        args.numbering = "multicolor"
+    grid = c2_grid(numbering=args.numbering)
     print(f"[bench] wing: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces} "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     s = Solver(grid, cfg, device=0)
@@ -650,7 +660,8 @@ def c2_main(args):
         "warmup": args.warmup, "ms_per_step": r64["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "swept wing NACA0012, sweep 30, taper 0.5, AR 4, prism layers + tets, "
-                               "Roe + SA + Venkat", "cells": grid.n_cells, "faces": grid.n_faces,
+                               "Roe + SA + Venkat", "numbering": args.numbering,
+                   "cells": grid.n_cells, "faces": grid.n_faces,
                    "nTCell": grid.n_tcell, "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree,
                    "reynolds": cfg.reynolds},
         "roofline": r64["roofline"], "step_roofline": r64["step_roofline"], "kernels": r64["kernels"],

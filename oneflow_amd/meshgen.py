@@ -129,6 +129,86 @@ def multicolor_order(g: Grid):
     return np.lexsort((np.arange(n), color))
 
 
+def _rounds_colouring(n, a, b, seed=1):
+    """Colouring of the graph (n vertices, edges a-b) in rounds: each round
+    colours the uncoloured vertices whose random priority beats every
+    uncoloured neighbour's, with the lowest colour no coloured neighbour has
+    (Jones-Plassmann).  Deterministic for a seed; vectorised."""
+    w = np.random.default_rng(seed).permutation(n)
+    col = np.full(n, -1, np.int64)
+    used = np.zeros(n, np.int64)  # bit c: some coloured neighbour has colour c
+    u = np.concatenate([a, b]).astype(np.int64)
+    v = np.concatenate([b, a]).astype(np.int64)
+    while True:
+        open_ = col < 0
+        if not open_.any():
+            return col
+        e = open_[u] & open_[v]
+        beaten = np.zeros(n, bool)
+        np.logical_or.at(beaten, u[e], w[v[e]] > w[u[e]])
+        pick = np.flatnonzero(open_ & ~beaten)
+        free = ~used[pick]
+        low = free & -free
+        col[pick] = np.round(np.log2(low.astype(np.float64))).astype(np.int64)
+        chosen = np.zeros(n, bool)
+        chosen[pick] = True
+        e2 = chosen[u]
+        np.bitwise_or.at(used, v[e2], np.left_shift(np.int64(1), col[u[e2]]))
+
+
+def cell_centroids(g: Grid):
+    """Approximate cell centres (mean of the cell's face centres, each face
+    centre the mean of its nodes): [n_cells][3].  For ordering, not metrics."""
+    cnt = g.face_node_count.astype(np.int64)
+    fid = np.repeat(np.arange(g.n_faces), cnt)
+    fn = g.face_nodes.astype(np.int64)
+    zs = g.z if g.dim == 3 else np.zeros_like(g.x)
+    fc = [np.bincount(fid, c[fn], g.n_faces) / cnt for c in (g.x, g.y, zs)]
+    nb = g.n_bfaces
+    owner = np.concatenate([g.lc, g.rc[nb:]]).astype(np.int64)
+    face = np.concatenate([np.arange(g.n_faces), np.arange(nb, g.n_faces)])
+    k = np.bincount(owner, minlength=g.n_cells)
+    return np.stack([np.bincount(owner, c[face], g.n_cells) / k for c in fc], 1)
+
+
+def _rcb(xyz, idx, nparts, out, base):
+    """Recursive coordinate bisection of cells idx into nparts parts of equal
+    size (split across the longest extent)."""
+    if nparts == 1:
+        out[idx] = base
+        return
+    ax = int(np.argmax(xyz[idx].max(0) - xyz[idx].min(0)))
+    h = nparts // 2
+    k = len(idx) * h // nparts
+    srt = idx[np.argsort(xyz[idx, ax], kind="stable")]
+    _rcb(xyz, srt[:k], h, out, base)
+    _rcb(xyz, srt[k:], nparts - h, out, base + h)
+
+
+def blocked_multicolor_order(g: Grid, part_cells: int = 20000, seed: int = 1):
+    """A colouring numbering that keeps neighbours close in memory.
+
+    The cells are cut into spatial parts of about part_cells cells (recursive
+    coordinate bisection).  The parts are coloured so that no two touching
+    parts share a colour, and the cells of each part are coloured along the
+    part's inner faces.  Cells are sorted by (part colour, part, cell colour,
+    old id).  A cell's lower neighbours then lie in its own part with a
+    smaller cell colour or in a part of smaller part colour, so the LU-SGS
+    dependency levels stay few (14 on the 2.5M-cell swept wing, against 6 for
+    a global colouring), while every kernel walks one part -- a few MB of
+    state -- at a time instead of sweeping the whole grid once per colour."""
+    n = g.n_cells
+    a, b = (np.asarray(t, np.int64) for t in cell_adjacency(g))
+    nparts = max(1, -(-n // part_cells))
+    part = np.empty(n, np.int64)
+    _rcb(cell_centroids(g), np.arange(n), nparts, part, 0)
+    cut = part[a] != part[b]
+    pe = np.unique(np.stack([part[a][cut], part[b][cut]], 1), axis=0)
+    pcol = _rounds_colouring(nparts, pe[:, 0], pe[:, 1], seed)
+    ccol = _rounds_colouring(n, a[~cut], b[~cut], seed)
+    return np.lexsort((np.arange(n), ccol, part, pcol[part]))
+
+
 def renumber(g: Grid, order) -> Grid:
     """Return a copy with cells renumbered: new cell k is old cell order[k]."""
     order = np.asarray(order)

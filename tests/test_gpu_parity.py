@@ -30,6 +30,8 @@ def synthetic(case):
     g = meshgen.naca_hybrid(n_surf=96, n_quad=8, ratio=1.3, reynolds=6.5e6, n_outer=12, radius=20.0, m=0.02, p=0.4, t=0.12)
     if case.endswith("mc"):
         g = meshgen.renumber(g, meshgen.multicolor_order(g))
+    elif case.endswith("blk"):  # blocked colouring: levels that are not runs of consecutive cells
+        g = meshgen.renumber(g, meshgen.blocked_multicolor_order(g, part_cells=400))
     cfg = CaseConfig(ischeme=1, ilim=2, vismodel=3, mach_ref=0.8, aoa_degree=1.25, reynolds=6.5e6,
                      centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
     return g, cfg
@@ -72,7 +74,7 @@ def sync(gpu, ora, names=STATE):
         gpu.set(n, ora.get(n))
 
 
-@pytest.mark.parametrize("case", CASES + ["syn_natural", "syn_mc"])
+@pytest.mark.parametrize("case", CASES + ["syn_natural", "syn_mc", "syn_blk"])
 def test_one_step_task_by_task(case):
     gpu, ora = make(case)
     for s in (gpu, ora):
@@ -163,7 +165,7 @@ def test_fifty_steps_against_goldens(case):
             assert harness_equal(x, y), (case, a, b)
 
 
-@pytest.mark.parametrize("case", ["syn_natural", "syn_mc"])
+@pytest.mark.parametrize("case", ["syn_natural", "syn_mc", "syn_blk"])
 def test_synthetic_twenty_steps(case):
     gpu, ora = make(case)
     gpu.init_freestream()
