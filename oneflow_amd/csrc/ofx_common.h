@@ -293,18 +293,50 @@ __device__ __forceinline__ double sqr3(double a, double b, double c) { return a 
 // this call has completed by stream order.
 constexpr int NSLOT = 64;
 
+// a where the mask bits are 0, b where they are 1 (m is all zeros or all ones).
+__device__ __forceinline__ double bit_blend(double a, double b, long long m) {
+    long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    return __longlong_as_double((x & ~m) | (y & m));
+}
+
 template <int NQ, bool IS_MAX>
 __device__ __forceinline__ void slot_reduce(const double* v, unsigned long long* buf, int par, int nq_total, int q0) {
     __shared__ double sm[NQ][16];
     const int nw = (int)(blockDim.x >> 6), w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-    for (int q = 0; q < NQ; ++q) {
-        double x = v[q];
-        for (int o = 32; o > 0; o >>= 1) {
-            double y = __shfl_xor(x, o);
-            x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
+    // Wave reduction as a reduce-scatter butterfly: the NQ values (padded to
+    // P, a power of two) halve at each exchange -- a lane keeps the half its
+    // offset bit selects and receives its partner's copy of that half -- so
+    // after log2(P) exchanges each lane holds one quantity, reduced over the
+    // lanes differing in those bits; plain exchanges over the remaining bits
+    // finish it: P - 1 + log2(64 / P) shuffles instead of 6 NQ.  The halves
+    // are picked by a bit blend under an opaque mask (a select here is turned
+    // into a dynamic index into the value array, a long select chain).
+    constexpr int P = NQ <= 1 ? 1 : NQ <= 2 ? 2 : NQ <= 4 ? 4 : NQ <= 8 ? 8 : 16;
+    static_assert(NQ <= 16, "at most 16 quantities");
+    const double ident = IS_MAX ? 0.0 : LARGE;
+    double x[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) x[q] = q < NQ ? v[q] : ident;
+    int qi = 0;
+#pragma unroll
+    for (int half = P / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
+        long long m = (lane & o) ? -1ll : 0ll;
+        asm volatile("" : "+v"(m));
+#pragma unroll
+        for (int k = 0; k < half; ++k) {
+            double keep = bit_blend(x[k], x[k + half], m);
+            double give = bit_blend(x[k + half], x[k], m);
+            double got = __shfl_xor(give, o);
+            x[k] = IS_MAX ? ((keep > got) ? keep : got) : ((keep < got) ? keep : got);
         }
-        if (lane == 0) sm[q][w] = x;
+        if (lane & o) qi += half;
     }
+#pragma unroll
+    for (int o = 32 / P; o > 0; o >>= 1) {
+        double got = __shfl_xor(x[0], o);
+        x[0] = IS_MAX ? ((x[0] > got) ? x[0] : got) : ((x[0] < got) ? x[0] : got);
+    }
+    if ((lane & (64 / P - 1)) == 0 && qi < NQ) sm[qi][w] = x[0];
     __syncthreads();
     int slot = (int)(blockIdx.x % NSLOT);
     unsigned long long* mine = buf + (size_t)par * nq_total * NSLOT;
