@@ -299,24 +299,24 @@ __device__ __forceinline__ double bit_blend(double a, double b, long long m) {
     return __longlong_as_double((x & ~m) | (y & m));
 }
 
+// Wave reduction of NQ values per lane as a reduce-scatter butterfly: the
+// values (padded to P, a power of two) halve at each exchange -- a lane keeps
+// the half its offset bit selects and receives its partner's copy of that
+// half -- so after log2(P) exchanges each lane holds one quantity, reduced
+// over the lanes differing in those bits; plain exchanges over the remaining
+// bits finish it: P - 1 + log2(64 / P) shuffles instead of 6 NQ.  Returns
+// the wave's reduction of quantity q (set per lane); `holder` marks one lane
+// per quantity q < NQ.  The halves are picked by a bit blend under an opaque
+// mask (a select here is turned into a dynamic index into the value array,
+// a long select chain).  Every lane of the wave must be active.
 template <int NQ, bool IS_MAX>
-__device__ __forceinline__ void slot_reduce(const double* v, unsigned long long* buf, int par, int nq_total, int q0) {
-    __shared__ double sm[NQ][16];
-    const int nw = (int)(blockDim.x >> 6), w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-    // Wave reduction as a reduce-scatter butterfly: the NQ values (padded to
-    // P, a power of two) halve at each exchange -- a lane keeps the half its
-    // offset bit selects and receives its partner's copy of that half -- so
-    // after log2(P) exchanges each lane holds one quantity, reduced over the
-    // lanes differing in those bits; plain exchanges over the remaining bits
-    // finish it: P - 1 + log2(64 / P) shuffles instead of 6 NQ.  The halves
-    // are picked by a bit blend under an opaque mask (a select here is turned
-    // into a dynamic index into the value array, a long select chain).
+__device__ __forceinline__ double wave_reduce_scatter(const double* v, int lane, int& q, bool& holder) {
     constexpr int P = NQ <= 1 ? 1 : NQ <= 2 ? 2 : NQ <= 4 ? 4 : NQ <= 8 ? 8 : 16;
     static_assert(NQ <= 16, "at most 16 quantities");
     const double ident = IS_MAX ? 0.0 : LARGE;
     double x[P];
 #pragma unroll
-    for (int q = 0; q < P; ++q) x[q] = q < NQ ? v[q] : ident;
+    for (int t = 0; t < P; ++t) x[t] = t < NQ ? v[t] : ident;
     int qi = 0;
 #pragma unroll
     for (int half = P / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
@@ -336,7 +336,19 @@ __device__ __forceinline__ void slot_reduce(const double* v, unsigned long long*
         double got = __shfl_xor(x[0], o);
         x[0] = IS_MAX ? ((x[0] > got) ? x[0] : got) : ((x[0] < got) ? x[0] : got);
     }
-    if ((lane & (64 / P - 1)) == 0 && qi < NQ) sm[qi][w] = x[0];
+    q = qi;
+    holder = (lane & (64 / P - 1)) == 0 && qi < NQ;
+    return x[0];
+}
+
+template <int NQ, bool IS_MAX>
+__device__ __forceinline__ void slot_reduce(const double* v, unsigned long long* buf, int par, int nq_total, int q0) {
+    __shared__ double sm[NQ][16];
+    const int nw = (int)(blockDim.x >> 6), w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    int qi;
+    bool holder;
+    const double xw = wave_reduce_scatter<NQ, IS_MAX>(v, lane, qi, holder);
+    if (holder) sm[qi][w] = xw;
     __syncthreads();
     int slot = (int)(blockIdx.x % NSLOT);
     unsigned long long* mine = buf + (size_t)par * nq_total * NSLOT;
@@ -495,17 +507,15 @@ __device__ __forceinline__ void zslot_reduce(const double* v, int zone, unsigned
         int first = __ffsll((long long)todo) - 1;
         int z = __shfl(zone, first);
         bool in = (zone == z);
-        for (int q = 0; q < NQ; ++q) {
-            double x = in ? v[q] : (IS_MAX ? 0.0 : LARGE);
-            for (int o = 32; o > 0; o >>= 1) {
-                double y = __shfl_xor(x, o);
-                x = IS_MAX ? ((x > y) ? x : y) : ((x < y) ? x : y);
-            }
-            if (lane == first) {
-                unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-                unsigned long long* a = mine + ((size_t)z * nq_total + q0 + q) * NSLOT + slot;
-                if (IS_MAX) atomicMax(a, bits); else atomicMin(a, bits);
-            }
+        double xin[NQ];
+        for (int q = 0; q < NQ; ++q) xin[q] = in ? v[q] : (IS_MAX ? 0.0 : LARGE);
+        int q;
+        bool holder;
+        const double x = wave_reduce_scatter<NQ, IS_MAX>(xin, lane, q, holder);
+        if (holder) {
+            unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+            unsigned long long* a = mine + ((size_t)z * nq_total + q0 + q) * NSLOT + slot;
+            if (IS_MAX) atomicMax(a, bits); else atomicMin(a, bits);
         }
         todo &= ~__ballot(in);
     }
