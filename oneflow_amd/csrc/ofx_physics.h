@@ -307,15 +307,16 @@ __device__ __forceinline__ double vencat(double x, double y, double eps) {
     return ((x2 + eps + cc * xy) / (x2 + cc * y2 + xy + eps));
 }
 __device__ __forceinline__ double vencat_fn(double dq, double minv, double maxv, double eps) {
-    if (dq > SMALL) return vencat(maxv, dq, eps);
-    else if (dq < -SMALL) return vencat(minv, dq, eps);
-    return 1.0;
+    // one evaluation on the selected extremum (the same operations as
+    // evaluating the taken branch; both branches would be if-converted)
+    double phi = vencat(dq > SMALL ? maxv : minv, dq, eps);
+    return (dq > SMALL || dq < -SMALL) ? phi : 1.0;
 }
 __device__ __forceinline__ double barth_fn(double dq, double minv, double maxv, double dot) {
+    // one division by the selected extremum (the taken branch's operation)
+    double r = (dq > maxv ? maxv : minv) / dq;
     if (dot < 0.0) return 0.0;
-    else if (dq > maxv) return maxv / dq;
-    else if (dq < minv) return minv / dq;
-    return 1.0;
+    return (dq > maxv || dq < minv) ? r : 1.0;
 }
 
 }  // namespace ofx
