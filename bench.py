@@ -43,7 +43,10 @@ TRAFFIC_SUMMARY = {2: ROOT / "profiles" / "naca0012_c1_hbm_traffic.txt",
                    3: ROOT / "profiles" / "swept_wing_c2_hbm_traffic.txt"}
 
 
-def kernel_bytes(C, B, F, dim, flux_bytes=8):
+VIS_RECORD_BYTES = {0: 32, 1: 0, 2: 160}  # per-face viscous geometry record read by face-gradient method
+
+
+def kernel_bytes(C, B, F, dim, flux_bytes=8, sa_vis_method=2, ns_vis_method=0):
     """Algorithmic bytes per launch of the NS residual kernels: every array the
     kernel must read or write counted once (FP64 values, int32 indices; the
     face fluxes are FP32 in the FP32-flux variant, flux_bytes = 4).  In 2-D the
@@ -51,7 +54,9 @@ def kernel_bytes(C, B, F, dim, flux_bytes=8):
     terms stored at mesh upload count as what the kernels read in place of the
     metrics they are formed from: Green-Gauss weights and area-weighted normal
     (4 doubles per face, 3-D + 1), face centre relative to both cell centres
-    (4 doubles, 3-D + 2), the viscous STD-gradient vector (4 doubles)."""
+    (4 doubles, 3-D + 2), and the viscous face-gradient record of the NS and
+    SA methods (ns_vis_method, sa_vis_method: STD 4 doubles, AVER none,
+    TEST 20 doubles)."""
     T = C + B
     nnz = 2 * F - B                  # entries of the cell -> face lists
     nq = 4 if dim == 2 else 5        # primitive components carried
@@ -66,17 +71,17 @@ def kernel_bytes(C, B, F, dim, flux_bytes=8):
         "limiter": 8 * nq * D * C + 8 * 2 * nq * C + idx + rec + 8 * nq * C,
         # lc/rc, face normal/area, centre offsets, viscous vector, q, grads (nq + T),
         # limiter, laminar + eddy viscosity -> inviscid and viscous face fluxes
-        "face": 8 * F + 8 * (D + 1) * F + rec + 32 * F + 8 * nq * T + 8 * (nq + 1) * D * T
+        "face": 8 * F + 8 * (D + 1) * F + rec + VIS_RECORD_BYTES[ns_vis_method] * F + 8 * nq * T + 8 * (nq + 1) * D * T
                 + 8 * nq * T + 16 * T + 2 * flux_bytes * nq * F,
         # face lists, both face fluxes -> residual
         "gather": idx + 2 * flux_bytes * nq * F + 8 * nq * C,
         # SA: nu_tilde, face lists (face + neighbour), weights, volume, wall-face
         # types -> boundary ghosts of nu_tilde and its gradient
         "sa_grad": 8 * T + idx + 4 * nnz + gg + 8 * C + 4 * B + 8 * B + 8 * D * C,
-        # lc/rc, normal/area, STD vector, rho + velocity, nu_tilde, mu_l, volume,
-        # grad nu_tilde, wall-face types -> 5 face terms
-        "sa_face": 8 * F + 8 * (D + 1) * F + 32 * F + 8 * (D + 1) * T + 8 * 3 * T + 8 * D * C + 4 * B
-                   + 40 * F,
+        # lc/rc, normal/area, viscous record, rho + velocity, nu_tilde, mu_l,
+        # volume, grad nu_tilde, wall-face types -> 5 face terms
+        "sa_face": 8 * F + 8 * (D + 1) * F + VIS_RECORD_BYTES[sa_vis_method] * F + 8 * (D + 1) * T
+                   + 8 * 3 * T + 8 * D * C + 4 * B + 40 * F,
         # face lists, 3 face terms, velocity gradient, rho, nu_tilde, mu_l, wall
         # distance, grad nu_tilde, volume, time step -> diagonal and residual
         "sa_cell": idx + 24 * F + 8 * D * D * C + 8 * 6 * C + 8 * D * C + 16 * C,
@@ -578,7 +583,8 @@ def residual_bench(s, grid, args, flux_bytes=8):
     ms_per_step = 1e3 * sec / args.steps
     copy_ms = s.copy_bench(args.copy_bytes, iters=20)
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms * 1e-3) / 1e9
-    kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, grid.dim, flux_bytes)
+    kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, grid.dim, flux_bytes, s.cfg.iturb_visflux,
+                      s.cfg.ivischeme)
     step_bytes = 552.0 * T + 64.0 * F if grid.dim == 2 else 693.0 * T + 64.0 * F
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
     # each NS residual kernel timed with HIP events on the solver's stream over
