@@ -6,7 +6,7 @@ Pipeline (all compute on the GPU through libofx.so):
   3. train the BF16 MLP (ofx_mlp_*) on them with the library's own kernels;
   4. for unseen cases, predict the field, write it as the initial state, and
      let the plain solver iterate from there to the same residual tolerance as
-     from freestream -- the solver is both the fallback and the judge.
+     from freestream -- the solver is both the fallback and the test of convergence.
 
 Inputs per cell (9): the centroid folded into the unit disk around mid-chord,
 x' = (x - 0.5) / (1 + r), y' = y / (1 + r), r = |(x - 0.5, y)|; the wall

@@ -6,7 +6,7 @@
  * codes/turb, codes/usolver, codes/grad), kept in the reference's face order,
  * cell order and accumulation order and compiled with -ffp-contract=off.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load it, and only as the checker or the timed CPU baseline.  The product
+ * load it, and only to check results or as the timed CPU baseline.  The product
  * (oneflow_amd/libofx.so) never calls it.
  *
  * The entry points mirror include/ofx.h one to one with an ofo_ prefix.

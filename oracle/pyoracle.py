@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboforacle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+bench.py's cpu_baseline leg, to check results or as the timed CPU baseline.  The
 product path (oneflow_amd) never imports this module.
 """
 from __future__ import annotations
