@@ -122,7 +122,7 @@ def test_one_step_task_by_task(case):
                 s.update_sa()
                 s.sa_boundary()
             cmp(gpu, ora, ["TURBQ", "VIST"])
-        # advance the oracle one full step so the next round starts from a non-trivial state
+        # advance the oracle one full step so the next pass starts from a non-trivial state
         ora.iterate(it, 1)
 
 
