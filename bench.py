@@ -107,15 +107,25 @@ def c1_config():
                       reynolds=6.5e6, centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
 
 
-def c1_grid(numbering="multicolor"):
+def _numbered(g, numbering, face_order="cell"):
+    """The grid under a cell numbering; face_order "spatial" then sorts the
+    interior faces by the cells' ids before the numbering (their spatial
+    order), "cell" keeps them sorted by their renumbered cells."""
+    from oneflow_amd import meshgen
+    if numbering == "natural":
+        return g
+    order = meshgen.blocked_multicolor_order(g) if numbering == "blocked" else meshgen.multicolor_order(g)
+    g2 = meshgen.renumber(g, order)
+    if face_order == "spatial":
+        g2 = meshgen.sort_interior_faces(g2, order)
+    return g2
+
+
+def c1_grid(numbering="multicolor", face_order="cell"):
     from oneflow_amd import meshgen
     g = meshgen.naca_hybrid(n_surf=512, n_quad=40, ratio=1.15, reynolds=6.5e6, n_outer=98, radius=30.0,
                             m=0.0, p=0.4, t=0.12)
-    if numbering == "multicolor":
-        g = meshgen.renumber(g, meshgen.multicolor_order(g))
-    elif numbering == "blocked":
-        g = meshgen.renumber(g, meshgen.blocked_multicolor_order(g))
-    return g
+    return _numbered(g, numbering, face_order)
 
 
 def cpu_baseline(grid, cfg, budget_s=10.0):
@@ -137,8 +147,12 @@ def cpu_baseline(grid, cfg, budget_s=10.0):
     return n, dt
 
 
-def _sweep_grid(case):
-    return case.grid("multicolor")
+def _sweep_grid(case, face_order="cell"):
+    return case.grid("multicolor", face_order)
+
+
+def _sweep_grid_spatial(case):
+    return case.grid("multicolor", "spatial")
 
 
 def batch_main(args):
@@ -157,7 +171,7 @@ def batch_main(args):
     cases = airfoil_sweep(args.cases, seed=args.seed)
     t0 = time.perf_counter()
     with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        grids = list(ex.map(_sweep_grid, cases))
+        grids = list(ex.map(_sweep_grid_spatial if args.face_order == "spatial" else _sweep_grid, cases))
     ncell = sum(g.n_cells for g in grids)
     nt = sum(g.n_tcell for g in grids)
     print(f"[bench] {len(grids)} grids, {ncell} cells, generated in {time.perf_counter() - t0:.1f} s",
@@ -221,7 +235,8 @@ def batch_main(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{len(cases)} NACA 4-digit airfoils, hybrid grids, Roe + SA + Venkat, FP64",
-                   "cells": ncell, "nTCell": nt, "seed": args.seed, "graph_steps": args.graph_steps,
+                   "cells": ncell, "nTCell": nt, "seed": args.seed, "face_order": args.face_order or "cell",
+                   "graph_steps": args.graph_steps,
                    "graph_launches": k},
         "sequential_ms_per_step": ms_seq,
         "graph_ms_per_step": ms_batch,
@@ -474,6 +489,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--numbering", default=None, choices=["natural", "multicolor", "blocked"],
                     help="cell numbering (default multicolor)")
+    ap.add_argument("--face-order", default=None, choices=["cell", "spatial"],
+                    help="interior faces sorted by their renumbered cells or by their spatial order "
+                         "(default: cell for c1, spatial for c2)")
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "batch", "warmstart"],
@@ -515,7 +533,9 @@ def main():
     cfg = c1_config()
     if args.numbering is None:
         args.numbering = "multicolor"
-    grid = c1_grid(args.numbering)
+    if args.face_order is None:
+        args.face_order = "cell"
+    grid = c1_grid(args.numbering, args.face_order)
     print(f"[bench] grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces}", file=sys.stderr, flush=True)
     s = Solver(grid, cfg, device=0)
     r = residual_bench(s, grid, args)
@@ -539,7 +559,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "NACA0012 hybrid quad/tri grid, Roe + SA + Venkat, FP64 residual",
-                   "numbering": args.numbering,
+                   "numbering": args.numbering, "face_order": args.face_order,
                    "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": s.n_tcell,
                    "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
         "roofline": r["roofline"],
@@ -622,14 +642,11 @@ def c2_config():
                       reynolds=1.2e7, centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
 
 
-def c2_grid(n_span=34, numbering="multicolor"):
+def c2_grid(n_span=34, numbering="multicolor", face_order="spatial"):
     from oneflow_amd import meshgen
     g = meshgen.swept_wing(n_surf=256, n_quad=25, ratio=1.2, reynolds=1.2e7, n_outer=40, radius=20.0,
                            n_span=n_span, sweep_deg=30.0, taper=0.5, aspect=4.0, t=0.12)
-    if numbering == "natural":
-        return g
-    order = meshgen.blocked_multicolor_order(g) if numbering == "blocked" else meshgen.multicolor_order(g)
-    return meshgen.renumber(g, order)
+    return _numbered(g, numbering, face_order)
 
 
 def c2_main(args):
@@ -642,7 +659,9 @@ def c2_main(args):
     t0 = time.perf_counter()
     if args.numbering is None:
         args.numbering = "multicolor"
-    grid = c2_grid(numbering=args.numbering)
+    if args.face_order is None:
+        args.face_order = "spatial"
+    grid = c2_grid(numbering=args.numbering, face_order=args.face_order)
     print(f"[bench] wing: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces} "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     s = Solver(grid, cfg, device=0)
@@ -666,7 +685,7 @@ def c2_main(args):
         "warmup": args.warmup, "ms_per_step": r64["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "swept wing NACA0012, sweep 30, taper 0.5, AR 4, prism layers + tets, "
-                               "Roe + SA + Venkat", "numbering": args.numbering,
+                               "Roe + SA + Venkat", "numbering": args.numbering, "face_order": args.face_order,
                    "cells": grid.n_cells, "faces": grid.n_faces,
                    "nTCell": grid.n_tcell, "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree,
                    "reynolds": cfg.reynolds},

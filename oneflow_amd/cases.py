@@ -38,14 +38,19 @@ class AirfoilCase:
         base.update(kw)
         return CaseConfig(**base)
 
-    def grid(self, numbering: str = "multicolor"):
+    def grid(self, numbering: str = "multicolor", face_order: str = "cell"):
+        """The case's grid; with the multicolour numbering, face_order "spatial"
+        sorts the interior faces by their cells' ids before it."""
         s = math.sqrt(self.target_cells / _REF_CELLS)
         n_surf = max(64, int(round(512 * s / 8.0)) * 8)
         n_outer = max(16, int(round(98 * s)))
         g = meshgen.naca_hybrid(n_surf=n_surf, n_quad=40, ratio=1.15, reynolds=self.reynolds, n_outer=n_outer,
                                 radius=30.0, m=self.m, p=self.p, t=self.t)
         if numbering == "multicolor":
-            g = meshgen.renumber(g, meshgen.multicolor_order(g))
+            order = meshgen.multicolor_order(g)
+            g = meshgen.renumber(g, order)
+            if face_order == "spatial":
+                g = meshgen.sort_interior_faces(g, order)
         return g
 
     def features(self) -> np.ndarray:

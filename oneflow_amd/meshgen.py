@@ -278,6 +278,29 @@ def _renumber_faces_nd(g: Grid, lc, rc) -> Grid:
     return out
 
 
+def sort_interior_faces(g: Grid, cell_key) -> Grid:
+    """Return a copy whose interior faces are sorted by the (smaller, larger)
+    cell_key of their two cells; boundary faces, every face's cells and node
+    order (so its normal) are unchanged.  With cell_key a spatial order of the
+    cells (for a colouring numbering, the cells' ids before it), the face
+    kernels walk the grid once in space instead of once per colour."""
+    nb = g.n_bfaces
+    key = np.asarray(cell_key, np.int64)
+    a, b = key[g.lc[nb:]], key[g.rc[nb:]]
+    srt = np.lexsort((np.maximum(a, b), np.minimum(a, b)))
+    perm = np.concatenate([np.arange(nb), nb + srt])
+    cnt = g.face_node_count.astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(cnt)])
+    lens = cnt[perm]
+    first = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    idx = np.repeat(off[perm] - first, lens) + np.arange(int(lens.sum()))
+    out = Grid(dim=g.dim, x=g.x, y=g.y, z=g.z, face_node_count=g.face_node_count[perm].copy(),
+               face_nodes=g.face_nodes[idx].astype(np.int32), lc=g.lc[perm].copy(), rc=g.rc[perm].copy(),
+               bc_type=g.bc_type.copy(), bc_name_id=g.bc_name_id.copy(), n_cells=g.n_cells)
+    out.check()
+    return out
+
+
 # ---------------------------------------------------------------- flat plate
 
 def geometric(n, h0, ratio):

@@ -28,8 +28,11 @@ def synthetic(case):
     from oneflow_amd import meshgen
     from oneflow_amd.config import CaseConfig
     g = meshgen.naca_hybrid(n_surf=96, n_quad=8, ratio=1.3, reynolds=6.5e6, n_outer=12, radius=20.0, m=0.02, p=0.4, t=0.12)
-    if case.endswith("mc"):
-        g = meshgen.renumber(g, meshgen.multicolor_order(g))
+    if case.endswith("mc") or case.endswith("mcsf"):
+        order = meshgen.multicolor_order(g)
+        g = meshgen.renumber(g, order)
+        if case.endswith("mcsf"):  # interior faces in their cells' spatial order
+            g = meshgen.sort_interior_faces(g, order)
     elif case.endswith("blk"):  # blocked colouring: levels that are not runs of consecutive cells
         g = meshgen.renumber(g, meshgen.blocked_multicolor_order(g, part_cells=400))
     cfg = CaseConfig(ischeme=1, ilim=2, vismodel=3, mach_ref=0.8, aoa_degree=1.25, reynolds=6.5e6,
@@ -74,7 +77,7 @@ def sync(gpu, ora, names=STATE):
         gpu.set(n, ora.get(n))
 
 
-@pytest.mark.parametrize("case", CASES + ["syn_natural", "syn_mc", "syn_blk"])
+@pytest.mark.parametrize("case", CASES + ["syn_natural", "syn_mc", "syn_mcsf", "syn_blk"])
 def test_one_step_task_by_task(case):
     gpu, ora = make(case)
     for s in (gpu, ora):
