@@ -246,7 +246,10 @@ int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
  * it).  kernel 0, the NS gradient: variant 0 (default) one thread per cell,
  * variant 1 one wave per quantity (more waves in flight on small grids).
  * kernel 1, the limiter, and kernel 2, the residual gather: variant 0 one
- * lane per face of a cell, variant 1 (default) one thread per cell. */
+ * lane per face of a cell, variant 1 (default) one thread per cell.
+ * kernel 3, the walk of every cell kernel over a colouring numbering:
+ * variant 0 (default) each XCD's share colour by colour, variant 1 in
+ * chunks of 8 blocks taking the colours in turn. */
 int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
 int ofx_get_kernel_variant(ofx_ctx* ctx, int kernel, int* variant);
 

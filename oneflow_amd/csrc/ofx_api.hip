@@ -481,21 +481,26 @@ int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
     return 0;
 }
 
+// chunk of the interleaved cell walk (kernel 3, variant 1), in blocks
+static constexpr int CELL_CHUNK = 8;
+
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel < 0 || kernel > 2 || variant < 0 || variant > 1) {
-        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter) or 2 (gather), variant 0 or 1");
+    if (kernel < 0 || kernel > 3 || variant < 0 || variant > 1) {
+        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather) or 3 (cell walk), variant 0 or 1");
         return OFX_EINVAL;
     }
-    (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant) = variant;
+    if (kernel == 3) c->m.cell_chunk = variant ? CELL_CHUNK : 0;
+    else (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant) = variant;
     ++c->gen;
     return 0;
 }
 
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
-    if (!variant || kernel < 0 || kernel > 2) { set_error("kernel variant: kernel 0, 1 or 2 and non-null out"); return OFX_EINVAL; }
-    *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant;
+    if (!variant || kernel < 0 || kernel > 3) { set_error("kernel variant: kernel 0..3 and non-null out"); return OFX_EINVAL; }
+    if (kernel == 3) *variant = c->m.cell_chunk > 0 ? 1 : 0;
+    else *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant;
     return 0;
 }
 
@@ -533,6 +538,20 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
             if (ms < best) { best = ms; pick = t; }
         }
         c->saface_tb = pick;
+    }
+    // cell walk (colour by colour, or interleaved chunks): timed on the whole
+    // NS + SA residual, every cell kernel follows it
+    if (c->m.nsegc > 1) {
+        best = 1e300;
+        pick = c->m.cell_chunk;
+        for (int v = 0; v < 2; ++v) {
+            c->m.cell_chunk = v ? CELL_CHUNK : 0;
+            double a = 0, b2 = 0;
+            OFX_TRY(ofx_time_kernel(c, 4, iters, &a));
+            if (c->k.vismodel == 3) OFX_TRY(ofx_time_kernel(c, 5, iters, &b2));
+            if (a + b2 < best) { best = a + b2; pick = c->m.cell_chunk; }
+        }
+        c->m.cell_chunk = pick;
     }
     ++c->gen;
     // leave the residual fields as one evaluation of the current state leaves them
