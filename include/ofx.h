@@ -249,7 +249,8 @@ int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
  * lane per face of a cell, variant 1 (default) one thread per cell.
  * kernel 3, the walk of every cell kernel over a colouring numbering:
  * variant 0 (default) each XCD's share colour by colour, variant 1 in
- * chunks of 8 blocks taking the colours in turn. */
+ * chunks of 8 blocks taking the colours in turn, variant K in 2..4096 in
+ * chunks of K blocks; ofx_get_kernel_variant returns the chunk (0: plain). */
 int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
 int ofx_get_kernel_variant(ofx_ctx* ctx, int kernel, int* variant);
 

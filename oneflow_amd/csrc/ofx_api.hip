@@ -486,11 +486,12 @@ static constexpr int CELL_CHUNK = 8;
 
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel < 0 || kernel > 3 || variant < 0 || variant > 1) {
-        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather) or 3 (cell walk), variant 0 or 1");
+    if (kernel < 0 || kernel > 3 || variant < 0 || (kernel < 3 && variant > 1) || variant > 4096) {
+        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather): variant 0 or 1; "
+                  "kernel 3 (cell walk): 0, 1 or a chunk of 2..4096 blocks");
         return OFX_EINVAL;
     }
-    if (kernel == 3) c->m.cell_chunk = variant ? CELL_CHUNK : 0;
+    if (kernel == 3) c->m.cell_chunk = variant == 1 ? CELL_CHUNK : variant;
     else (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant) = variant;
     ++c->gen;
     return 0;
@@ -499,7 +500,7 @@ int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
     if (!variant || kernel < 0 || kernel > 3) { set_error("kernel variant: kernel 0..3 and non-null out"); return OFX_EINVAL; }
-    if (kernel == 3) *variant = c->m.cell_chunk > 0 ? 1 : 0;
+    if (kernel == 3) *variant = c->m.cell_chunk;
     else *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant;
     return 0;
 }
@@ -542,14 +543,15 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     // cell walk (colour by colour, or interleaved chunks): timed on the whole
     // NS + SA residual, every cell kernel follows it
     if (c->m.nsegc > 1) {
+        static const int chunks[4] = {0, CELL_CHUNK, 4 * CELL_CHUNK, 16 * CELL_CHUNK};
         best = 1e300;
         pick = c->m.cell_chunk;
-        for (int v = 0; v < 2; ++v) {
-            c->m.cell_chunk = v ? CELL_CHUNK : 0;
+        for (int K : chunks) {
+            c->m.cell_chunk = K;
             double a = 0, b2 = 0;
             OFX_TRY(ofx_time_kernel(c, 4, iters, &a));
             if (c->k.vismodel == 3) OFX_TRY(ofx_time_kernel(c, 5, iters, &b2));
-            if (a + b2 < best) { best = a + b2; pick = c->m.cell_chunk; }
+            if (a + b2 < best) { best = a + b2; pick = K; }
         }
         c->m.cell_chunk = pick;
     }

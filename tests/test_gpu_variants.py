@@ -60,15 +60,16 @@ def test_cell_walk_variant_bitwise(face_order):
     cfg = CaseConfig(ischeme=1, ilim=2, vismodel=3, mach_ref=0.8, aoa_degree=1.25, reynolds=6.5e6,
                      centropy1=0.001, centropy2=0.001, vencat_coef=0.001)
     runs = []
-    for v in (0, 1):
+    for v, chunk in ((0, 0), (1, 8), (3, 3), (64, 64)):
         s = Solver(g, cfg)
         s.set_kernel_variant(3, v)
-        assert s.kernel_variant(3) == v
+        assert s.kernel_variant(3) == chunk
         s.init_freestream()
         s.iterate(1, 12, with_norms=False)
         s.residual_ns()
         s.residual_sa()
         runs.append(s)
     for n in FIELDS + ["TURBRES"]:
-        assert np.array_equal(runs[0].get(n), runs[1].get(n)), (face_order, n)
+        for k in range(1, len(runs)):
+            assert np.array_equal(runs[0].get(n), runs[k].get(n)), (face_order, n, k)
     print(f"\n[variants] cell walk, faces in {face_order} order: both walks agree", flush=True)
