@@ -1042,7 +1042,8 @@ __device__ __forceinline__ void ns_lusgs_face(const Consts& k, double xfn, doubl
 // unrolled to MAXF: all neighbour loads are issued first, then the face
 // terms are added in ascending face order.
 template <bool UPPER, int MAXF>
-__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int i, double mindt1) {
+__device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int i, double mindt1,
+                                              bool store = true) {
     const int T = m.T;
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const double mindt = zone_mindt(m, d, i, mindt1);
@@ -1092,6 +1093,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
         }
     }
     double rad = ns_impsr(m, d, k, i, mindt);
+    if (!store) return;
     if (!UPPER) {
         for (int e = 0; e < 5; ++e) { double t = d.res[e * T + i] - rhs0[e]; t /= rad; d.dq[e * T + i] = t; }
     } else {
@@ -1119,11 +1121,16 @@ __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, C
 template <bool UPPER, int MAXF>
 __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int base,
                                                        int par) {
-    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
-    int idx = b + blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= e) return;
+    // Every lane stays active (the minimum time step is reduced per wave, its
+    // slot loads overlapping the cell's own): lanes past the level's end
+    // evaluate its last cell and store nothing.
+    const int idx0 = b + blockIdx.x * TB + threadIdx.x;
+    const bool live = idx0 < e;
+    const int idx = live ? idx0 : e - 1;
+    double mindt;
+    wave_slot_result<1, false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN, &mindt);
     int i = base >= 0 ? base + (idx - b) : (UPPER ? m.upOrder : m.loOrder)[idx];
-    ns_lusgs_cell<UPPER, MAXF>(m, d, k, i, mindt);
+    ns_lusgs_cell<UPPER, MAXF>(m, d, k, i, mindt, live);
 }
 
 // Launch policy: few wide levels (multicolour numbering) -> one grid-wide
