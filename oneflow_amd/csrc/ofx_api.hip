@@ -486,12 +486,13 @@ static constexpr int CELL_CHUNK = 8;
 
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel < 0 || kernel > 3 || variant < 0 || (kernel < 3 && variant > 1) || variant > 4096) {
-        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather): variant 0 or 1; "
-                  "kernel 3 (cell walk): 0, 1 or a chunk of 2..4096 blocks");
+    if (kernel < 0 || kernel > 4 || variant < 0 || (kernel != 3 && variant > 1) || variant > 4096) {
+        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather), 4 (NS face split): "
+                  "variant 0 or 1; kernel 3 (cell walk): 0, 1 or a chunk of 2..4096 blocks");
         return OFX_EINVAL;
     }
-    if (kernel == 3) c->m.cell_chunk = variant == 1 ? CELL_CHUNK : variant;
+    if (kernel == 4) c->face_split = variant;
+    else if (kernel == 3) c->m.cell_chunk = variant == 1 ? CELL_CHUNK : variant;
     else (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant) = variant;
     ++c->gen;
     return 0;
@@ -499,8 +500,9 @@ int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
 
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
-    if (!variant || kernel < 0 || kernel > 3) { set_error("kernel variant: kernel 0..3 and non-null out"); return OFX_EINVAL; }
-    if (kernel == 3) *variant = c->m.cell_chunk;
+    if (!variant || kernel < 0 || kernel > 4) { set_error("kernel variant: kernel 0..4 and non-null out"); return OFX_EINVAL; }
+    if (kernel == 4) *variant = c->face_split;
+    else if (kernel == 3) *variant = c->m.cell_chunk;
     else *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant;
     return 0;
 }
@@ -511,12 +513,19 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     static const int cand[4] = {64, 128, 256, 512};
     double best = 1e300, ms = 0;
     int pick = c->face_tb;
+    c->face_split = 0;
     for (int t : cand) {
         c->face_tb = t;
         OFX_TRY(ofx_time_kernel(c, 2, iters, &ms));
         if (ms < best) { best = ms; pick = t; }
     }
     c->face_tb = pick;
+    // the split face kernel (viscous FP64 runs), against the best tile
+    if (c->k.vismodel != 0 && !c->k.flux_fp32) {
+        c->face_split = 1;
+        OFX_TRY(ofx_time_kernel(c, 2, iters, &ms));
+        c->face_split = ms < best ? 1 : 0;
+    }
     // NS gradient, limiter and gather variants (kernel parts 0, 1, 3)
     int* var[3] = {&c->grad_variant, &c->lim_variant, &c->gather_variant};
     const int part[3] = {0, 1, 3};

@@ -170,6 +170,7 @@ struct ofx_ctx {
     int grad_variant = 0;  // NS gradient: 0 one thread per cell, 1 one wave per quantity
     int lim_variant = 1;     // limiter: 0 lanes per cell, 1 one thread per cell
     int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
+    int face_split = 0;      // viscous NS face kernel: 1 inviscid and viscous parts on separate waves
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
     bool dist_valid = false;  // wall distance computed for the current mesh / zones
     int fuse_l0 = 0;  // > 0 while a full step fuses the first lower-sweep level into the gathers

@@ -716,9 +716,14 @@ __device__ __forceinline__ void ns_face_visc(const DevMesh& m, const DevFields& 
 // F32: the FP32-flux variant -- the inviscid flux is evaluated in FP32 from
 // the FP64 reconstruction and both face fluxes are stored as FP32 (the
 // gather accumulates them in FP64).
-template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32>
+// SPLIT (viscous runs): a block of two waves covers 64 faces, the first wave
+// computing their inviscid fluxes and the second their viscous fluxes (each
+// wave loads what its part needs): half the work and registers per wave,
+// twice the waves in flight.
+template <int DIM, int LIM, bool SLAU2, bool VISC, bool F32, bool SPLIT = false>
 __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts k0, int tb) {
-    int f = face_block(m, tb) * tb + threadIdx.x;
+    const int role = SPLIT ? (int)(threadIdx.x >> 6) : 0;  // wave-uniform: 0 inviscid, 1 viscous
+    int f = SPLIT ? face_block(m, 64) * 64 + (int)(threadIdx.x & 63) : face_block(m, tb) * tb + (int)threadIdx.x;
     if (f >= m.F) return;
     const Consts k = zoned(k0, m, face_zone(m, f));
     const int T = m.T;
@@ -731,6 +736,7 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
         ql[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + l];
         qr[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + r];
     }
+    if (!SPLIT || role == 0) {
     if (bnd) {
         for (int e = 0; e < 5; ++e) { double tmp = 0.5 * (ql[e] + qr[e]); p1[e] = tmp; p2[e] = tmp; }
         if (m.bct[f] == BC_SOLID_SURFACE && k.vismodel != 0) {
@@ -784,8 +790,9 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
         else roe_flux<double>(k.ieigenfix, k.centropy1, k.centropy2, p1, p2, k.gama, k.gama, xfn, yfn, zfn, fl);
         for (int e = 0; e < 5; ++e) d.invflux[e * m.F + f] = area * fl[e];
     }
+    }  // inviscid part
 
-    if (!VISC) return;
+    if (!VISC || (SPLIT && role == 0)) return;
     // the face-gradient method is uniform: one dispatch, each branch a
     // separate instantiation of the viscous flux
     if (k.ivischeme == 0) ns_face_visc<DIM, F32, 0>(m, d, k, f, l, r, rg, ql, qr, xfn, yfn, zfn, area);
@@ -841,6 +848,10 @@ template <int DIM, int LIM, bool SLAU2>
 static void launch_face(ofx_ctx* c, bool visc) {
     const int tb = c->face_tb;  // face tile, chosen by ofx_autotune
     int nb = (c->m.F + tb - 1) / tb;
+    if (visc && c->face_split && !c->k.flux_fp32) {
+        k_ns_face<DIM, LIM, SLAU2, true, false, true><<<(c->m.F + 63) / 64, 128, 0, c->stream>>>(c->m, c->d, c->k, 64);
+        return;
+    }
     if (c->k.flux_fp32) {
         if (visc) k_ns_face<DIM, LIM, SLAU2, true, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k, tb);
         else k_ns_face<DIM, LIM, SLAU2, false, true><<<nb, tb, 0, c->stream>>>(c->m, c->d, c->k, tb);

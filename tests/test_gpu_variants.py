@@ -3,7 +3,8 @@
 The NS gradient (one thread per cell; one wave per quantity), the limiter
 and the residual gather (one lane per face; one thread per cell) have two
 kernels each, the cell kernels two walks over a colouring (colour by colour;
-interleaved chunks), and the face kernels take any face tile.  Every combination must give the
+interleaved chunks), the viscous face kernel a split form (inviscid and
+viscous parts on separate waves), and the face kernels take any face tile.  Every combination must give the
 same state, gradients, limiter and residual bit for bit, here after twelve
 pseudo-time steps from freestream (through the CFL ramp) on 2-D and 3-D grids
 and several option sets.  Variant 0 with the default tile is checked against
@@ -25,9 +26,9 @@ def test_grad_variant_and_tiles_bitwise(grid, opt):
     from oneflow_amd.solver import Solver
     base, _, g = make(grid, opt)
     runs = [base]
-    # (gradient, limiter, gather) variants and face tiles
-    for variants, tiles in (((1, 0, 0), (256, 256)), ((1, 1, 1), (64, 128)), ((0, 1, 0), (512, 64)),
-                            ((0, 0, 1), (128, 512))):
+    # (gradient, limiter, gather, cell walk, face split) variants and face tiles
+    for variants, tiles in (((1, 0, 0, 0, 1), (256, 256)), ((1, 1, 1, 8, 0), (64, 128)),
+                            ((0, 1, 0, 32, 1), (512, 64)), ((0, 0, 1, 0, 0), (128, 512))):
         s = Solver(g, base.cfg)
         for kernel, v in enumerate(variants):
             s.set_kernel_variant(kernel, v)
