@@ -633,8 +633,7 @@ def residual_bench(s, grid, args, flux_bytes=8):
     return {"value": T * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline, "kernels": kernels,
             "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
             "copy_kernel_gbs": copy_gbs,
-            "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "cell_chunk": s.kernel_variant(3),
-                       "face_split": s.kernel_variant(4)}}
+            "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "face_split": s.kernel_variant(3)}}
 
 
 def c2_config():

@@ -247,11 +247,7 @@ int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
  * variant 1 one wave per quantity (more waves in flight on small grids).
  * kernel 1, the limiter, and kernel 2, the residual gather: variant 0 one
  * lane per face of a cell, variant 1 (default) one thread per cell.
- * kernel 3, the walk of every cell kernel over a colouring numbering:
- * variant 0 (default) each XCD's share colour by colour, variant 1 in
- * chunks of 8 blocks taking the colours in turn, variant K in 2..4096 in
- * chunks of K blocks; ofx_get_kernel_variant returns the chunk (0: plain).
- * kernel 4, the viscous FP64 NS face kernel: variant 0 (default) one thread
+ * kernel 3, the viscous FP64 NS face kernel: variant 0 (default) one thread
  * per face, variant 1 two waves per 64 faces (inviscid / viscous part). */
 int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
 int ofx_get_kernel_variant(ofx_ctx* ctx, int kernel, int* variant);

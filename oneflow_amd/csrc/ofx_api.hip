@@ -481,29 +481,24 @@ int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
     return 0;
 }
 
-// chunk of the interleaved cell walk (kernel 3, variant 1), in blocks
-static constexpr int CELL_CHUNK = 8;
-
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel < 0 || kernel > 4 || variant < 0 || (kernel != 3 && variant > 1) || variant > 4096) {
-        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather), 4 (NS face split): "
-                  "variant 0 or 1; kernel 3 (cell walk): 0, 1 or a chunk of 2..4096 blocks");
+    if (kernel < 0 || kernel > 3 || variant < 0 || variant > 1) {
+        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather) or 3 (NS face split), "
+                  "variant 0 or 1");
         return OFX_EINVAL;
     }
-    if (kernel == 4) c->face_split = variant;
-    else if (kernel == 3) c->m.cell_chunk = variant == 1 ? CELL_CHUNK : variant;
-    else (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant) = variant;
+    (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : kernel == 2 ? c->gather_variant : c->face_split) =
+        variant;
     ++c->gen;
     return 0;
 }
 
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
-    if (!variant || kernel < 0 || kernel > 4) { set_error("kernel variant: kernel 0..4 and non-null out"); return OFX_EINVAL; }
-    if (kernel == 4) *variant = c->face_split;
-    else if (kernel == 3) *variant = c->m.cell_chunk;
-    else *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : c->gather_variant;
+    if (!variant || kernel < 0 || kernel > 3) { set_error("kernel variant: kernel 0..3 and non-null out"); return OFX_EINVAL; }
+    *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : kernel == 2 ? c->gather_variant
+                                                                                          : c->face_split;
     return 0;
 }
 
@@ -548,21 +543,6 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
             if (ms < best) { best = ms; pick = t; }
         }
         c->saface_tb = pick;
-    }
-    // cell walk (colour by colour, or interleaved chunks): timed on the whole
-    // NS + SA residual, every cell kernel follows it
-    if (c->m.nsegc > 1) {
-        static const int chunks[4] = {0, CELL_CHUNK, 4 * CELL_CHUNK, 16 * CELL_CHUNK};
-        best = 1e300;
-        pick = c->m.cell_chunk;
-        for (int K : chunks) {
-            c->m.cell_chunk = K;
-            double a = 0, b2 = 0;
-            OFX_TRY(ofx_time_kernel(c, 4, iters, &a));
-            if (c->k.vismodel == 3) OFX_TRY(ofx_time_kernel(c, 5, iters, &b2));
-            if (a + b2 < best) { best = a + b2; pick = K; }
-        }
-        c->m.cell_chunk = pick;
     }
     ++c->gen;
     // leave the residual fields as one evaluation of the current state leaves them

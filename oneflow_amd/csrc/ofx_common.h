@@ -92,9 +92,6 @@ struct DevMesh {
     // every segment on the same XCD (xcd_block_seg).
     int nsegc = 1, nsegf = 1;
     int segc[MAXSEG + 1] = {0}, segf[MAXSEG + 1] = {0};
-    // cell kernels: 0 walk each XCD's share colour by colour; K > 0 walk it in
-    // chunks of K blocks taking the colours in turn (a launch variant)
-    int cell_chunk = 0;
 };
 
 // The case-dependent constants (Mach, AoA, sideslip, Reynolds) of one zone.
@@ -248,45 +245,6 @@ __host__ __device__ __forceinline__ int xcd_remap_seg_by(int b, int nblk, CEIL c
 __host__ __device__ __forceinline__ int xcd_remap_seg(int b, int nblk, int per, const int* start, int nseg) {
     return xcd_remap_seg_by(b, nblk, [per](int v) { return (v + per - 1) / per; }, start, nseg);
 }
-// The same blocks per XCD as xcd_remap_seg, walked in a different order: the
-// XCD's share of every segment is cut into chunks of K blocks and the chunks
-// are taken segment after segment in turn (chunk 0 of every segment, then
-// chunk 1, ...), so blocks resident together cover one region in all
-// segments.  Bijective on [0, nblk) for the same launches.
-__host__ __device__ __forceinline__ int xcd_remap_seg_chunked(int b, int nblk, int per, const int* start, int nseg,
-                                                              int K) {
-    const int x = b & 7;
-    int slot = b >> 3, rr = 0;
-    int base[MAXSEG], cnt[MAXSEG];
-#pragma unroll
-    for (int s = 0; s < MAXSEG; ++s) {
-        base[s] = 0;
-        cnt[s] = 0;
-        if (s < nseg) {
-            int b0 = (start[s] + per - 1) / per;
-            int b1 = (s + 1 == nseg) ? nblk : (start[s + 1] + per - 1) / per;
-            int n = b1 - b0, q = n >> 3, r = n & 7;
-            unsigned w = ((1u << r) - 1u) << rr;
-            unsigned mask = (w | (w >> 8)) & 0xFFu;
-            cnt[s] = q + (int)((mask >> x) & 1u);
-            base[s] = b0 + x * q + __builtin_popcount(mask & ((1u << x) - 1u));
-            rr = (rr + r) & 7;
-        }
-    }
-    for (int lo = 0;; lo += K) {
-        bool any = false;
-#pragma unroll
-        for (int s = 0; s < MAXSEG; ++s) {
-            if (s < nseg && lo < cnt[s]) {
-                int take = cnt[s] - lo < K ? cnt[s] - lo : K;
-                any = true;
-                if (slot < take) return base[s] + lo + slot;
-                slot -= take;
-            }
-        }
-        if (!any) return b;  // not reached for a consistent launch
-    }
-}
 // The same for per = 1 << lg.
 __host__ __device__ __forceinline__ int xcd_remap_seg_lg(int b, int nblk, int lg, const int* start, int nseg) {
     return xcd_remap_seg_by(
@@ -306,8 +264,6 @@ __device__ __forceinline__ void cell_faces(const DevMesh& m, int i, int* ent, in
 
 // Logical block of a launch over the cells (faces), `per` items per block.
 __device__ __forceinline__ int cell_block(const DevMesh& m, int per) {
-    if (m.cell_chunk > 0 && m.nsegc > 1)
-        return xcd_remap_seg_chunked((int)blockIdx.x, (int)gridDim.x, per, m.segc, m.nsegc, m.cell_chunk);
     return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segc, m.nsegc);
 }
 // per: a power of two (the face tiles are 64 .. 512)

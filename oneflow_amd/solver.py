@@ -226,7 +226,7 @@ class Solver:
         return int(v[0])
 
     def set_kernel_variant(self, kernel, variant):
-        """kernel 0 gradient, 1 limiter, 2 gather, 3 cell walk; variant 0 or 1 (results identical)."""
+        """kernel 0 gradient, 1 limiter, 2 gather, 3 NS face split; variant 0 or 1 (results identical)."""
         self._check(self._L.ofx_set_kernel_variant(self.h, kernel, variant))
 
     def autotune(self, iters=20):

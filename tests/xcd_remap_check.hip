@@ -2,8 +2,7 @@
 // shapes and segment cuts, xcd_remap_seg must be a bijection on [0, nblk)
 // whose blocks of segment s stay inside s, and with one segment it must equal
 // xcd_remap; for power-of-two block sizes the shift form xcd_remap_seg_lg
-// must agree with it block for block, and the chunked walk
-// xcd_remap_seg_chunked must be a bijection as well.  Prints "ok <blocks>" or the first failure; runs no GPU code.
+// must agree with it block for block.  Prints "ok <blocks>" or the first failure; runs no GPU code.
 #include "../oneflow_amd/csrc/ofx_common.h"
 
 #include <algorithm>
@@ -27,22 +26,19 @@ int main() {
         for (int s = 1; s < nseg; ++s) st[s] = cut[s - 1];
         st[nseg] = n;
         int nblk = (n + per - 1) / per;
-        std::vector<int> seen(nblk, 0), seen2(nblk, 0);
+        std::vector<int> seen(nblk, 0);
         for (int b = 0; b < nblk; ++b) {
             int lb = xcd_remap_seg(b, nblk, per, st.data(), nseg);
             if (lb < 0 || lb >= nblk) { std::printf("out of range: trial %d block %d -> %d\n", trial, b, lb); return 1; }
             ++seen[lb];
             if (nseg == 1 && lb != xcd_remap(b, nblk)) { std::printf("one segment differs: trial %d\n", trial); return 1; }
-            int lc = xcd_remap_seg_chunked(b, nblk, per, st.data(), nseg, 1 + trial % 9);
-            if (lc < 0 || lc >= nblk) { std::printf("chunked out of range: trial %d block %d\n", trial, b); return 1; }
-            ++seen2[lc];
             if ((per & (per - 1)) == 0 && lb != xcd_remap_seg_lg(b, nblk, __builtin_ctz((unsigned)per), st.data(), nseg)) {
                 std::printf("shift form differs: trial %d block %d\n", trial, b);
                 return 1;
             }
         }
         for (int b = 0; b < nblk; ++b)
-            if (seen[b] != 1 || seen2[b] != 1) {
+            if (seen[b] != 1) {
                 std::printf("not a bijection: trial %d n %d per %d nseg %d\n", trial, n, per, nseg);
                 return 1;
             }
