@@ -235,7 +235,7 @@ def renumber(g: Grid, order) -> Grid:
                face_nodes=np.concatenate([fb[srtb], fi[srt]]).astype(np.int32).ravel(),
                lc=np.concatenate([lb[srtb], li[srt]]).astype(np.int32),
                rc=np.concatenate([-np.ones(nb, np.int64), ri[srt]]).astype(np.int32),
-               bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_type[srtb].copy(), n_cells=g.n_cells)
+               bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_name_id[srtb].copy(), n_cells=g.n_cells)
     out.check()
     return out
 
@@ -273,7 +273,7 @@ def _renumber_faces_nd(g: Grid, lc, rc) -> Grid:
     out = Grid(dim=g.dim, x=g.x, y=g.y, z=g.z, face_node_count=fc, face_nodes=nodes[nodes >= 0].astype(np.int32),
                lc=np.concatenate([lb[srtb], li[srt]]).astype(np.int32),
                rc=np.concatenate([-np.ones(nb, np.int64), ri[srt]]).astype(np.int32),
-               bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_type[srtb].copy(), n_cells=g.n_cells)
+               bc_type=g.bc_type[srtb].copy(), bc_name_id=g.bc_name_id[srtb].copy(), n_cells=g.n_cells)
     out.check()
     return out
 
@@ -809,7 +809,7 @@ def concat_grids(grids):
     node_off = np.concatenate([[0], np.cumsum([g.n_nodes for g in grids])])
     cell_off = np.concatenate([[0], np.cumsum([g.n_cells for g in grids])])
     bface_off = np.concatenate([[0], np.cumsum([g.n_bfaces for g in grids])])
-    fnc_b, fnc_i, fn_b, fn_i, lc_b, lc_i, rc_i, bct = [], [], [], [], [], [], [], []
+    fnc_b, fnc_i, fn_b, fn_i, lc_b, lc_i, rc_i, bct, bnm = [], [], [], [], [], [], [], [], []
     for z, g in enumerate(grids):
         nb = g.n_bfaces
         off = np.concatenate([[0], np.cumsum(g.face_node_count)])
@@ -821,6 +821,7 @@ def concat_grids(grids):
         lc_i.append(g.lc[nb:] + cell_off[z])
         rc_i.append(g.rc[nb:] + cell_off[z])
         bct.append(g.bc_type)
+        bnm.append(g.bc_name_id if g.bc_name_id is not None else g.bc_type)
     B = int(bface_off[-1])
     out = Grid(dim=dim, x=np.concatenate([g.x for g in grids]), y=np.concatenate([g.y for g in grids]),
                z=np.concatenate([g.z for g in grids]),
@@ -828,7 +829,7 @@ def concat_grids(grids):
                face_nodes=np.concatenate(fn_b + fn_i).astype(np.int32),
                lc=np.concatenate(lc_b + lc_i).astype(np.int32),
                rc=np.concatenate([-np.ones(B, np.int64)] + rc_i).astype(np.int32),
-               bc_type=np.concatenate(bct).astype(np.int32), bc_name_id=np.concatenate(bct).astype(np.int32),
+               bc_type=np.concatenate(bct).astype(np.int32), bc_name_id=np.concatenate(bnm).astype(np.int32),
                n_cells=int(cell_off[-1]))
     out.check()
     return out, cell_off.astype(np.int32), bface_off.astype(np.int32)

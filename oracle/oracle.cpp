@@ -1664,6 +1664,15 @@ int ofo_get_field(ofo_case* c, int field, double* h) {
 }
 
 int ofo_set_field(ofo_case* c, int field, const double* h) {
+    if (field == OFX_F_DQDX) {
+        const int T = c->nTCell;
+        for (int e = 0; e < 5; ++e) {
+            std::copy(h + (0 * 5 + e) * T, h + (0 * 5 + e + 1) * T, c->dqdx[e].begin());
+            std::copy(h + (1 * 5 + e) * T, h + (1 * 5 + e + 1) * T, c->dqdy[e].begin());
+            std::copy(h + (2 * 5 + e) * T, h + (2 * 5 + e + 1) * T, c->dqdz[e].begin());
+        }
+        return 0;
+    }
     int neq, n;
     RV* p = field_ptr(c, field, neq, n);
     if (!p || field == OFX_F_DIST || field == OFX_F_VOL) return fail("field not settable");

@@ -56,3 +56,26 @@ def test_fp64_gradient_matches_finite_differences():
         lm, _ = MlpOracle(pm, bf16=False).gradients(x, t)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[j]) <= 1e-6 * max(1.0, abs(fd)), (j, fd, g[j])
+
+
+def test_silu_and_derivative_point_values():
+    """silu(z) = z sigmoid(z) and its derivative at points with known values:
+    silu(0) = 0, silu'(0) = 1/2; odd-part identity silu(z) - silu(-z) = z;
+    silu'(z) + silu'(-z) = 1; ln 3 gives sigmoid = 3/4 exactly; large |z|
+    tends to z (resp. 0) with derivative 1 (resp. 0)."""
+    from oracle.mlp_oracle import dsilu, silu
+    assert silu(0.0) == 0.0 and dsilu(0.0) == 0.5
+    z = np.log(3.0)  # sigmoid(ln 3) = 3/4
+    assert abs(silu(z) - 0.75 * z) <= 1e-15
+    assert abs(dsilu(z) - 0.75 * (1.0 + z * 0.25)) <= 1e-15
+    zs = np.linspace(-6.0, 6.0, 49)
+    np.testing.assert_allclose(silu(zs) - silu(-zs), zs, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(dsilu(zs) + dsilu(-zs), 1.0, rtol=0, atol=1e-14)
+    assert abs(silu(40.0) - 40.0) <= 1e-12 and abs(silu(-40.0)) <= 1e-15
+    assert abs(dsilu(40.0) - 1.0) <= 1e-15 and abs(dsilu(-40.0)) <= 1e-15
+    # the derivative is that of silu: central differences at sample points
+    h = 1e-6
+    np.testing.assert_allclose(dsilu(zs), (silu(zs + h) - silu(zs - h)) / (2 * h), rtol=0, atol=1e-8)
+    # the minimum of silu: silu'(z*) = 0 where 1 + z* + exp(z*) = 0, z* = -1 - W(1/e)
+    zstar = -1.2784645427610738
+    assert abs(dsilu(zstar)) <= 1e-14
