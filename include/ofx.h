@@ -62,6 +62,8 @@ typedef struct ofx_config {
     int32_t turb_ilim;         /* SA reconstruction (only -1 supported)       */
     int32_t tns_ilim;          /* NS state used by SA convection (only -1)    */
     int32_t flux_fp32;         /* 1: FP32 face fluxes, FP64 accumulation      */
+    int32_t time_integral;     /* 1 multi-stage (Runge-Kutta), 2 LU-SGS       */
+    int32_t rk_stage;          /* stages of the multi-stage scheme (>= 1)     */
     double  vencat_coef;
     double  centropy1, centropy2;
     double  cflst, cfled, max_time_ratio;
@@ -173,9 +175,21 @@ int ofx_lusgs_ns(ofx_ctx* ctx);
 int ofx_lusgs_sa(ofx_ctx* ctx);
 
 /* UPDATE_FLOWFIELD_LUSGS (codes/uns/src/UNsUpdate.cpp:46-146,
- * codes/uturb/src/UTurbUpdate.cpp:85-120). */
+ * codes/uturb/src/UTurbUpdate.cpp:85-120), with CALC_BOUNDARY of the same
+ * solver folded in.  With time_integral = 1 (multi-stage) this is
+ * UPDATE_FLOWFIELD: the increment is the residual field itself, as the
+ * reference binds dq to res under MULTI_STAGE (codes/uns/src/UNsCom.cpp:85-88,
+ * codes/uturb/src/UTurbCom.cpp:79-82), i.e. res / turbres after ofx_lhs. */
 int ofx_update_ns(ofx_ctx* ctx);
 int ofx_update_sa(ofx_ctx* ctx);
+
+/* CALC_LHS of the multi-stage scheme: ULhs::CalcLHS
+ * (codes/usolver/src/ULhs.cpp:47-62, registered at
+ * codes/lhs/src/LhsTaskReg.cpp:38-44).  For every inner cell c and equation
+ * e of the solver (0 NS: res[5], 1 SA: turbres[1]):
+ * r[e][c] *= (dt[c] / vol[c]) * lhscoef, dt the time step of the last
+ * ofx_timestep (after ModifyTimeStep).  Ghost entries are left alone. */
+int ofx_lhs(ofx_ctx* ctx, int solver, double lhscoef);
 
 /* Residual norms of the current res / turbres fields (DUMP_RESIDUAL,
  * codes/residual/src/ResidualTask.cpp:75-126). */
@@ -184,8 +198,14 @@ int ofx_norms_get(ofx_ctx* ctx, ofx_norms* out);
 /* n whole pseudo-time steps starting at step `first_iter` (1-based outer
  * step, drives the CFL ramp), the task sequence of TimeIntegral::Lusgs for
  * NS then SA (codes/solver/src/TimeIntegral.cpp:113-130,
- * codes/multigrid/src/Multigrid.cpp:339-375).  norms_out (may be NULL)
- * receives n entries. */
+ * codes/multigrid/src/Multigrid.cpp:339-375).  With time_integral = 1 each
+ * solver instead runs TimeIntegral::RungeKutta
+ * (codes/solver/src/TimeIntegral.cpp:81-110): CALC_TIME_STEP, then per stage
+ * s = 0..rk_stage-1 with lhscoef = 1/(s+1) (codes/global/src/Ctrl.cpp:74-79)
+ * UPDATE_RESIDUALS, CALC_LHS, UPDATE_FLOWFIELD and CALC_BOUNDARY (LOAD_Q and
+ * LOAD_RESIDUALS only move multigrid storage and leave a one-level run's
+ * fields as they are).  norms_out (may be NULL) receives n entries, the norms
+ * of res / turbres as the step leaves them (DUMP_RESIDUAL runs after it). */
 int ofx_iterate(ofx_ctx* ctx, int first_iter, int n, ofx_norms* norms_out);
 
 /* Wall force coefficients of the current state: out[0..3] = CL, CD, CDp, CDf

@@ -30,6 +30,8 @@ class OfxConfig(ctypes.Structure):
         ("turb_ilim", ctypes.c_int32),
         ("tns_ilim", ctypes.c_int32),
         ("flux_fp32", ctypes.c_int32),
+        ("time_integral", ctypes.c_int32),
+        ("rk_stage", ctypes.c_int32),
         ("vencat_coef", ctypes.c_double),
         ("centropy1", ctypes.c_double),
         ("centropy2", ctypes.c_double),
@@ -72,6 +74,8 @@ class CaseConfig:
     turb_ilim: int = -1
     tns_ilim: int = -1
     flux_fp32: int = 0
+    time_integral: int = 2
+    rk_stage: int = 1
     vencat_coef: float = 0.0001
     centropy1: float = 0.0001
     centropy2: float = 0.0001

@@ -265,6 +265,10 @@ int ofx_config_set(ofx_ctx* c, const ofx_config* cfg) {
     if (cfg->turb_ilim != -1 || cfg->tns_ilim != -1) { set_error("only turb_ilim = tns_ilim = -1 supported"); return OFX_EINVAL; }
     if (cfg->timestepModel != 0) { set_error("only local time stepping supported"); return OFX_EINVAL; }
     if (cfg->flux_fp32 != 0 && cfg->flux_fp32 != 1) { set_error("flux_fp32 must be 0 or 1"); return OFX_EINVAL; }
+    if (cfg->time_integral != 1 && cfg->time_integral != 2) {
+        set_error("time_integral must be 1 (multi-stage) or 2 (LU-SGS)"); return OFX_EINVAL;
+    }
+    if (cfg->time_integral == 1 && cfg->rk_stage < 1) { set_error("rk_stage must be >= 1"); return OFX_EINVAL; }
     c->cfg = *cfg;
     set_consts(c);
     c->has_cfg = true;
@@ -364,6 +368,27 @@ int ofx_lusgs_ns(ofx_ctx* c) { CHECK_READY(c); return ns_lusgs(c); }
 int ofx_lusgs_sa(ofx_ctx* c) { CHECK_READY(c); return sa_lusgs(c); }
 int ofx_update_ns(ofx_ctx* c) { CHECK_READY(c); return ns_update(c); }
 int ofx_update_sa(ofx_ctx* c) { CHECK_READY(c); return sa_update(c); }
+int ofx_lhs(ofx_ctx* c, int solver, double lhscoef) {
+    CHECK_READY(c);
+    if (solver != 0 && solver != 1) { set_error("lhs: solver must be 0 (NS) or 1 (SA)"); return OFX_EINVAL; }
+    return lhs(c, solver, lhscoef);
+}
+
+}  // extern "C"
+
+namespace ofx {
+// OFX_ENAN if the current residual (NS or SA) holds a NaN.
+int nan_stop(ofx_ctx* c) {
+    ofx_norms nm;
+    OFX_TRY(norms(c, &nm));
+    bool bad = std::isnan(nm.sa);
+    for (int e = 0; e < 5; ++e) bad = bad || std::isnan(nm.ns[e]);
+    if (bad) { set_error("NotANumber in the residual"); return OFX_ENAN; }
+    return 0;
+}
+}  // namespace ofx
+
+extern "C" {
 
 int ofx_norms_get(ofx_ctx* c, ofx_norms* out) {
     CHECK_READY(c);
@@ -384,6 +409,24 @@ namespace ofx {
 // One pseudo-time step, kernels only (no host synchronisation, no memset), so
 // it can also be recorded into a HIP graph.
 int step_launch(ofx_ctx* c, int iter) {
+    if (c->cfg.time_integral == 1) {
+        // TimeIntegral::RungeKutta of NS, then of SA (codes/solver/src/TimeIntegral.cpp:81-110);
+        // the SA solver registers no CALC_TIME_STEP and uses the NS time step
+        OFX_TRY(ns_timestep(c, iter));
+        for (int s = 0; s < c->cfg.rk_stage; ++s) {
+            OFX_TRY(ns_residual(c));
+            OFX_TRY(lhs(c, 0, 1.0 / (s + 1)));
+            OFX_TRY(ns_update_fused(c));
+        }
+        if (c->k.vismodel == 3) {
+            for (int s = 0; s < c->cfg.rk_stage; ++s) {
+                OFX_TRY(sa_residual(c));
+                OFX_TRY(lhs(c, 1, 1.0 / (s + 1)));
+                OFX_TRY(sa_update_fused(c));
+            }
+        }
+        return 0;
+    }
     OFX_TRY(ns_timestep(c, iter));
     // the first lower-sweep level is computed by the residual gathers when the
     // schedule allows it (fusable_l0), and the sweeps start at level 1
@@ -417,6 +460,10 @@ int ofx_iterate(ofx_ctx* c, int first_iter, int n, ofx_norms* norms_out) {
         OFX_TRY(step_launch(c, first_iter + s));
         if (norms_out) OFX_TRY(ofx_norms_get(c, norms_out + s));
     }
+    // Without per-step norms the NaN stop of DUMP_RESIDUAL
+    // (codes/residual/src/ResidualTask.cpp:95-100) is checked once, on the
+    // residual of the last step: a NaN state keeps producing NaN residuals.
+    if (!norms_out && n > 0) OFX_TRY(nan_stop(c));
     OFX_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -635,7 +682,7 @@ int ofx_zones_set(ofx_ctx* c, int n, const int* cellStart, const int* bfaceStart
 int ofx_zones_norms(ofx_ctx* c, ofx_norms* out) {
     CHECK_READY(c);
     if (!out) { set_error("null out"); return OFX_EINVAL; }
-    if (c->m.nz == 1) return ofx_norms_get(c, out);
+    if (c->m.nz == 1) return norms(c, out);
     for (int z = 0; z < c->m.nz; ++z) OFX_TRY(norms_range(c, c->zc[z], c->zc[z + 1], out + z));
     return 0;
 }

@@ -166,6 +166,14 @@ int ofx_batch_iterate(ofx_batch* b, int first_iter, int n, int* graph_launches) 
     }
     OFX_TRY(sync_cases(b));
     if (graph_launches) *graph_launches = launches;
+    // the NaN stop of DUMP_RESIDUAL (codes/residual/src/ResidualTask.cpp:95-100),
+    // once per call on each case's last residual
+    if (n > 0)
+        for (size_t i = 0; i < b->ctxs.size(); ++i) {
+            int rc = nan_stop(b->ctxs[i]);
+            if (rc == OFX_ENAN) set_error("NotANumber in the residual of batch case " + std::to_string(i));
+            if (rc) return rc;
+        }
     return 0;
 }
 

@@ -568,6 +568,7 @@ int sa_lusgs(ofx_ctx* c, int lo_first = 0);
 int ns_update(ofx_ctx* c);
 int sa_update(ofx_ctx* c);
 int norms(ofx_ctx* c, ofx_norms* out);
+int nan_stop(ofx_ctx* c);
 int wall_distance(ofx_ctx* c);
 inline int ensure_dist(ofx_ctx* c) { return c->dist_valid ? 0 : wall_distance(c); }
 int norms_range(ofx_ctx* c, int c0, int c1, ofx_norms* out);
@@ -577,6 +578,17 @@ int fill_ghost_grad(ofx_ctx* c);
 int materialize_timestep(ofx_ctx* c);
 int ns_update_fused(ofx_ctx* c);
 int sa_update_fused(ofx_ctx* c);
+int lhs(ofx_ctx* c, int solver, double lhscoef);
+
+// The fields the update kernels read: under the multi-stage scheme the
+// increment is the residual itself (UNsField::Init and UTurbField::Init bind
+// dq to res when time_integral is MULTI_STAGE, codes/uns/src/UNsCom.cpp:85-88,
+// codes/uturb/src/UTurbCom.cpp:79-82).
+inline DevFields update_fields(const ofx_ctx* c) {
+    DevFields d = c->d;
+    if (c->cfg.time_integral == 1) { d.dq = d.res; d.turbdq = d.turbres; }
+    return d;
+}
 int aero(ofx_ctx* c, double* out4);
 int init_freestream(ofx_ctx* c);
 

@@ -1256,8 +1256,31 @@ __device__ void ns_solution_fix(const DevMesh& m, const DevFields& d, int i, dou
     for (int s = 0; s < 5; ++s) prim[s] *= rVol;
 }
 
+// NsCalcBoundary of one updated cell: its temperature and laminar viscosity,
+// and the ghost states of its boundary faces (boundary faces come first in
+// the cell's list; entries of faces where it is the right cell are ~f < 0).
+__device__ __forceinline__ void ns_cell_boundary(const DevMesh& m, const DevFields& d, const Consts& k, int i,
+                                                 const double* prim) {
+    const Consts kz = zoned(k, m, cell_zone(m, i));
+    double t = temperature(prim[0], prim[4], kz.statecoef);
+    d.tempr[i] = t;
+    d.visl[i] = sutherland(t, kz.ts_suth);
+    for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
+        int f = m.c2f[kk];
+        if (f < 0 || f >= m.B) break;
+        ns_bc_face(m, d, kz, f, prim, t);
+    }
+}
+
 // Pass 2: UPDATE_FLOWFIELD; with BC also the following CALC_BOUNDARY
 // (NsCalcBoundary): T / viscosity of the cell and its boundary faces' ghosts.
+// When some cell needs SolutionFix, the reference's serial loop gives its
+// neighbour average the half-updated state: lower-numbered neighbours
+// updated, higher-numbered ones not.  The lower-sweep levels of the LU-SGS
+// schedule have exactly that order (a cell's lower-numbered neighbours lie in
+// earlier levels, its higher-numbered ones in later levels), so block 0 walks
+// them in order, a barrier between levels, updating each level's cells in
+// parallel in place; the boundary pass follows once every cell is final.
 template <bool BC>
 __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, Consts k, int par) {
     const int T = m.T;
@@ -1268,54 +1291,66 @@ __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, 
         double prim[5];
         ns_update_cell(d, k, T, i, prim);
         for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
-        if (BC) {
-            const Consts kz = zoned(k, m, cell_zone(m, i));
-            double t = temperature(prim[0], prim[4], kz.statecoef);
-            d.tempr[i] = t;
-            d.visl[i] = sutherland(t, kz.ts_suth);
-            for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
-                int f = m.c2f[kk];
-                // boundary faces (owner side, f < B) come first; entries of faces
-                // where the cell is the right cell are stored as ~f < 0
-                if (f < 0 || f >= m.B) break;
-                ns_bc_face(m, d, kz, f, prim, t);
-            }
-        }
+        if (BC) ns_cell_boundary(m, d, k, i, prim);
         return;
     }
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int i = 0; i < m.C; ++i) {
-        double prim[5];
-        if (!ns_update_cell(d, k, T, i, prim)) ns_solution_fix(m, d, i, prim);
-        for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
+    if (blockIdx.x != 0) return;
+    for (int L = 0; L < m.nLevLo; ++L) {
+        for (int idx = m.loOff[L] + threadIdx.x; idx < m.loOff[L + 1]; idx += blockDim.x) {
+            const int i = m.loOrder[idx];
+            double prim[5];
+            if (!ns_update_cell(d, k, T, i, prim)) ns_solution_fix(m, d, i, prim);
+            for (int e = 0; e < 5; ++e) d.q[e * T + i] = prim[e];
+        }
+        __syncthreads();
     }
     if (BC) {
-        for (int i = 0; i < m.C; ++i) {
-            double t = temperature(d.q[i], d.q[4 * T + i], zoned(k, m, cell_zone(m, i)).statecoef);
-            d.tempr[i] = t;
-            d.visl[i] = sutherland(t, k.ts_suth);
-        }
-        for (int f = 0; f < m.B; ++f) {
-            int l = m.lc[f];
-            double s[5];
-            for (int e = 0; e < 5; ++e) s[e] = d.q[e * T + l];
-            ns_bc_face(m, d, zoned(k, m, face_zone(m, f)), f, s, d.tempr[l]);
+        for (int i = threadIdx.x; i < m.C; i += blockDim.x) {
+            double prim[5];
+            for (int e = 0; e < 5; ++e) prim[e] = d.q[e * T + i];
+            ns_cell_boundary(m, d, k, i, prim);
         }
     }
 }
 
 int ns_update(ofx_ctx* c) {
+    const DevFields d = update_fields(c);
     c->par_fix ^= 1;
-    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
-    k_ns_update_apply<false><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
+    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
+    k_ns_update_apply<false><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
     OFX_HIP(hipGetLastError());
     return 0;
 }
 
 int ns_update_fused(ofx_ctx* c) {
+    const DevFields d = update_fields(c);
     c->par_fix ^= 1;
-    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
-    k_ns_update_apply<true><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->par_fix);
+    k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
+    k_ns_update_apply<true><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+// CALC_LHS, ULhs::CalcLHS (codes/usolver/src/ULhs.cpp:47-62):
+// r[e][c] *= (dt / vol) * lhscoef over the inner cells, with the time step as
+// ModifyTimeStep leaves it (the clamp is applied here, where it is consumed).
+template <int NEQ>
+__global__ void __launch_bounds__(TB) k_lhs(DevMesh m, DevFields d, Consts k, double* r, double lhscoef, int par) {
+    double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
+    int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    mindt = zone_mindt(m, d, i, mindt);
+    const double dt = clamped_dt(d, k, i, mindt);
+    const double s = (dt / m.vol[i]) * lhscoef;
+#pragma unroll
+    for (int e = 0; e < NEQ; ++e) r[(size_t)e * m.T + i] *= s;
+}
+
+int lhs(ofx_ctx* c, int solver, double lhscoef) {
+    if (solver == 0)
+        k_lhs<5><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->d.res, lhscoef, c->par_dt);
+    else if (c->k.vismodel == 3)
+        k_lhs<1><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k, c->d.turbres, lhscoef, c->par_dt);
     OFX_HIP(hipGetLastError());
     return 0;
 }

@@ -528,14 +528,14 @@ __global__ void __launch_bounds__(TBS) k_sa_update_fused(DevMesh m, DevFields d,
 
 int sa_update_fused(ofx_ctx* c) {
     if (c->k.vismodel != 3) return 0;
-    k_sa_update_fused<<<nblks(c->m.C), TBS, 0, c->stream>>>(c->m, c->d, c->k);
+    k_sa_update_fused<<<nblks(c->m.C), TBS, 0, c->stream>>>(c->m, update_fields(c), c->k);
     OFX_HIP(hipGetLastError());
     return 0;
 }
 
 int sa_update(ofx_ctx* c) {
     if (c->k.vismodel != 3) return 0;
-    k_sa_update<<<nblks(c->m.C), TBS, 0, c->stream>>>(c->m, c->d);
+    k_sa_update<<<nblks(c->m.C), TBS, 0, c->stream>>>(c->m, update_fields(c));
     OFX_HIP(hipGetLastError());
     return 0;
 }

@@ -29,7 +29,14 @@ class OfxNorms(ctypes.Structure):
 
 
 class OfxError(RuntimeError):
-    pass
+    """A libofx call failed; `code` is the OFX_E* return value."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+OFX_ENAN = -5
 
 
 _lib = None
@@ -59,6 +66,7 @@ def load_library():
     L.ofx_get_field.argtypes = [P, ctypes.c_int, dp]
     L.ofx_set_field.argtypes = [P, ctypes.c_int, dp]
     L.ofx_timestep.argtypes = [P, ctypes.c_int]
+    L.ofx_lhs.argtypes = [P, ctypes.c_int, ctypes.c_double]
     L.ofx_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
     L.ofx_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
     L.ofx_aero.argtypes = [P, dp]
@@ -114,7 +122,7 @@ class Solver:
 
     def _check(self, rc):
         if rc != 0:
-            raise OfxError(f"libofx error {rc}: {self._L.ofx_last_error().decode()}")
+            raise OfxError(f"libofx error {rc}: {self._L.ofx_last_error().decode()}", rc)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -173,12 +181,18 @@ class Solver:
     def update_sa(self):              # UPDATE_FLOWFIELD_LUSGS (SA)
         self._check(self._L.ofx_update_sa(self.h))
 
+    def lhs(self, solver, lhscoef):  # CALC_LHS (multi-stage): solver 0 NS, 1 SA
+        self._check(self._L.ofx_lhs(self.h, solver, lhscoef))
+
     def sync(self):
         self._check(self._L.ofx_sync(self.h))
 
-    def norms(self):                  # DUMP_RESIDUAL
+    def norms(self, check=True):      # DUMP_RESIDUAL
+        """Residual norms [ns0..ns4, sa]; a NaN raises (OFX_ENAN) unless check is False."""
         n = OfxNorms()
-        self._check(self._L.ofx_norms_get(self.h, ctypes.byref(n)))
+        rc = self._L.ofx_norms_get(self.h, ctypes.byref(n))
+        if check or rc != OFX_ENAN:
+            self._check(rc)
         return np.array(list(n.ns) + [n.sa])
 
     def iterate(self, first_iter, n, with_norms=True):
@@ -258,7 +272,7 @@ class Batch:
 
     def _check(self, rc):
         if rc != 0:
-            raise OfxError(f"ofx error {rc}: {self._L.ofx_last_error().decode()}")
+            raise OfxError(f"ofx error {rc}: {self._L.ofx_last_error().decode()}", rc)
 
     def iterate(self, first_iter, n):
         """Advance every case n steps; returns the number of graph replays used."""

@@ -143,20 +143,49 @@ class ZoneView:
         return self.zs.get_zone(name, self.z)
 
 
+CONVERGED, MAX_STEPS, DIVERGED = "converged", "max_steps", "diverged"
+
+
+def _iterate_nan_tolerant(solver, it, n):
+    """solver.iterate(it, n) without norms; a NaN residual (OFX_ENAN, checked
+    after the last step) is left to the caller's per-case norms."""
+    from .solver import OFX_ENAN, OfxError
+    try:
+        solver.iterate(it, n, with_norms=False)
+    except OfxError as e:
+        if e.code != OFX_ENAN:
+            raise
+
+
+def _case_status(norm, limit, it, max_steps):
+    """Status of a case after step it-1, or None while it keeps running."""
+    if not np.isfinite(norm):
+        return DIVERGED
+    if norm <= limit:
+        return CONVERGED
+    if it > max_steps:
+        return MAX_STEPS
+    return None
+
+
 def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None, rebuild=None,
                              compact=0.5):
     """All cases as zones of one context advance together until every
-    zone's density residual norm is below tol * refs[case].  With `rebuild`
-    (case ids -> a new ZoneSolver for those cases), converged cases leave:
-    once the unconverged cases are at most `compact` of the context's zones,
-    their state moves into a new context holding only them (ghosts and
-    viscosities recomputed by the boundary tasks), so converged cases stop
-    costing work.  Returns (wall seconds including any rebuilds, step at which
-    each case first met its tolerance, forces [CL, CD, CDp, CDf] of each case
-    at that step)."""
+    zone's density residual norm is below tol * refs[case], its norm is not
+    finite (diverged: the reference stops on NaN,
+    codes/residual/src/ResidualTask.cpp:95-100) or max_steps is passed.
+    With `rebuild` (case ids -> a new ZoneSolver for those cases), finished
+    cases leave: once the running cases are at most `compact` of the
+    context's zones, or some case diverged, their state moves into a new
+    context holding only them (ghosts and viscosities recomputed by the
+    boundary tasks), so finished cases stop costing work.  Returns (wall
+    seconds including any rebuilds, step at which each case finished, forces
+    [CL, CD, CDp, CDf] of each case at that step (None if diverged), status
+    of each case: CONVERGED, MAX_STEPS or DIVERGED)."""
     n = zs.n_zones
     steps = [0] * n
     aero = [None] * n
+    status = [None] * n
     zs.sync()
     t0 = time.perf_counter()
     ramp = zs.cfg.ncfl
@@ -164,21 +193,29 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
     cur, ids = zs, list(range(n))
     while True:
         nstep = chunk + (ramp - it if it < ramp else 0)
-        cur.iterate(it, nstep, with_norms=False)
+        _iterate_nan_tolerant(cur, it, nstep)
         it += nstep
         nr = cur.zone_norms()[:, 0]
         za = None
+        diverged = False
         for z, cid in enumerate(ids):
-            if not steps[cid] and (nr[z] <= tol * refs[cid] or it > max_steps):
+            if status[cid] is None:
+                st = _case_status(nr[z], tol * refs[cid], it, max_steps)
+                if st is None:
+                    continue
+                status[cid] = st
                 steps[cid] = it - 1
-                za = cur.zone_aero() if za is None else za
-                aero[cid] = za[z].tolist()
+                if st == DIVERGED:
+                    diverged = True
+                else:
+                    za = cur.zone_aero() if za is None else za
+                    aero[cid] = za[z].tolist()
                 if progress:
                     progress(cid, steps[cid])
-        live = [cid for cid in ids if not steps[cid]]
+        live = [cid for cid in ids if status[cid] is None]
         if not live:
             break
-        if rebuild is not None and len(live) < len(ids) and len(live) <= compact * len(ids):
+        if rebuild is not None and len(live) < len(ids) and (diverged or len(live) <= compact * len(ids)):
             new = rebuild(live)
             q, nu = cur.get("Q"), cur.get("TURBQ")
             qn, nun = new.get("Q"), new.get("TURBQ")
@@ -196,19 +233,21 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
             cur, ids = new, live
     if cur is not zs:
         cur.close()
-    return time.perf_counter() - t0, steps, aero
+    return time.perf_counter() - t0, steps, aero, status
 
 
 def solve_to_tolerance(solvers, refs, tol=1e-8, chunk=500, max_steps=150_000, batched=True, steps_per_graph=10,
                        progress=None):
     """Iterate every case until its density residual norm falls below
-    tol * refs[i] (an absolute level per case).  Batched: all unconverged
-    cases advance together in one recorded graph; converged cases leave the
-    batch.  Sequential: one case after another.  `progress(case, steps)` is
-    called as each case finishes.  Returns (wall seconds, steps per case)."""
+    tol * refs[i] (an absolute level per case), is not finite (diverged) or
+    max_steps is passed.  Batched: all running cases advance together in one
+    recorded graph; finished cases leave the batch.  Sequential: one case
+    after another.  `progress(case, steps)` is called as each case finishes.
+    Returns (wall seconds, steps per case, status per case)."""
+    from .solver import OFX_ENAN, OfxError
     n = len(solvers)
     steps = [0] * n
-    done = [False] * n
+    status = [None] * n
     for s in solvers:
         s.sync()
     t0 = time.perf_counter()
@@ -223,29 +262,38 @@ def solve_to_tolerance(solvers, refs, tol=1e-8, chunk=500, max_steps=150_000, ba
                 # after the ramp, whole graphs only: no leftover step moves the
                 # reduction parities, so the recorded graph stays valid
                 nstep = chunk + (ramp - it if it < ramp else 0)
-                b.iterate(it, nstep)
+                try:
+                    b.iterate(it, nstep)
+                except OfxError as e:
+                    if e.code != OFX_ENAN:
+                        raise
                 it += nstep
-                nr = b.norms()[:, 0]
-                hit = [i for i, r in zip(live, nr) if r <= tol * refs[i] or it > max_steps]
-                for i in live:
+                nr = [solvers[i].norms(check=False)[0] for i in live]
+                hit = []
+                for i, r in zip(live, nr):
                     steps[i] = it - 1
+                    st = _case_status(r, tol * refs[i], it, max_steps)
+                    if st is not None:
+                        status[i] = st
+                        hit.append(i)
                 if hit:
-                    for i in hit:
-                        done[i] = True
-                        if progress:
+                    if progress:
+                        for i in hit:
                             progress(i, steps[i])
-                    live = [i for i in live if not done[i]]
+                    live = [i for i in live if status[i] is None]
                     break
             b.close()
     else:
         for i, s in enumerate(solvers):
             it = 1
             while True:
-                s.iterate(it, chunk, with_norms=False)
+                _iterate_nan_tolerant(s, it, chunk)
                 it += chunk
-                if s.norms()[0] <= tol * refs[i] or it > max_steps:
+                st = _case_status(s.norms(check=False)[0], tol * refs[i], it, max_steps)
+                if st is not None:
+                    status[i] = st
                     break
             steps[i] = it - 1
             if progress:
                 progress(i, steps[i])
-    return time.perf_counter() - t0, steps
+    return time.perf_counter() - t0, steps, status

@@ -1256,10 +1256,14 @@ void ns_lusgs(ofo_case& c) {
 
 // UNsUpdate::UpdateFlowField + NsUpdate::CalcFlowField/WeekSolutionFix + SolutionFix
 // (codes/uns/src/UNsUpdate.cpp:46-146, codes/ns/src/NsUpdate.cpp:40-67,192-217)
+// Under the multi-stage scheme the increment is the residual field itself:
+// UNsField::Init binds dq to res when time_integral is MULTI_STAGE
+// (codes/uns/src/UNsCom.cpp:85-88).
 void ns_update(ofo_case& c) {
+    const RV* inc = c.cfg.time_integral == 1 ? c.res : c.dq;
     for (int i = 0; i < c.nCells; ++i) {
         double prim0[5], q0[5], q[5], dq[5], prim[5];
-        for (int e = 0; e < 5; ++e) { prim0[e] = c.q[e][i]; dq[e] = c.dq[e][i]; }
+        for (int e = 0; e < 5; ++e) { prim0[e] = c.q[e][i]; dq[e] = inc[e][i]; }
         double gama = c.gama[i];
         prim_to_q(prim0, gama, q0);
         for (int e = 0; e < 5; ++e) q[e] = q0[e] + dq[e];
@@ -1512,14 +1516,29 @@ void sa_lusgs(ofo_case& c) {
     }
 }
 
-// TurbUpdate::CalcFlowField1Equ, crelax = 1 (codes/turb/src/TurbUpdate.cpp:58-69)
+// TurbUpdate::CalcFlowField1Equ, crelax = 1 (codes/turb/src/TurbUpdate.cpp:58-69);
+// the increment is turbres under the multi-stage scheme (codes/uturb/src/UTurbCom.cpp:79-82)
 void sa_update(ofo_case& c) {
+    const RV& inc = c.cfg.time_integral == 1 ? c.turbres : c.turbdq;
     for (int i = 0; i < c.nCells; ++i) {
-        double qOld = c.turbq[i], dqNew = c.turbdq[i];
+        double qOld = c.turbq[i], dqNew = inc[i];
         double qNew = qOld + dqNew;
         qNew = (1.0 - 1.0) * qOld + 1.0 * qNew;
         c.turbq[i] = qNew;
     }
+}
+
+// ULhs::CalcLHS (codes/usolver/src/ULhs.cpp:47-62): the residual of each
+// equation scaled by the local time step over the cell volume and lhscoef,
+// inner cells only.
+void calc_lhs(ofo_case& c, int solver, double lhscoef) {
+    RV* r = solver == 0 ? c.res : &c.turbres;
+    int nEqu = solver == 0 ? 5 : 1;
+    for (int e = 0; e < nEqu; ++e)
+        for (int i = 0; i < c.nCells; ++i) {
+            double dt = c.timestep[i];
+            r[e][i] *= (dt / c.vol[i]) * lhscoef;
+        }
 }
 
 bool has_nan(const RV& v, int n) {
@@ -1583,6 +1602,8 @@ int ofo_mesh_info(ofo_case* c, int* out5) {
 }
 
 int ofo_config_set(ofo_case* c, const ofx_config* cfg) {
+    if (cfg->time_integral != 1 && cfg->time_integral != 2) return fail("time_integral must be 1 or 2");
+    if (cfg->time_integral == 1 && cfg->rk_stage < 1) return fail("rk_stage must be >= 1");
     if (cfg->ischeme != 1 && cfg->ischeme != 12) return fail("ischeme must be 1 (Roe) or 12 (SLAU2)");
     if (cfg->ilim < -1 || cfg->ilim > 2) return fail("ilim must be -1..2");
     if (cfg->vismodel != 0 && cfg->vismodel != 1 && cfg->vismodel != 3) return fail("vismodel must be 0, 1 or 3");
@@ -1729,9 +1750,50 @@ int ofo_norms_get(ofo_case* c, ofx_norms* out) {
 }
 
 // One StrongIter per step: NS then SA, each TimeIntegral::Lusgs
+int ofo_lhs(ofo_case* c, int solver, double lhscoef) {
+    if (solver != 0 && solver != 1) return fail("lhs: solver must be 0 or 1");
+    if (solver == 1 && c->cfg.vismodel != 3) return 0;
+    calc_lhs(*c, solver, lhscoef);
+    return 0;
+}
+
+// TimeIntegral::RungeKutta of the NS solver, then of the SA solver
+// (codes/solver/src/TimeIntegral.cpp:81-110, one grid level): CALC_TIME_STEP,
+// then per stage with lhscoef = rk_coef[stage] = 1/(stage+1)
+// (codes/global/src/Ctrl.cpp:74-79) UPDATE_RESIDUALS, CALC_LHS,
+// UPDATE_FLOWFIELD, CALC_BOUNDARY.  LOAD_Q and LOAD_RESIDUALS copy into
+// multigrid storage that a one-level run never reads back.  The SA solver
+// registers no CALC_TIME_STEP (system/turb/function/funcMap.txt).
+static int rk_step(ofo_case* c, int it) {
+    ofo_timestep(c, it);
+    for (int s = 0; s < c->cfg.rk_stage; ++s) {
+        int rc = ofo_residual_ns(c);
+        if (rc) return rc;
+        calc_lhs(*c, 0, 1.0 / (s + 1));
+        ns_update(*c);
+        ns_boundary(*c);
+    }
+    if (c->cfg.vismodel == 3) {
+        for (int s = 0; s < c->cfg.rk_stage; ++s) {
+            int rc = ofo_residual_sa(c);
+            if (rc) return rc;
+            calc_lhs(*c, 1, 1.0 / (s + 1));
+            sa_update(*c);
+            sa_vist(*c); sa_bc(*c);
+        }
+    }
+    return 0;
+}
+
 int ofo_iterate(ofo_case* c, int first_iter, int n, ofx_norms* norms_out) {
     for (int k = 0; k < n; ++k) {
         int it = first_iter + k;
+        if (c->cfg.time_integral == 1) {
+            int rc = rk_step(c, it);
+            if (rc) return rc;
+            if (norms_out) ofo_norms_get(c, norms_out + k);
+            continue;
+        }
         ofo_timestep(c, it);
         int rc = ofo_residual_ns(c);
         if (rc) return rc;

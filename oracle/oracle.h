@@ -42,6 +42,7 @@ int  ofo_lusgs_ns(ofo_case* c);
 int  ofo_lusgs_sa(ofo_case* c);
 int  ofo_update_ns(ofo_case* c);
 int  ofo_update_sa(ofo_case* c);
+int  ofo_lhs(ofo_case* c, int solver, double lhscoef);
 int  ofo_norms_get(ofo_case* c, ofx_norms* out);
 int  ofo_iterate(ofo_case* c, int first_iter, int n, ofx_norms* norms_out);
 int  ofo_aero(ofo_case* c, double* out4);

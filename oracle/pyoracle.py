@@ -55,6 +55,7 @@ def lib():
         L.ofo_get_field.argtypes = [P, ctypes.c_int, dp]
         L.ofo_set_field.argtypes = [P, ctypes.c_int, dp]
         L.ofo_timestep.argtypes = [P, ctypes.c_int]
+        L.ofo_lhs.argtypes = [P, ctypes.c_int, ctypes.c_double]
         L.ofo_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
         L.ofo_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
         L.ofo_aero.argtypes = [P, dp]
@@ -148,6 +149,9 @@ class OracleCase:
 
     def update_sa(self):
         self._check(lib().ofo_update_sa(self.h))
+
+    def lhs(self, solver, lhscoef):
+        self._check(lib().ofo_lhs(self.h, solver, lhscoef))
 
     def norms(self):
         n = OfxNorms()
