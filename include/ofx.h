@@ -70,6 +70,7 @@ typedef struct ofx_config {
     double  mach_ref, aoa_degree, sideslip_degree, reynolds;
     double  gama_ref, prl, prt, tref_dim, twall_dim;
     double  ref_sa, max_vis_ratio, rprod, turb_cfl_ratio;
+    double  aref;              /* reference area of the force coefficients    */
 } ofx_config;
 
 /* Field identifiers for ofx_get_field / ofx_set_field.  Sizes in doubles. */
@@ -211,6 +212,16 @@ int ofx_iterate(ofx_ctx* ctx, int first_iter, int n, ofx_norms* norms_out);
 /* Wall force coefficients of the current state: out[0..3] = CL, CD, CDp, CDf
  * (codes/force/src/AeroForceTask.cpp:186-380). */
 int ofx_aero(ofx_ctx* ctx, double* out4);
+
+/* The value columns of one aero.dat row (AerodynamicForceTask::Dump,
+ * codes/force/src/AeroForceTask.cpp:96-150, with AeroForceInfo::CalcCoef,
+ * codes/force/src/Force.cpp:259-276): out[0..11] = CL, CD, CD_PR, CD_SF,
+ * CDL2 (= CD - CL^2 / (pi 9.5)), Xcp, Fx, Fy, Fz, Cmx, Cmy, Cmz, where
+ * F = total wall force / aref.  The reference's AeroForce::AddForce
+ * (codes/force/src/Force.cpp:202-208) sums no moments, so its moment
+ * coefficients are zero and Xcp is +-0 by the sign of Fy; these columns
+ * reproduce that. */
+int ofx_aero_row(ofx_ctx* ctx, double* out12);
 
 /* DUMP_PRESSURE_COEFF (the idump_pres rows of CalcAeroForce,
  * codes/force/src/AeroForceTask.cpp:242-262,426-440, written to wallaeroFile):

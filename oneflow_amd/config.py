@@ -51,6 +51,7 @@ class OfxConfig(ctypes.Structure):
         ("max_vis_ratio", ctypes.c_double),
         ("rprod", ctypes.c_double),
         ("turb_cfl_ratio", ctypes.c_double),
+        ("aref", ctypes.c_double),
     ]
 
 
@@ -95,6 +96,7 @@ class CaseConfig:
     max_vis_ratio: float = 1.0e6
     rprod: float = 1.0
     turb_cfl_ratio: float = 1.0
+    aref: float = 1.0
 
     def to_c(self) -> OfxConfig:
         c = OfxConfig()

@@ -265,6 +265,7 @@ int ofx_config_set(ofx_ctx* c, const ofx_config* cfg) {
     if (cfg->turb_ilim != -1 || cfg->tns_ilim != -1) { set_error("only turb_ilim = tns_ilim = -1 supported"); return OFX_EINVAL; }
     if (cfg->timestepModel != 0) { set_error("only local time stepping supported"); return OFX_EINVAL; }
     if (cfg->flux_fp32 != 0 && cfg->flux_fp32 != 1) { set_error("flux_fp32 must be 0 or 1"); return OFX_EINVAL; }
+    if (!(cfg->aref > 0.0)) { set_error("aref must be positive"); return OFX_EINVAL; }
     if (cfg->time_integral != 1 && cfg->time_integral != 2) {
         set_error("time_integral must be 1 (multi-stage) or 2 (LU-SGS)"); return OFX_EINVAL;
     }
@@ -486,6 +487,12 @@ int ofx_aero(ofx_ctx* c, double* out4) {
     CHECK_READY(c);
     if (!out4) { set_error("null out"); return OFX_EINVAL; }
     return aero(c, out4);
+}
+
+int ofx_aero_row(ofx_ctx* c, double* out12) {
+    CHECK_READY(c);
+    if (!out12) { set_error("null out"); return OFX_EINVAL; }
+    return aero_row_zone(c, 0, out12);
 }
 
 int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {

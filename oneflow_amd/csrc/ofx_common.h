@@ -573,6 +573,7 @@ int wall_distance(ofx_ctx* c);
 inline int ensure_dist(ofx_ctx* c) { return c->dist_valid ? 0 : wall_distance(c); }
 int norms_range(ofx_ctx* c, int c0, int c1, ofx_norms* out);
 int aero_zone(ofx_ctx* c, int z, double* out4);
+int aero_row_zone(ofx_ctx* c, int z, double* out12);
 int wall_data(ofx_ctx* c, double* host_out);
 int fill_ghost_grad(ofx_ctx* c);
 int materialize_timestep(ofx_ctx* c);

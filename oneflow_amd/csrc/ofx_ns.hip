@@ -1498,7 +1498,8 @@ int wall_data(ofx_ctx* c, double* host_out) {
 }
 
 // Forces of zone z (its boundary faces [f0, f1), its constants and angles).
-int aero_zone(ofx_ctx* c, int z, double* out4) {
+// Summed wall forces (total and pressure part) of zone z, and its configuration.
+static int zone_forces(ofx_ctx* c, int z, double* h6, const ofx_config** cfg) {
     double* buf = c->d.norms + 8;
     int f0 = 0, f1 = c->m.B;
     Consts k = c->k;
@@ -1513,15 +1514,44 @@ int aero_zone(ofx_ctx* c, int z, double* out4) {
     }
     k_aero<<<1, 64, 0, c->stream>>>(c->m, c->d, k, buf, f0, f1);
     OFX_HIP(hipGetLastError());
-    double h[6];
-    OFX_HIP(hipMemcpyAsync(h, buf, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    OFX_HIP(hipMemcpyAsync(h6, buf, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     OFX_HIP(hipStreamSynchronize(c->stream));
-    double aoa = g->aoa_degree * PI / 180.0, aos = g->sideslip_degree * PI / 180.0;
+    *cfg = g;
+    return 0;
+}
+
+// AeroForceInfo::CalcCoef (codes/force/src/Force.cpp:259-276) with AeroCom::CalcCL /
+// CalcCD (codes/force/src/Force.cpp:150-158) -> the 12 aero.dat value columns.
+// The moments the reference never accumulates are zero.
+static void aero_coefficients(const double* tot, const double* pres, const ofx_config& g, double* out12) {
+    double aoa = g.aoa_degree * PI / 180.0, aos = g.sideslip_degree * PI / 180.0;
     double sina = std::sin(aoa), cosa = std::cos(aoa), sinb = std::sin(aos), cosb = std::cos(aos);
-    double cl = -sina * h[0] + cosa * h[1];
-    double cd = cosa * cosb * h[0] + sina * cosb * h[1] + sinb * h[2];
-    double cdp = cosa * cosb * h[3] + sina * cosb * h[4] + sinb * h[5];
-    out4[0] = cl; out4[1] = cd; out4[2] = cdp; out4[3] = cd - cdp;
+    double cf[3], cp[3], cmom[3];
+    for (int a = 0; a < 3; ++a) { cf[a] = tot[a] / g.aref; cp[a] = pres[a] / g.aref; cmom[a] = 0.0 / (g.aref * 1.0); }
+    const double AR = 9.5;
+    double cl = -sina * cf[0] + cosa * cf[1];
+    double cd = cosa * cosb * cf[0] + sina * cosb * cf[1] + sinb * cf[2];
+    double cd_pres = cosa * cosb * cp[0] + sina * cosb * cp[1] + sinb * cp[2];
+    double cd_vis = cd - cd_pres;
+    double cdl = cd - (cl * cl) / (PI * AR);
+    double sgn = 1.0 * std::fabs(cf[1]) / (cf[1] + SMALL);  // SIGN(1.0, cf.y)
+    double pres_center = sgn * cmom[2] / (std::fabs(cf[1]) + SMALL);
+    const double row[12] = {cl, cd, cd_pres, cd_vis, cdl, pres_center, cf[0], cf[1], cf[2], cmom[0], cmom[1], cmom[2]};
+    for (int i = 0; i < 12; ++i) out12[i] = row[i];
+}
+
+int aero_row_zone(ofx_ctx* c, int z, double* out12) {
+    double h[6];
+    const ofx_config* g = nullptr;
+    OFX_TRY(zone_forces(c, z, h, &g));
+    aero_coefficients(h, h + 3, *g, out12);
+    return 0;
+}
+
+int aero_zone(ofx_ctx* c, int z, double* out4) {
+    double row[12];
+    OFX_TRY(aero_row_zone(c, z, row));
+    for (int i = 0; i < 4; ++i) out4[i] = row[i];
     return 0;
 }
 

@@ -86,3 +86,26 @@ def wallaero_lines(rows) -> list[str]:
 
 def write_wallaero(path, rows) -> None:
     Path(path).write_text("\n".join(wallaero_lines(rows)) + "\n")
+
+
+AERO_COLUMNS = ["CL", "CD", "CD_PR", "CD_SF", "CDL2", "Xcp", "Fx", "Fy", "Fz", "Cmx", "Cmy", "Cmz"]
+
+
+def aero_lines(first_step: int, rows, dt: float = 1.0e-3, time0: float = 0.0) -> list[str]:
+    """aero.dat text (AerodynamicForceTask::Dump, codes/force/src/AeroForceTask.cpp:96-150):
+    the header, then per step `iter  sub-iter  time` and the 12 value columns
+    of `rows` [n][12] (ofx_aero_row), four spaces after every field.  The time
+    column is the reference's ctrl.currTime, advanced by global_dt each outer
+    step (codes/multigrid/src/Multigrid.cpp:137-155) from time0."""
+    head = ['Title="Aerodynamic Force"', "Variables=", '"iter"', '"sub-iter"', '"time"']
+    head += [f'"{c}"' for c in AERO_COLUMNS]
+    out, t = [], time0
+    for i, row in enumerate(np.asarray(rows)):
+        t += dt
+        fields = [str(first_step + i), "1", f"{t:.6e}"] + [f"{float(v):.4e}" for v in row[:12]]
+        out.append("".join(f + "    " for f in fields))
+    return head + out
+
+
+def write_aero(path, first_step: int, rows, dt: float = 1.0e-3, time0: float = 0.0) -> None:
+    Path(path).write_text("\n".join(aero_lines(first_step, rows, dt, time0)) + "\n")

@@ -70,6 +70,7 @@ def load_library():
     L.ofx_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
     L.ofx_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
     L.ofx_aero.argtypes = [P, dp]
+    L.ofx_aero_row.argtypes = [P, dp]
     L.ofx_time_kernel.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
     L.ofx_copy_bench.argtypes = [P, ctypes.c_size_t, ctypes.c_int, dp]
     L.ofx_zones_set.argtypes = [P, ctypes.c_int, ip, ip, ctypes.c_void_p]
@@ -206,6 +207,11 @@ class Solver:
     def aero(self):                   # DUMP_AERODYNAMIC: CL, CD, CD_pres, CD_visc
         a = np.zeros(4)
         self._check(self._L.ofx_aero(self.h, _dp(a)))
+        return a
+
+    def aero_row(self):               # the 12 value columns of an aero.dat row
+        a = np.zeros(12)
+        self._check(self._L.ofx_aero_row(self.h, _dp(a)))
         return a
 
     KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5,

@@ -1602,6 +1602,7 @@ int ofo_mesh_info(ofo_case* c, int* out5) {
 }
 
 int ofo_config_set(ofo_case* c, const ofx_config* cfg) {
+    if (!(cfg->aref > 0.0)) return fail("aref must be positive");
     if (cfg->time_integral != 1 && cfg->time_integral != 2) return fail("time_integral must be 1 or 2");
     if (cfg->time_integral == 1 && cfg->rk_stage < 1) return fail("rk_stage must be >= 1");
     if (cfg->ischeme != 1 && cfg->ischeme != 12) return fail("ischeme must be 1 (Roe) or 12 (SLAU2)");
@@ -1854,20 +1855,43 @@ static void wall_face(const ofo_case* c, int f, double* pf, double* vf, double& 
 
 // CalcAeroForce + AeroForceInfo::CalcCoef (codes/force/src/AeroForceTask.cpp:186-440,
 // codes/force/src/Force.cpp:148-270, codes/solver/src/Stress.cpp), aref = 1.
-int ofo_aero(ofo_case* c, double* out4) {
+// AerodynamicForceTask::Run + Dump (codes/force/src/AeroForceTask.cpp:59-150):
+// the wall forces summed in face order (AeroForce::AddForce adds total, pres,
+// vis and power, codes/force/src/Force.cpp:202-208 -- not the moments), then
+// AeroForceInfo::CalcCoef (codes/force/src/Force.cpp:259-276) -> the value
+// columns of one aero.dat row: CL, CD, CD_PR, CD_SF, CDL2, Xcp, Fx, Fy, Fz,
+// Cmx, Cmy, Cmz.
+int ofo_aero_row(ofo_case* c, double* out12) {
     double aoa = c->cfg.aoa_degree * PI / 180.0, aos = c->cfg.sideslip_degree * PI / 180.0;
     double sina = std::sin(aoa), cosa = std::cos(aoa), sinb = std::sin(aos), cosb = std::cos(aos);
-    double tot[3] = {0, 0, 0}, pres[3] = {0, 0, 0};
+    double tot[3] = {0, 0, 0}, pres[3] = {0, 0, 0}, mom[3] = {0, 0, 0};
     for (int f = 0; f < c->nBFaces; ++f) {
         if (c->bcType[f] != BC_SOLID_SURFACE) continue;
         double pf[3], vf[3], cp;
         wall_face(c, f, pf, vf, cp);
-        for (int k = 0; k < 3; ++k) { tot[k] += pf[k] + vf[k]; pres[k] += pf[k]; }
+        double face_total[3];
+        for (int k = 0; k < 3; ++k) face_total[k] = pf[k] + vf[k];
+        for (int k = 0; k < 3; ++k) { tot[k] += face_total[k]; pres[k] += pf[k]; }
     }
-    double cl = -sina * tot[0] + cosa * tot[1];
-    double cd = cosa * cosb * tot[0] + sina * cosb * tot[1] + sinb * tot[2];
-    double cdp = cosa * cosb * pres[0] + sina * cosb * pres[1] + sinb * pres[2];
-    out4[0] = cl; out4[1] = cd; out4[2] = cdp; out4[3] = cd - cdp;
+    double aref = c->cfg.aref, cmoment = c->cfg.aref * 1.0;  // lref = 1 in every case script
+    double cf[3], cpres[3], cmom[3];
+    for (int k = 0; k < 3; ++k) { cf[k] = tot[k] / aref; cpres[k] = pres[k] / aref; cmom[k] = mom[k] / cmoment; }
+    const double AR = 9.5;
+    double cl = -sina * cf[0] + cosa * cf[1];
+    double cd = cosa * cosb * cf[0] + sina * cosb * cf[1] + sinb * cf[2];
+    double cd_pres = cosa * cosb * cpres[0] + sina * cosb * cpres[1] + sinb * cpres[2];
+    double cd_vis = cd - cd_pres;
+    double cdl = cd - SQR(cl) / (PI * AR);
+    double pres_center = SIGN(1.0, cf[1]) * cmom[2] / (ABS(cf[1]) + SMALL);
+    double row[12] = {cl, cd, cd_pres, cd_vis, cdl, pres_center, cf[0], cf[1], cf[2], cmom[0], cmom[1], cmom[2]};
+    for (int k = 0; k < 12; ++k) out12[k] = row[k];
+    return 0;
+}
+
+int ofo_aero(ofo_case* c, double* out4) {
+    double row[12];
+    ofo_aero_row(c, row);
+    for (int k = 0; k < 4; ++k) out4[k] = row[k];
     return 0;
 }
 

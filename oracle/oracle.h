@@ -46,6 +46,7 @@ int  ofo_lhs(ofo_case* c, int solver, double lhscoef);
 int  ofo_norms_get(ofo_case* c, ofx_norms* out);
 int  ofo_iterate(ofo_case* c, int first_iter, int n, ofx_norms* norms_out);
 int  ofo_aero(ofo_case* c, double* out4);
+int  ofo_aero_row(ofo_case* c, double* out12);
 /* Wall faces (face order): count, and x, y, z of the face centre, -cp, cf
  * per face -> out[5 * k .. 5 * k + 4]. */
 int  ofo_wall_count(ofo_case* c);

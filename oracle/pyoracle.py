@@ -59,6 +59,7 @@ def lib():
         L.ofo_norms_get.argtypes = [P, ctypes.POINTER(OfxNorms)]
         L.ofo_iterate.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OfxNorms)]
         L.ofo_aero.argtypes = [P, dp]
+        L.ofo_aero_row.argtypes = [P, dp]
         L.ofo_wall_count.argtypes = [P]
         L.ofo_wall_data.argtypes = [P, dp]
         L.ofo_face_flux.argtypes = [P, dp, dp]
@@ -166,6 +167,11 @@ class OracleCase:
     def aero(self):
         a = np.zeros(4)
         self._check(lib().ofo_aero(self.h, _dp(a)))
+        return a
+
+    def aero_row(self):
+        a = np.zeros(12)
+        self._check(lib().ofo_aero_row(self.h, _dp(a)))
         return a
 
     def wall_data(self):

@@ -12,7 +12,7 @@ import pytest
 from oneflow_amd.config import REFERENCE_CASES
 from oneflow_amd.mesh import read_ofl
 from oracle.pyoracle import OracleCase
-from oneflow_amd.restart import wallaero_lines
+from oneflow_amd.restart import aero_lines, wallaero_lines
 from tests.golden_util import GOLDEN, harness_equal, load_res
 from tests.parity_util import report
 
@@ -134,10 +134,11 @@ def test_fifty_steps_against_goldens(case):
     gpu, ora = make(case)
     gpu.init_freestream()
     ora.init_freestream()
-    norms, aero = [], []
+    norms, aero, rows = [], [], []
     for it in range(1, 51):
         norms.append(gpu.iterate(it, 1)[0])
         aero.append(gpu.aero())
+        rows.append(gpu.aero_row())
     norms = np.array(norms)
     onorms = ora.iterate(1, 50)
     e = report("norms", norms, onorms)
@@ -156,6 +157,9 @@ def test_fifty_steps_against_goldens(case):
     for it, row in enumerate(rows):
         for mine, ref in zip(aero[it][:2], row[3:5]):
             assert harness_equal("%.4e" % mine, "%.4e" % float(ref)), (case, it + 1, aero[it], row[3:5])
+    # the whole aero.dat file (15 columns) from the GPU's rows
+    from tests.test_oracle_golden import compare_aero_file
+    compare_aero_file(case, aero_lines(1, rows), (GOLDEN / case / "aero.dat").read_text().splitlines())
     # wallaero.dat after step 50: per wall face -cp and cf at 11 digits
     wall = gpu.wall_data()
     ew = report("wall", wall, ora.wall_data())
