@@ -195,3 +195,12 @@ def test_continue_from_reference_restart():
     assert e <= 1.0e-10, e
     assert np.allclose(aero, oaero, rtol=1e-10, atol=1e-16)
     check_against_goldens(norms, aero)
+
+
+def test_step_reproduces_reference_restart_state_bitwise():
+    """One GPU pseudo-step from the reference restart file's q2 gives its q1
+    bit for bit (see tests/test_restart.py)."""
+    from oneflow_amd.solver import Solver
+    from tests.test_restart import step_from_q2
+    q, r = step_from_q2(Solver)
+    assert np.array_equal(q, r.q1)

@@ -69,3 +69,29 @@ def check_against_goldens(norms, aero):
 def test_oracle_continues_reference_restart():
     norms, aero = continued(OracleCase)
     check_against_goldens(norms, aero)
+
+
+def step_from_q2(solver_cls_or_oracle):
+    """The restart file holds q1 (the state after step 34950) and q2 (after
+    step 34949): UPDATE_UNSTEADY_FLOW copies q2 <- q1, q1 <- q after every
+    outer step (codes/unsteady/src/Unsteady.cpp:44-58) and DUMP_RESTART
+    writes both (codes/restart/src/Restart.cpp:99-121).  One pseudo-step
+    (step 34950, CFL at its final value) from q2 must give q1."""
+    g = read_ofl(GRID)
+    r = read_flow_dat(CASE / "flow.dat", g.n_tcell)
+    s = solver_cls_or_oracle(g, REFERENCE_CASES["plateuns2dslau2"])
+    s.init_freestream()
+    s.set("Q", np.ascontiguousarray(r.q2))
+    s.ns_boundary()
+    s.iterate(r.outer_steps, 1)
+    return s.get("Q"), r
+
+
+def test_oracle_step_reproduces_reference_state_bitwise():
+    """A full-precision, field-level pin of the whole laminar SLAU2 pseudo-step
+    (gradients, reconstruction, SLAU2 and viscous fluxes, time step, LU-SGS,
+    update, boundary conditions) against state the reference itself wrote:
+    every cell and ghost value of q1, bit for bit."""
+    q, r = step_from_q2(OracleCase)
+    assert np.array_equal(q, r.q1)
+    assert not np.array_equal(r.q1, r.q2)  # the state still moves by ~3e-6 per step
