@@ -1,25 +1,32 @@
 """Benchmark: cell-residual evaluations/s of the NS + SA residual on one MI355X.
 
-Workload: NACA0012 from the analytic 4-digit formula, hybrid grid of 40 quad
-wall layers (first height for y+ ~ 1 at Re 6.5e6, growth 1.15) plus
-triangles out to 30 chords, 120,832 cells; Mach 0.8, AoA 1.25 deg, Re 6.5e6,
-Roe + Spalart-Allmaras + Venkatakrishnan limiter, FP64.  Synthetic grid,
-freestream start, then a few pseudo-time steps so the limiter and SA terms
-are active before timing.
+Default workload (the largest configuration that fits one GPU): the 3-D
+swept wing -- NACA0012 section, sweep 30 deg, taper 0.5, aspect ratio 4, 25
+prism wall layers plus tets, 2,524,160 cells; Mach 0.84, AoA 3 deg, Re 1.2e7,
+Roe + Spalart-Allmaras + Venkatakrishnan limiter, FP64, multicolour cell
+numbering with interior faces in spatial order.  Synthetic grid, freestream
+start, then a few pseudo-time steps so the limiter and SA terms are active
+before timing.  `--workload c1` runs the 2-D NACA0012 hybrid grid (120,832
+cells), `batch` the 64-case airfoil sweep, `warmstart` time to a 1e-8
+residual drop, learned vs plain.
 
 One step = one full residual evaluation: the NS residual (gradients, limiter,
 MUSCL + Roe, viscous flux, ordered gather) and the SA residual (spectrum,
 source, convection, diffusion, gather) -- UPDATE_RESIDUALS of both solvers.
-value = nTCell * steps / seconds (cell-residual evaluations per second).
+value = nCells * steps / seconds (cell-residual evaluations per second).
 
-roofline: the dominant kernel of the step (largest average duration among the
-NS residual kernels, each timed with HIP events on the solver's stream over
---kernel-iters back-to-back launches right after the timed steps); achieved = its algorithmic bytes per launch (each array it must
-read or write counted once, see kernel_bytes) / its average duration; peak
-8 TB/s.  step_roofline: the whole evaluation's algorithmic bytes
-552 * nTCell + 64 * F over ms_per_step.  traffic: measured HBM bytes per
-launch of that kernel from the committed rocprofv3 counter summary, when one
-for this workload is present.
+roofline: the evaluation against HBM: achieved = its algorithmic bytes
+552 * nTCell + 64 * nFaces (every array of the evaluation counted once:
+state, geometry, wall distance, gradients and limiter written and read once,
+residual; face normals, centres and cell ids) over the evaluation's average
+duration, measured with HIP events on the solver's stream over back-to-back
+evaluations; peak 8 TB/s.  traffic: the bytes that left L2 per evaluation
+(FETCH_SIZE + WRITE_SIZE of every kernel, rocprofv3 counters, scaled by a
+copy kernel) from the committed counter summary of the same workload.
+`kernels` lists each kernel with its own algorithmic bytes and HIP-event
+time; `pseudo_step` the whole pseudo-time step (residual, LU-SGS, update and
+boundary conditions of both solvers); `oracle_error` the FP64 and FP32-flux
+residuals against the CPU oracle on a slice of the same grid.
 
 cpu_baseline: the CPU oracle (single thread, the reference's loops) timed on
 this host on a bounded sample of the same workload.
@@ -217,13 +224,27 @@ def batch_main(args):
     zs.sync()
     t1 = time.perf_counter()
     ms_zones = 1e3 * (t1 - t0) / args.steps
+    # the zones' steps recorded into one HIP graph: one launch advances every
+    # case graph_steps steps
+    it = ramp + 1 + args.warmup + args.steps
+    zb = Batch([zs], steps_per_graph=args.graph_steps)
+    zb.iterate(it, args.graph_steps)  # records the graph, first replay
+    it += args.graph_steps
+    zs.sync()
+    nz = max(args.graph_steps, args.steps // args.graph_steps * args.graph_steps)
+    t0 = time.perf_counter()
+    kz = zb.iterate(it, nz)
+    zs.sync()
+    t1 = time.perf_counter()
+    ms_zgraph = 1e3 * (t1 - t0) / nz
+    zb.close()
     zs.close()
-    print(f"[bench] graph batch {ms_batch:.3f} ms/step, zones {ms_zones:.3f} ms/step, case after case {ms_seq:.3f}",
-          file=sys.stderr, flush=True)
-    ms_best = min(ms_batch, ms_zones)
+    print(f"[bench] graph batch {ms_batch:.3f} ms/step, zones {ms_zones:.3f}, zones in one graph {ms_zgraph:.3f}, "
+          f"case after case {ms_seq:.3f}", file=sys.stderr, flush=True)
+    ms_best = min(ms_batch, ms_zones, ms_zgraph)
     out = {
         "metric": "cell-steps/s of the batched airfoil sweep (full NS+SA pseudo-time steps)",
-        "value": nt * 1e3 / ms_best,
+        "value": ncell * 1e3 / ms_best,
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -241,6 +262,8 @@ def batch_main(args):
         "sequential_ms_per_step": ms_seq,
         "graph_ms_per_step": ms_batch,
         "zones_ms_per_step": ms_zones,
+        "zones_graph_ms_per_step": ms_zgraph,
+        "zones_graph_launches": kz,
         "batch_speedup": ms_seq / ms_batch,
         "zones_speedup": ms_seq / ms_zones,
         "face_tiles": {str(t): int(n) for t, n in zip(*np.unique(tiles[:, 0], return_counts=True))},
@@ -510,9 +533,9 @@ def main():
                          "(default: cell for c1, spatial for c2)")
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "batch", "warmstart"],
-                    help="c1: single-grid residual throughput (default); c2: the 3-D swept wing; "
-                         "batch: the 64-case sweep; "
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "batch", "warmstart"],
+                    help="c2: the 3-D swept wing (default, the largest single-GPU configuration); "
+                         "c1: the 2-D NACA0012 grid; batch: the 64-case sweep; "
                          "warmstart: time to 1e-8, learned vs plain")
     ap.add_argument("--n-train", type=int, default=48)
     ap.add_argument("--n-test", type=int, default=16)
@@ -542,61 +565,103 @@ def main():
     if args.workload == "warmstart":
         return warmstart_main(args)
 
-    if args.workload == "c2":
-        return c2_main(args)
-    from oneflow_amd.solver import Solver
+    return grid_main(args)
 
-    cfg = c1_config()
+
+def grid_main(args):
+    """One grid (c1: the 2-D NACA0012; c2: the 3-D swept wing): FP64 residual
+    throughput and roofline, the FP32-flux variant (c2), the whole pseudo-step,
+    the error against the oracle and the CPU baseline."""
+    from oneflow_amd.solver import Solver
+    c2 = args.workload == "c2"
+    cfg = c2_config() if c2 else c1_config()
     if args.numbering is None:
         args.numbering = "multicolor"
     if args.face_order is None:
-        args.face_order = "cell"
-    grid = c1_grid(args.numbering, args.face_order)
-    print(f"[bench] grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces}", file=sys.stderr, flush=True)
+        args.face_order = "spatial" if c2 else "cell"
+    t0 = time.perf_counter()
+    grid = c2_grid(numbering=args.numbering, face_order=args.face_order) if c2 else \
+        c1_grid(args.numbering, args.face_order)
+    log(f"grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces} ({time.perf_counter() - t0:.1f} s)")
     s = Solver(grid, cfg, device=0)
-    r = residual_bench(s, grid, args)
+    r64 = residual_bench(s, grid, args)
+    s.close()
+    log(f"fp64: {r64['ms_per_step']:.4f} ms per evaluation (host), {r64['roofline']['avg_us']:.1f} us (events), "
+        f"pseudo-step {r64['pseudo_step']['ms']:.4f} ms")
+    r32 = None
+    if c2:
+        s = Solver(grid, cfg.replace(flux_fp32=1), device=0)
+        r32 = residual_bench(s, grid, args, flux_bytes=4)
+        s.close()
+        log(f"fp32 flux: {r32['ms_per_step']:.4f} ms per evaluation")
+    sample = c2_grid(n_span=2) if c2 else grid
+    err = oracle_error(sample, cfg, fp32=c2)
+    log(f"oracle error: {err}")
     cpu = None
     if not args.no_cpu:
-        n, dt = cpu_baseline(grid, cfg, args.cpu_budget)
-        cpu = {"value": s.n_tcell * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
-               "sample": f"{n} NS+SA residual evaluations of the same {grid.n_cells}-cell grid, CPU oracle, "
-                         f"1 thread of {os.cpu_count()} host cores, {dt:.1f} s"}
+        n, dt = cpu_baseline(sample, cfg, args.cpu_budget)
+        what = (f"a {sample.n_cells}-cell 2-layer span slice of the same wing grid" if c2
+                else f"the same {grid.n_cells}-cell grid")
+        cpu = {"value": sample.n_cells * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
+               "sample": f"{n} NS+SA residual evaluations of {what}, CPU oracle, 1 thread of "
+                         f"{os.cpu_count()} host cores, {dt:.1f} s"}
+    workload = ("swept wing NACA0012, sweep 30 deg, taper 0.5, AR 4, 25 prism layers + tets, Roe + SA + Venkat, "
+                "FP64 residual" if c2 else "NACA0012 hybrid quad/tri grid, Roe + SA + Venkat, FP64 residual")
     out = {
         "metric": "cell-residual evals/s + % HBM peak; time-to-1e-8 residual, learned vs plain",
-        "value": r["value"],
+        "value": r64["value"],
         "unit": "cell-residual evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"],
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "NACA0012 hybrid quad/tri grid, Roe + SA + Venkat, FP64 residual",
-                   "numbering": args.numbering, "face_order": args.face_order,
-                   "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": s.n_tcell,
+        "config": {"workload": workload, "numbering": args.numbering, "face_order": args.face_order,
+                   "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": grid.n_tcell,
                    "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
-        "roofline": r["roofline"],
-        "step_roofline": r["step_roofline"],
-        "kernels": r["kernels"],
-        "copy_kernel_gbs": r["copy_kernel_gbs"],
-        "launch": r["launch"],
+        "roofline": r64["roofline"],
+        "pseudo_step": r64["pseudo_step"],
+        "oracle_error": err,
+        "kernels": r64["kernels"],
+        "dominant_kernel": r64["dominant_kernel"],
+        "copy_kernel_gbs": r64["copy_kernel_gbs"],
+        "launch": r64["launch"],
         "cpu_baseline": cpu,
     }
+    if r32 is not None:
+        out["fp32_flux"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "roofline": r32["roofline"],
+                            "pseudo_step": r32["pseudo_step"], "kernels": r32["kernels"]}
     print(json.dumps(out))
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def step_bytes(grid):
+    """Algorithmic bytes of one NS + SA residual evaluation: 552 per cell
+    (ghosts included) + 64 per face, FP64 values and int32 ids, every array
+    counted once (state 48, centre and volume 32, wall distance 8, gradients
+    of 5 primitives, T and nu_tilde written and read 336, limiter written and
+    read 80, residual 48; per face normal, area and centre 56 and the two
+    cell ids 8)."""
+    return 552.0 * grid.n_tcell + 64.0 * grid.n_faces
 
 
 def residual_bench(s, grid, args, flux_bytes=8):
     """Time `args.steps` NS + SA residual evaluations of one solver (after
-    `args.pre_iters` pseudo-steps and `args.warmup` untimed evaluations), then
-    each NS residual kernel over back-to-back launches."""
+    `args.pre_iters` pseudo-steps and `args.warmup` untimed evaluations), the
+    evaluation and each residual kernel with HIP events over back-to-back
+    launches, and the whole pseudo-step."""
     s.init_freestream()
     s.iterate(1, args.pre_iters, with_norms=False)
     s.timestep(args.pre_iters + 1)
-    # launch configuration: the gradient variants timed here for the record,
-    # then the autotuner picks variant and face tiles on this state
+    # launch configuration: the variants timed here for the record, then the
+    # autotuner picks variant and face tiles on this state
     variant_us = {}
     for kernel, key in ((0, "grad"), (1, "limiter"), (2, "gather")):
         t = {}
@@ -617,15 +682,12 @@ def residual_bench(s, grid, args, flux_bytes=8):
     s.sync()
     sec = time.perf_counter() - t0
     ms_per_step = 1e3 * sec / args.steps
+    # the evaluation with HIP events on the solver's stream, back to back
+    eval_ms = s.time_kernel("residual", iters=args.steps)
     copy_ms = s.copy_bench(args.copy_bytes, iters=20)
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms * 1e-3) / 1e9
     kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, grid.dim, flux_bytes, s.cfg.iturb_visflux,
                       s.cfg.ivischeme)
-    step_bytes = 552.0 * T + 64.0 * F if grid.dim == 2 else 693.0 * T + 64.0 * F
-    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
-    # each NS residual kernel timed with HIP events on the solver's stream over
-    # back-to-back launches on the state the timed steps left
-    # kernel names of the launch configuration the autotuner chose
     names = {"grad": ("k_ns_grad", "k_ns_grad_q")[s.kernel_variant(0)],
              "limiter": ("k_ns_limiter", "k_ns_limiter_c")[s.kernel_variant(1)],
              "face": "k_ns_face",
@@ -639,17 +701,82 @@ def residual_bench(s, grid, args, flux_bytes=8):
     for key, kname in names.items():
         ms = prof[key]
         kernels[kname] = {"avg_us": 1e3 * ms, "bytes_per_launch": kb[key],
-                          "achieved_gbs": kb[key] / (ms * 1e-3) / 1e9 if ms > 0 else None}
+                          "achieved_gbs": kb[key] / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                          "traffic_bytes": read_traffic(kname, grid.dim)}
     dom = max(names, key=lambda k: prof[k])
     ach = kb[dom] / (prof[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(names[dom], grid.dim),
-                "kernel": names[dom], "avg_us": 1e3 * prof[dom], "bytes_per_launch": kb[dom],
-                "launches_timed": args.kernel_iters}
-    return {"value": T * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline, "kernels": kernels,
-            "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
-            "copy_kernel_gbs": copy_gbs,
+    dominant = {"kernel": names[dom], "avg_us": 1e3 * prof[dom], "bytes_per_launch": kb[dom], "achieved": ach,
+                "frac": ach / HBM_PEAK_GBS, "launches_timed": args.kernel_iters}
+    sb = step_bytes(grid)
+    sgbs = sb / (eval_ms * 1e-3) / 1e9
+    traffic = [kernels[k]["traffic_bytes"] for k in kernels]
+    roofline = {"bound": "hbm", "achieved": sgbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sgbs / HBM_PEAK_GBS,
+                "traffic": float(sum(traffic)) if all(t is not None for t in traffic) else None,
+                "traffic_kind": "bytes leaving L2 per evaluation (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled), "
+                                "committed counter summary of this workload",
+                "kernel": "NS + SA residual evaluation (7 kernels)", "avg_us": 1e3 * eval_ms,
+                "bytes_per_launch": sb, "launches_timed": args.steps}
+    # the whole pseudo-time step (residual, LU-SGS, update, boundary of both
+    # solvers), HIP events; this advances the state, so it runs last
+    ps_ms = s.time_kernel("pseudo_step", iters=args.steps)
+    pseudo = {"ms": ps_ms, "residual_ms": eval_ms, "implicit_ms": ps_ms - eval_ms,
+              "cell_steps_per_s": grid.n_cells / (ps_ms * 1e-3)}
+    return {"value": grid.n_cells * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline,
+            "kernels": kernels, "dominant_kernel": dominant, "pseudo_step": pseudo, "copy_kernel_gbs": copy_gbs,
             "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "face_split": s.kernel_variant(3)}}
+
+
+def oracle_error(grid, cfg, fp32=True):
+    """FP64 and FP32-flux NS + SA residuals against the CPU oracle on the same
+    state (the oracle's after 10 pseudo-steps from freestream): normwise
+    ||res_gpu - res_oracle|| / ||res_oracle|| and the largest per-cell error
+    relative to the cell's flux scale sum_f |F_f| A_f."""
+    from oneflow_amd.solver import Solver
+    from oracle.pyoracle import OracleCase
+    ora = OracleCase(grid, cfg)
+    ora.init_freestream()
+    ora.iterate(1, 10)
+    ora.timestep(11)
+    ora.residual_ns()
+    ora.residual_sa()
+    C = grid.n_cells
+    ref = np.concatenate([ora.get("RES")[:, :C], ora.get("TURBRES")[None, :C]])
+    scale = _flux_scale(ora, grid)
+    out = {"sample": f"{C} cells, state after 10 oracle pseudo-steps"}
+    for name, c in (("fp64", cfg), ("fp32_flux", cfg.replace(flux_fp32=1))):
+        if name == "fp32_flux" and not fp32:
+            continue
+        s = Solver(grid, c)
+        s.init_freestream()
+        for n in ("Q", "TURBQ", "TEMPR", "VISL", "VIST", "TIMESTEP"):
+            s.set(n, ora.get(n))
+        s.residual_ns()
+        s.residual_sa()
+        got = np.concatenate([s.get("RES")[:, :C], s.get("TURBRES")[None, :C]])
+        s.close()
+        d = np.abs(got - ref)
+        out[name] = {"norm_rel": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+                     "max_cell_rel_ns": float(np.max(d[:5] / np.maximum(scale, 1e-300))),
+                     "bitwise_frac": float(np.mean(got == ref))}
+    return out
+
+
+def _flux_scale(ora, g):
+    """Per cell and NS equation: sum over the cell's faces of the area times
+    the magnitude of the physical flux terms of the cell's own state."""
+    q = ora.get("Q")
+    area = ora.get("FACE_NORMAL")[3]
+    C = g.n_cells
+    lc, rc = np.asarray(g.lc), np.asarray(g.rc)
+    rho, u, v, w, p = q[:, :C]
+    h = 1.4 / 0.4 * p / rho + 0.5 * (u * u + v * v + w * w)
+    vm = np.sqrt(u * u + v * v + w * w)
+    mag = np.stack([rho * vm, rho * vm * vm + p, rho * vm * vm + p, rho * vm * vm + p, rho * vm * h])
+    scale = np.zeros((5, C))
+    for side in (lc, rc):
+        ok = (side >= 0) & (side < C)
+        np.add.at(scale, (slice(None), side[ok]), mag[:, side[ok]] * area[ok])
+    return scale
 
 
 def c2_config():
@@ -663,56 +790,6 @@ def c2_grid(n_span=34, numbering="multicolor", face_order="spatial"):
     g = meshgen.swept_wing(n_surf=256, n_quad=25, ratio=1.2, reynolds=1.2e7, n_outer=40, radius=20.0,
                            n_span=n_span, sweep_deg=30.0, taper=0.5, aspect=4.0, t=0.12)
     return _numbered(g, numbering, face_order)
-
-
-def c2_main(args):
-    """The 3-D swept wing: NACA0012 section, sweep 30 deg, taper 0.5, AR 4,
-    25 wall prism layers plus tets, 2.52M cells, between symmetry planes;
-    Mach 0.84, AoA 3 deg, Re 1.2e7, Roe + SA + Venkat.  NS + SA residual
-    throughput in FP64 and in the FP32-flux variant."""
-    from oneflow_amd.solver import Solver
-    cfg = c2_config()
-    t0 = time.perf_counter()
-    if args.numbering is None:
-        args.numbering = "multicolor"
-    if args.face_order is None:
-        args.face_order = "spatial"
-    grid = c2_grid(numbering=args.numbering, face_order=args.face_order)
-    print(f"[bench] wing: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces} "
-          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    s = Solver(grid, cfg, device=0)
-    r64 = residual_bench(s, grid, args)
-    s.close()
-    print(f"[bench] fp64 {r64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
-    s = Solver(grid, cfg.replace(flux_fp32=1), device=0)
-    r32 = residual_bench(s, grid, args, flux_bytes=4)
-    s.close()
-    print(f"[bench] fp32 flux {r32['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
-    cpu = None
-    if not args.no_cpu:
-        sample = c2_grid(n_span=2)
-        n, dt = cpu_baseline(sample, cfg, args.cpu_budget)
-        cpu = {"value": sample.n_tcell * n / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
-               "sample": f"{n} NS+SA residual evaluations of a {sample.n_cells}-cell 2-layer span slice of the "
-                         f"same wing grid, CPU oracle, 1 thread, {dt:.1f} s"}
-    out = {
-        "metric": "cell-residual evals/s + % HBM peak (3-D swept wing)",
-        "value": r64["value"], "unit": "cell-residual evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": r64["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "swept wing NACA0012, sweep 30, taper 0.5, AR 4, prism layers + tets, "
-                               "Roe + SA + Venkat", "numbering": args.numbering, "face_order": args.face_order,
-                   "cells": grid.n_cells, "faces": grid.n_faces,
-                   "nTCell": grid.n_tcell, "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree,
-                   "reynolds": cfg.reynolds},
-        "roofline": r64["roofline"], "step_roofline": r64["step_roofline"], "kernels": r64["kernels"],
-        "fp32_flux": {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "roofline": r32["roofline"],
-                      "kernels": r32["kernels"]},
-        "copy_kernel_gbs": r64["copy_kernel_gbs"],
-        "launch": r64["launch"],
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -236,8 +236,10 @@ int ofx_wall_data(ofx_ctx* ctx, double* out);
  * returns the average ms per launch.  which: 0 NS gradient + limiter min/max,
  * 1 limiter, 2 NS face fluxes (MUSCL + Roe/SLAU2 + viscous), 3 NS ordered
  * gather, 4 whole NS residual, 5 whole SA residual, 6 SA gradient (+ ghost
- * values), 7 SA face terms, 8 SA cell terms (source + ordered gather).
- * Every part is
+ * values), 7 SA face terms, 8 SA cell terms (source + ordered gather),
+ * 9 NS + SA residual (one cell-residual evaluation), 10 one whole
+ * pseudo-time step past the CFL ramp (as ofx_iterate; this one advances the
+ * state).  Parts 0-9 are
  * idempotent on the current state, so the fields are left as one
  * ofx_residual_ns / ofx_residual_sa call leaves them (which 0 also leaves the
  * limiter stale until the next limiter launch). */

@@ -497,8 +497,8 @@ int ofx_aero_row(ofx_ctx* c, double* out12) {
 
 int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
     CHECK_READY(c);
-    if (!ms_out || iters < 1 || which < 0 || which > 8) {
-        set_error("time_kernel: which must be 0..8 and iters >= 1");
+    if (!ms_out || iters < 1 || which < 0 || which > 10) {
+        set_error("time_kernel: which must be 0..10 and iters >= 1");
         return OFX_EINVAL;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -509,6 +509,8 @@ int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
         if (which <= 3) return ns_residual_parts(c, 1u << which);
         if (which == 4) return ns_residual(c);
         if (which == 5) return sa_residual(c);
+        if (which == 9) { OFX_TRY(ns_residual(c)); return sa_residual(c); }
+        if (which == 10) return step_launch(c, c->cfg.ncfl + 1);
         return sa_residual_parts(c, 1u << (which - 6));
     };
     rc = once();  // warm-up launch

@@ -215,7 +215,7 @@ class Solver:
         return a
 
     KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5,
-               "sa_grad": 6, "sa_face": 7, "sa_cell": 8}
+               "sa_grad": 6, "sa_face": 7, "sa_cell": 8, "residual": 9, "pseudo_step": 10}
 
     def wall_data(self):               # DUMP_PRESSURE_COEFF
         """[n_wall][5]: x, y, z of each wall face centre, -cp, cf (face order)."""

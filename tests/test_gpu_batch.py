@@ -89,3 +89,28 @@ def test_batch_graph_replays_match_eager():
     assert k == 3
     same(eager, bat, "chunk 4")
     b.close()
+
+
+def test_zones_in_one_graph_match_eager_zones():
+    """All cases as zones of one context (every kernel spans all cases), their
+    post-ramp steps recorded into one HIP graph: one graph launch advances
+    every case steps_per_graph steps, bit for bit as eager zone stepping,
+    which equals each case alone (tests/test_gpu_zones.py)."""
+    from oneflow_amd.solver import Batch, ZoneSolver
+    cs = cases()
+    grids, cfgs = [g for g, _ in cs], [c for _, c in cs]
+    eager, graphed = ZoneSolver(grids, cfgs), ZoneSolver(grids, cfgs)
+    for z in (eager, graphed):
+        z.init_freestream()
+    b = Batch([graphed], steps_per_graph=6)
+    k = b.iterate(1, 23)           # 5 ramp steps direct, 3 replays of 6, no leftover
+    eager.iterate(1, 23, with_norms=False)
+    assert k == 3
+    same([eager], [graphed], "zones graph")
+    assert np.array_equal(eager.zone_norms(), graphed.zone_norms())
+    k = b.iterate(24, 12)
+    eager.iterate(24, 12, with_norms=False)
+    assert k == 2
+    same([eager], [graphed], "zones graph, second chunk")
+    assert np.array_equal(eager.zone_aero(), graphed.zone_aero())
+    b.close()
