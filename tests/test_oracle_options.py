@@ -218,3 +218,44 @@ def test_sa_production_form1_differs_from_rotational_on_strain():
     o2.residual_sa()
     r1, r2 = o.get("TURBRES")[: g.n_cells], o2.get("TURBRES")[: g2.n_cells]
     assert np.max(np.abs(r1 - r2)) > 1e-6 * np.max(np.abs(r2))
+
+
+@pytest.mark.parametrize("rho_scale", [1.0, 3.0])
+def test_isothermal_wall_ghost_closed_form(rho_scale):
+    """IsothermalVisWallBc (codes/ns/src/NsBcSolver.cpp:271-329): the ghost of a
+    no-slip wall at T_w = twall_dim / tref_dim carries -u, -v, -w, the owner
+    pressure p, and density max(0.5 rho_w, 2 rho_w - rho) with
+    rho_w = p / (statecoef T_w) (CalcDensity, :439-444, one species).
+    rho_scale 3 puts the owner density above 1.5 rho_w, where the floor
+    0.5 rho_w is taken."""
+    wall = lambda xm, ym: BC_SOLID_SURFACE if ym < 1e-12 else BC_FARFIELD
+    g = meshgen.box2d(6, 4, bc=wall)
+    cfg = CaseConfig(dim=2, ischeme=1, ilim=0, vismodel=1, isowallbc=1, mach_ref=0.5, reynolds=1e4,
+                     twall_dim=250.0, tref_dim=300.0)
+    o = OracleCase(g, cfg)
+    o.init_freestream()
+    q = o.get("Q")
+    xc = o.get("XCC")
+    C = g.n_cells
+    q[0, :C] = rho_scale * (1.0 + 0.2 * np.sin(4 * xc[0, :C]))
+    q[1, :C] = 0.8 + 0.1 * xc[1, :C]
+    q[2, :C] = 0.05 * np.cos(3 * xc[0, :C])
+    q[4, :C] = (1.0 + 0.1 * xc[0, :C]) / (GAMMA * 0.25)
+    o.set("Q", q)
+    o.ns_boundary()
+    out = o.get("Q")
+    statecoef = 1.0 / (GAMMA * 0.5 * 0.5)
+    tw = 250.0 / 300.0
+    walls = [f for f in range(g.n_bfaces) if g.bc_type[f] == BC_SOLID_SURFACE]
+    assert len(walls) == 6
+    took_floor = 0
+    for f in walls:
+        l, gh = int(g.lc[f]), C + f
+        rho, u, v, w, p = q[:, l]
+        rw = p / (statecoef * tw * 1.0)
+        rg = max(0.5 * rw, 2.0 * rw - rho)
+        took_floor += rg == 0.5 * rw
+        assert out[0, gh] == rg
+        assert out[1, gh] == -u and out[2, gh] == -v and out[3, gh] == -w
+        assert out[4, gh] == p
+    assert took_floor == (6 if rho_scale == 3.0 else 0)
