@@ -169,7 +169,7 @@ def _case_status(norm, limit, it, max_steps):
 
 
 def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None, rebuild=None,
-                             compact=0.5):
+                             compact=0.5, first_iter=1):
     """All cases as zones of one context advance together until every
     zone's density residual norm is below tol * refs[case], its norm is not
     finite (diverged: the reference stops on NaN,
@@ -181,7 +181,8 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
     boundary tasks), so finished cases stop costing work.  Returns (wall
     seconds including any rebuilds, step at which each case finished, forces
     [CL, CD, CDp, CDf] of each case at that step (None if diverged), status
-    of each case: CONVERGED, MAX_STEPS or DIVERGED)."""
+    of each case: CONVERGED, MAX_STEPS or DIVERGED).  first_iter: the outer
+    step the context is at (steps already taken + 1)."""
     n = zs.n_zones
     steps = [0] * n
     aero = [None] * n
@@ -189,7 +190,7 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
     zs.sync()
     t0 = time.perf_counter()
     ramp = zs.cfg.ncfl
-    it = 1
+    it = first_iter
     cur, ids = zs, list(range(n))
     while True:
         nstep = chunk + (ramp - it if it < ramp else 0)
