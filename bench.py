@@ -532,6 +532,8 @@ def main():
                     help="interior faces sorted by their renumbered cells or by their spatial order "
                          "(default: cell for c1, spatial for c2)")
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
+    ap.add_argument("--storage", type=int, default=None, choices=[0, 1],
+                    help="device storage order: 0 the grid's, 1 spatial (library default)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "batch", "warmstart"],
                     help="c2: the 3-D swept wing (default, the largest single-GPU configuration); "
@@ -583,14 +585,14 @@ def grid_main(args):
     grid = c2_grid(numbering=args.numbering, face_order=args.face_order) if c2 else \
         c1_grid(args.numbering, args.face_order)
     log(f"grid: cells={grid.n_cells} faces={grid.n_faces} bfaces={grid.n_bfaces} ({time.perf_counter() - t0:.1f} s)")
-    s = Solver(grid, cfg, device=0)
+    s = Solver(grid, cfg, device=0, storage_order=args.storage)
     r64 = residual_bench(s, grid, args)
     s.close()
     log(f"fp64: {r64['ms_per_step']:.4f} ms per evaluation (host), {r64['roofline']['avg_us']:.1f} us (events), "
         f"pseudo-step {r64['pseudo_step']['ms']:.4f} ms")
     r32 = None
     if c2:
-        s = Solver(grid, cfg.replace(flux_fp32=1), device=0)
+        s = Solver(grid, cfg.replace(flux_fp32=1), device=0, storage_order=args.storage)
         r32 = residual_bench(s, grid, args, flux_bytes=4)
         s.close()
         log(f"fp32 flux: {r32['ms_per_step']:.4f} ms per evaluation")
@@ -621,6 +623,7 @@ def grid_main(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "numbering": args.numbering, "face_order": args.face_order,
+                   "storage_order": "grid" if args.storage == 0 else "spatial",
                    "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": grid.n_tcell,
                    "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
         "roofline": r64["roofline"],

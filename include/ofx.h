@@ -131,6 +131,17 @@ int ofx_mesh_upload(ofx_ctx* ctx, int dim, int nNodes,
                     const int* faceNodeCount, const int* faceNodes,
                     const int* lc, const int* rc, const int* bcType);
 
+/* Storage order of the device arrays (no counterpart in the reference; a
+ * layout choice, results do not depend on it): 0 the grid's own cell and
+ * face order, 1 (default) spatial -- the cells of each connected part of the
+ * grid in Morton order of their centroids and the interior faces sorted by
+ * their cells' positions, so that neighbours share cache lines whatever the
+ * numbering (e.g. a colouring chosen for the LU-SGS sweeps).  Call before
+ * ofx_mesh_upload.  Every entry point keeps the grid's numbering: fields are
+ * read and written in grid order, each cell sums its faces in ascending face
+ * id and the LU-SGS sweeps split neighbours by cell id. */
+int ofx_set_storage_order(ofx_ctx* ctx, int mode);
+
 /* Mesh sizes: out[0..4] = nCells, nFaces, nBFaces, nTCell, nLevels (LU-SGS). */
 int ofx_mesh_info(ofx_ctx* ctx, int* out5);
 

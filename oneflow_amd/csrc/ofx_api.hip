@@ -312,37 +312,84 @@ static int field_view(ofx_ctx* c, int field, double** p, size_t* n) {
     return -1;
 }
 
+}  // extern "C"
+
+namespace {
+// A field's device blocks: `nb` arrays of `len` doubles, each indexed by cell
+// (cells then ghosts; kind 0), by cell without ghosts (kind 1) or by face
+// (kind 2).  The grid-order host copy is the blocks one after another.
+struct FieldBlocks {
+    std::vector<double*> p;
+    size_t len = 0;
+    int kind = 0;
+};
+
+int field_blocks(ofx_ctx* c, int field, FieldBlocks& fb) {
+    const size_t T = c->m.T, F = c->m.F, C = c->m.C;
+    const DevMesh& m = c->m;
+    double* g = c->d.grad;
+    fb.p.clear();
+    fb.kind = 0;
+    fb.len = T;
+    switch (field) {
+        case OFX_F_DQDX:  // [3][5][T]: x, y, z of rho,u,v,w,p
+            for (int a = 0; a < 3; ++a)
+                for (int e = 0; e < 5; ++e) fb.p.push_back(g + (a * 6 + e) * T);
+            return 0;
+        case OFX_F_DTDX:
+            for (int a = 0; a < 3; ++a) fb.p.push_back(g + (a * 6 + 5) * T);
+            return 0;
+        case OFX_F_TURBDQDX:
+            for (int a = 0; a < 3; ++a) fb.p.push_back(c->d.tgrad + a * T);
+            return 0;
+        case OFX_F_XCC:
+            fb.p = {m.xcc, m.ycc, m.zcc};
+            return 0;
+        case OFX_F_FACE_NORMAL:
+            fb.p = {m.xfn, m.yfn, m.zfn, m.area};
+            fb.kind = 2; fb.len = F;
+            return 0;
+        case OFX_F_FACE_CENTER:
+            fb.p = {m.xfc, m.yfc, m.zfc};
+            fb.kind = 2; fb.len = F;
+            return 0;
+        default: break;
+    }
+    double* p;
+    size_t n;
+    if (field_view(c, field, &p, &n)) return -1;
+    if (field == OFX_F_DIST) { fb.kind = 1; fb.len = C; }
+    for (size_t k = 0; k < n / fb.len; ++k) fb.p.push_back(p + k * fb.len);
+    return 0;
+}
+
+// Grid position -> storage position of entry k of a block of this kind.
+inline size_t storage_index(const ofx_ctx* c, int kind, size_t k) {
+    if (kind == 2) return c->fperm.empty() ? k : (size_t)c->fperm[k];
+    if (c->cperm.empty() || k >= (size_t)c->m.C) return k;  // ghosts keep their positions
+    return (size_t)c->cperm[k];
+}
+}  // namespace
+
+extern "C" {
+
 int ofx_get_field(ofx_ctx* c, int field, double* host) {
     CHECK_READY(c);
     if (field == OFX_F_DIST) OFX_TRY(ensure_dist(c));
     if (!host) { set_error("null host pointer"); return OFX_EINVAL; }
-    const size_t T = c->m.T, F = c->m.F;
-    const DevMesh& m = c->m;
-    auto D2H = [&](double* h, const double* dsrc, size_t n) {
-        return hipMemcpyAsync(h, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    };
-    if (field == OFX_F_DQDX) {
-        OFX_TRY(fill_ghost_grad(c));
-        // [3][5][T]: x, y, z of rho,u,v,w,p
-        for (int a = 0; a < 3; ++a) OFX_HIP(D2H(host + a * 5 * T, c->d.grad + a * 6 * T, 5 * T));
-    } else if (field == OFX_F_DTDX) {
-        OFX_TRY(fill_ghost_grad(c));
-        for (int a = 0; a < 3; ++a) OFX_HIP(D2H(host + a * T, c->d.grad + (a * 6 + 5) * T, T));
-    } else if (field == OFX_F_TURBDQDX) {
-        OFX_TRY(fill_ghost_grad(c));
-        OFX_HIP(D2H(host, c->d.tgrad, 3 * T));
-    } else if (field == OFX_F_XCC) {
-        OFX_HIP(D2H(host, m.xcc, T)); OFX_HIP(D2H(host + T, m.ycc, T)); OFX_HIP(D2H(host + 2 * T, m.zcc, T));
-    } else if (field == OFX_F_FACE_NORMAL) {
-        OFX_HIP(D2H(host, m.xfn, F)); OFX_HIP(D2H(host + F, m.yfn, F));
-        OFX_HIP(D2H(host + 2 * F, m.zfn, F)); OFX_HIP(D2H(host + 3 * F, m.area, F));
-    } else if (field == OFX_F_FACE_CENTER) {
-        OFX_HIP(D2H(host, m.xfc, F)); OFX_HIP(D2H(host + F, m.yfc, F)); OFX_HIP(D2H(host + 2 * F, m.zfc, F));
-    } else {
-        double* p; size_t n;
-        if (field_view(c, field, &p, &n)) { set_error("unknown field id"); return OFX_EINVAL; }
-        if (field == OFX_F_TIMESTEP || field == OFX_F_IMPSR) OFX_TRY(materialize_timestep(c));
-        OFX_HIP(D2H(host, p, n));
+    FieldBlocks fb;
+    if (field_blocks(c, field, fb)) { set_error("unknown field id"); return OFX_EINVAL; }
+    if (field == OFX_F_DQDX || field == OFX_F_DTDX || field == OFX_F_TURBDQDX) OFX_TRY(fill_ghost_grad(c));
+    if (field == OFX_F_TIMESTEP || field == OFX_F_IMPSR) OFX_TRY(materialize_timestep(c));
+    const bool perm = fb.kind == 2 ? !c->fperm.empty() : !c->cperm.empty();
+    std::vector<double> tmp(perm ? fb.len : 0);
+    for (size_t b = 0; b < fb.p.size(); ++b) {
+        double* dst = host + b * fb.len;
+        OFX_HIP(hipMemcpyAsync(perm ? tmp.data() : dst, fb.p[b], fb.len * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        if (!perm) continue;
+        OFX_HIP(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < fb.len; ++k) dst[k] = tmp[storage_index(c, fb.kind, k)];
     }
     OFX_HIP(hipStreamSynchronize(c->stream));
     return 0;
@@ -351,12 +398,33 @@ int ofx_get_field(ofx_ctx* c, int field, double* host) {
 int ofx_set_field(ofx_ctx* c, int field, const double* host) {
     CHECK_READY(c);
     if (!host) { set_error("null host pointer"); return OFX_EINVAL; }
-    double* p; size_t n;
+    double* p;
+    size_t n;
     if (field == OFX_F_DIST || field == OFX_F_VOL || field_view(c, field, &p, &n)) {
         set_error("field cannot be set"); return OFX_EINVAL;
     }
-    OFX_HIP(hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    FieldBlocks fb;
+    if (field_blocks(c, field, fb)) { set_error("unknown field id"); return OFX_EINVAL; }
+    const bool perm = !c->cperm.empty();
+    std::vector<double> tmp(perm ? fb.len : 0);
+    for (size_t b = 0; b < fb.p.size(); ++b) {
+        const double* src = host + b * fb.len;
+        if (perm) {
+            for (size_t k = 0; k < fb.len; ++k) tmp[storage_index(c, fb.kind, k)] = src[k];
+            src = tmp.data();
+        }
+        OFX_HIP(hipMemcpyAsync(fb.p[b], src, fb.len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if (perm) OFX_HIP(hipStreamSynchronize(c->stream));
+    }
     OFX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ofx_set_storage_order(ofx_ctx* c, int mode) {
+    CHECK_CTX(c);
+    if (c->has_mesh) { set_error("set_storage_order: call before ofx_mesh_upload"); return OFX_ESTATE; }
+    if (mode != 0 && mode != 1) { set_error("set_storage_order: mode must be 0 or 1"); return OFX_EINVAL; }
+    c->storage_order = mode;
     return 0;
 }
 

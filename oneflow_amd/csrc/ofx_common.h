@@ -74,6 +74,10 @@ struct DevMesh {
     int *ellf = nullptr, *elln = nullptr;
     // cell -> face CSR in ascending face id; entry = f if the cell is lc[f], ~f if rc[f]
     int *c2fOff = nullptr, *c2f = nullptr;
+    // per cell: bit j set when the neighbour of entry j of its face list has a
+    // lower cell id in the grid's numbering (the LU-SGS lower / upper split;
+    // storage positions need not follow that numbering)
+    int *lomask = nullptr;
     int *c2n = nullptr;  // neighbour cell of each c2f entry (ghost C+f on boundary faces)
     // LU-SGS level schedule: cells grouped by dependency depth
     int *loOrder = nullptr, *loOff = nullptr, *upOrder = nullptr, *upOff = nullptr;
@@ -156,6 +160,13 @@ struct ofx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool has_mesh = false, has_cfg = false;
+    // Storage order (chosen before the mesh upload): 0 the grid's own order,
+    // 1 spatial -- cells of each connected part in Morton order of their
+    // centroids, interior faces by their lower cell, boundary faces and ghosts
+    // unchanged.  cperm / fperm map a grid id to its storage position (empty:
+    // identity); host copies of fields are permuted on the way in and out.
+    int storage_order = 1;
+    std::vector<int> cperm, fperm;
     ofx_config cfg{};
     ofx::Consts k{};
     ofx::DevMesh m;

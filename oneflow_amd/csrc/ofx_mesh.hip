@@ -354,6 +354,110 @@ __global__ void __launch_bounds__(256) k_wall_distance(DevMesh m) {
 
 // ---------------------------------------------------------------- host side
 
+// Spatial storage order (host, once per mesh).  Cells: grouped by connected
+// part (parts in the order of their lowest cell id, so the cells of
+// disconnected zones stay in their id ranges), within a part sorted by the
+// Morton code of the centroid of their faces' node averages (21 bits per axis
+// in 3-D, 32 in 2-D, over the part's bounding box), ties by id.  Faces:
+// boundary faces keep their positions (their ghosts are cells C + f);
+// interior faces sorted by the positions of their two cells, lower first.
+static uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+static uint64_t spread2(uint64_t v) {  // 32 bits -> every second bit
+    v &= 0xffffffffull;
+    v = (v | v << 16) & 0x0000ffff0000ffffull;
+    v = (v | v << 8) & 0x00ff00ff00ff00ffull;
+    v = (v | v << 4) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | v << 2) & 0x3333333333333333ull;
+    v = (v | v << 1) & 0x5555555555555555ull;
+    return v;
+}
+
+static void spatial_order(int dim, int C, int F, int B, const double* x, const double* y, const double* z,
+                          const std::vector<int>& fnOff, const std::vector<int>& nodes, const std::vector<int>& lc,
+                          const std::vector<int>& rc, std::vector<int>& cperm, std::vector<int>& fperm) {
+    // centroid estimate: mean over the cell's faces of the face's node average
+    std::vector<double> cx(C, 0.0), cy(C, 0.0), cz(C, 0.0);
+    std::vector<int> cnt(C, 0);
+    for (int f = 0; f < F; ++f) {
+        double fx = 0, fy = 0, fz = 0;
+        int n = fnOff[f + 1] - fnOff[f];
+        for (int k = fnOff[f]; k < fnOff[f + 1]; ++k) { fx += x[nodes[k]]; fy += y[nodes[k]]; fz += z[nodes[k]]; }
+        fx /= n; fy /= n; fz /= n;
+        int cs[2] = {lc[f], f < B ? -1 : rc[f]};
+        for (int c : cs)
+            if (c >= 0) { cx[c] += fx; cy[c] += fy; cz[c] += fz; ++cnt[c]; }
+    }
+    for (int i = 0; i < C; ++i) { cx[i] /= cnt[i]; cy[i] /= cnt[i]; cz[i] /= cnt[i]; }
+    // connected parts (union-find over interior faces; root = lowest id)
+    std::vector<int> par(C);
+    std::iota(par.begin(), par.end(), 0);
+    auto find = [&](int a) { while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; } return a; };
+    for (int f = B; f < F; ++f) {
+        int a = find(lc[f]), b = find(rc[f]);
+        if (a != b) { if (a < b) par[b] = a; else par[a] = b; }
+    }
+    std::vector<int> part(C);
+    for (int i = 0; i < C; ++i) part[i] = find(i);
+    // per-part bounding boxes
+    std::vector<double> lo(3 * (size_t)C, 0.0), hi(3 * (size_t)C, 0.0);
+    std::vector<char> seen(C, 0);
+    for (int i = 0; i < C; ++i) {
+        int r = part[i];
+        double v[3] = {cx[i], cy[i], cz[i]};
+        for (int a = 0; a < 3; ++a) {
+            double& l = lo[3 * (size_t)r + a];
+            double& h = hi[3 * (size_t)r + a];
+            if (!seen[r]) { l = v[a]; h = v[a]; }
+            else { l = std::min(l, v[a]); h = std::max(h, v[a]); }
+        }
+        seen[r] = 1;
+    }
+    std::vector<uint64_t> key(C);
+    for (int i = 0; i < C; ++i) {
+        int r = part[i];
+        double v[3] = {cx[i], cy[i], cz[i]};
+        uint64_t q[3];
+        const double scale = dim == 3 ? 2097151.0 : 4294967295.0;
+        for (int a = 0; a < 3; ++a) {
+            double l = lo[3 * (size_t)r + a], h = hi[3 * (size_t)r + a];
+            double t = h > l ? (v[a] - l) / (h - l) : 0.0;
+            q[a] = (uint64_t)std::llround(std::min(1.0, std::max(0.0, t)) * scale);
+        }
+        key[i] = dim == 3 ? (spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2) : (spread2(q[0]) | spread2(q[1]) << 1);
+    }
+    std::vector<int> ord(C);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+        if (part[a] != part[b]) return part[a] < part[b];
+        if (key[a] != key[b]) return key[a] < key[b];
+        return a < b;
+    });
+    cperm.assign(C, 0);
+    for (int k = 0; k < C; ++k) cperm[ord[k]] = k;
+    std::vector<int> ford(F - B);
+    std::iota(ford.begin(), ford.end(), B);
+    auto lohi = [&](int f) {
+        int a = cperm[lc[f]], b = cperm[rc[f]];
+        return std::make_pair(std::min(a, b), std::max(a, b));
+    };
+    std::sort(ford.begin(), ford.end(), [&](int a, int b) {
+        auto ka = lohi(a), kb = lohi(b);
+        if (ka != kb) return ka < kb;
+        return a < b;
+    });
+    fperm.assign(F, 0);
+    for (int f = 0; f < B; ++f) fperm[f] = f;
+    for (int k = 0; k < F - B; ++k) fperm[ford[k]] = B + k;
+}
+
 int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, const int* fnc,
                const int* fnn, const int* lcIn, const int* rcIn, const int* bct) {
     DevMesh& m = c->m;
@@ -385,15 +489,44 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
         if (lc[f] < 0 || lc[f] >= C || rc[f] < 0 || rc[f] >= C || lc[f] == rc[f]) {
             set_error("ofx_mesh_upload: interior face cell index out of range"); return OFX_EINVAL;
         }
-    // CellTopo::CalcC2f (codes/geometry/src/CellTopo.cpp:95-119): ascending face ids
+    // ---- storage order: grid id -> storage position of cells and faces
+    std::vector<int> cperm, fperm;
+    if (c->storage_order == 1) spatial_order(m.dim, C, F, B, x, y, z, fnOff, nodes, lc, rc, cperm, fperm);
+    else {
+        cperm.resize(C); fperm.resize(F);
+        std::iota(cperm.begin(), cperm.end(), 0);
+        std::iota(fperm.begin(), fperm.end(), 0);
+    }
+    std::vector<int> cinv(C), finv(F);
+    for (int i = 0; i < C; ++i) cinv[cperm[i]] = i;
+    for (int f = 0; f < F; ++f) finv[fperm[f]] = f;
+    bool ident = true;
+    for (int i = 0; i < C && ident; ++i) ident = cperm[i] == i;
+    for (int f = 0; f < F && ident; ++f) ident = fperm[f] == f;
+    if (ident) { c->cperm.clear(); c->fperm.clear(); }
+    else { c->cperm = cperm; c->fperm = fperm; }
+    // faces in storage order: cells by position, ghosts of boundary face f at C + f
+    {
+        std::vector<int> lc2(F), rc2(F), off2(F + 1, 0), nodes2(nodes.size());
+        for (int s2 = 0; s2 < F; ++s2) {
+            int f = finv[s2];
+            lc2[s2] = cperm[lc[f]];
+            rc2[s2] = f < B ? C + f : cperm[rc[f]];
+            off2[s2 + 1] = off2[s2] + (fnOff[f + 1] - fnOff[f]);
+            std::copy(nodes.begin() + fnOff[f], nodes.begin() + fnOff[f + 1], nodes2.begin() + off2[s2]);
+        }
+        lc.swap(lc2); rc.swap(rc2); fnOff.swap(off2); nodes.swap(nodes2);
+    }
+    // CellTopo::CalcC2f (codes/geometry/src/CellTopo.cpp:95-119): each cell's
+    // faces in ascending face id (grid numbering), as storage positions
     std::vector<int> off(T + 1, 0);
-    for (int f = 0; f < B; ++f) off[lc[f] + 1]++;
-    for (int f = B; f < F; ++f) { off[lc[f] + 1]++; off[rc[f] + 1]++; }
+    for (int f = 0; f < B; ++f) off[lc[fperm[f]] + 1]++;
+    for (int f = B; f < F; ++f) { off[lc[fperm[f]] + 1]++; off[rc[fperm[f]] + 1]++; }
     for (int i = 0; i < T; ++i) off[i + 1] += off[i];
     std::vector<int> c2f(off[T]);
     std::vector<int> pos(off.begin(), off.end() - 1);
-    for (int f = 0; f < B; ++f) c2f[pos[lc[f]]++] = f;
-    for (int f = B; f < F; ++f) { c2f[pos[lc[f]]++] = f; c2f[pos[rc[f]]++] = ~f; }
+    for (int f = 0; f < B; ++f) { int s2 = fperm[f]; c2f[pos[lc[s2]]++] = s2; }
+    for (int f = B; f < F; ++f) { int s2 = fperm[f]; c2f[pos[lc[s2]]++] = s2; c2f[pos[rc[s2]]++] = ~s2; }
     std::vector<int> c2n(off[T]);
     for (int k = 0; k < off[T]; ++k) {
         int e = c2f[k], f = e >= 0 ? e : ~e;
@@ -402,28 +535,34 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     int maxf = 0;
     for (int i = 0; i < C; ++i) maxf = std::max(maxf, off[i + 1] - off[i]);
     m.maxFacesPerCell = maxf;
-    // LU-SGS level schedule (exact: a cell depends only on lower / upper
-    // neighbours, UNsLusgs::CanNotLowerSolve / CanNotUpperSolve,
-    // codes/uns/src/UNsLusgs.cpp:174-196)
-    std::vector<int> lev(C, 0), levU(C, 0);
+    if (maxf > 31) { set_error("ofx_mesh_upload: cells with more than 31 faces are not supported"); return OFX_EINVAL; }
+    // grid id of the neighbour of each entry (interior faces; ghosts -1)
+    auto nb_id = [&](int k) { int j = c2n[k]; return j < C ? cinv[j] : -1; };
+    std::vector<int> lomask(C, 0);
+    for (int sc = 0; sc < C; ++sc)
+        for (int k = off[sc]; k < off[sc + 1]; ++k) {
+            int j = nb_id(k);
+            if (j >= 0 && j < cinv[sc]) lomask[sc] |= 1 << (k - off[sc]);
+        }
+    // LU-SGS level schedule in the grid's numbering (exact: a cell depends
+    // only on lower / upper neighbours, UNsLusgs::CanNotLowerSolve /
+    // CanNotUpperSolve, codes/uns/src/UNsLusgs.cpp:174-196); the level lists
+    // hold storage positions, ascending within a level
+    std::vector<int> lev(C, 0), levU(C, 0);  // by grid id
     int nlo = 0, nup = 0;
     for (int i = 0; i < C; ++i) {
-        int mx = -1;
-        for (int k = off[i]; k < off[i + 1]; ++k) {
-            int e = c2f[k], f = e >= 0 ? e : ~e;
-            if (f < B) continue;
-            int j = (e >= 0) ? rc[f] : lc[f];
-            if (j < i) mx = std::max(mx, lev[j]);
+        int sc = cperm[i], mx = -1;
+        for (int k = off[sc]; k < off[sc + 1]; ++k) {
+            int j = nb_id(k);
+            if (j >= 0 && j < i) mx = std::max(mx, lev[j]);
         }
         lev[i] = mx + 1;
         nlo = std::max(nlo, lev[i] + 1);
     }
     for (int i = C - 1; i >= 0; --i) {
-        int mx = -1;
-        for (int k = off[i]; k < off[i + 1]; ++k) {
-            int e = c2f[k], f = e >= 0 ? e : ~e;
-            if (f < B) continue;
-            int j = (e >= 0) ? rc[f] : lc[f];
+        int sc = cperm[i], mx = -1;
+        for (int k = off[sc]; k < off[sc + 1]; ++k) {
+            int j = nb_id(k);
             if (j > i) mx = std::max(mx, levU[j]);
         }
         levU[i] = mx + 1;
@@ -435,7 +574,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
         for (int l = 0; l < nl; ++l) lo[l + 1] += lo[l];
         ord.assign(C, 0);
         std::vector<int> p(lo.begin(), lo.end() - 1);
-        for (int i = 0; i < C; ++i) ord[p[L[i]]++] = i;
+        for (int sc = 0; sc < C; ++sc) ord[p[L[cinv[sc]]]++] = sc;  // storage order within a level
     };
     std::vector<int> loOrd, upOrd;
     order_by(lev, nlo, loOrd, c->sched.loOff);
@@ -452,14 +591,14 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     runs(loOrd, c->sched.loOff, c->sched.loBase);
     runs(upOrd, c->sched.upOff, c->sched.upBase);
     m.nLevLo = nlo; m.nLevUp = nup;
-    // Segments for XCD placement: a numbering whose cells are sorted by
-    // lower-sweep level with at most 8 levels is a colouring stored colour by
-    // colour; each colour is one segment, and the interior faces (sorted by
-    // left cell) split at the same cells after the boundary-face segment.
+    // Segments for XCD placement: cells stored sorted by lower-sweep level with
+    // at most 8 levels are a colouring stored colour by colour; each colour is
+    // one segment, and the interior faces (sorted by left cell) split at the
+    // same cells after the boundary-face segment.
     m.nsegc = 1; m.segc[0] = 0; m.segc[1] = C;
     m.nsegf = 1; m.segf[0] = 0; m.segf[1] = F;
     bool sorted = nlo <= MAXSEG - 1;
-    for (int i = 1; i < C && sorted; ++i) sorted = lev[i] >= lev[i - 1];
+    for (int sc = 1; sc < C && sorted; ++sc) sorted = lev[cinv[sc]] >= lev[cinv[sc - 1]];
     for (int f = B + 1; f < F && sorted; ++f) sorted = lc[f] >= lc[f - 1];
     if (sorted && nlo > 1) {
         m.nsegc = nlo;
@@ -502,6 +641,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(A((void**)&m.c2fOff, sizeof(int) * (T + 1)));
     OFX_TRY(A((void**)&m.c2f, sizeof(int) * std::max(1, off[T])));
     OFX_TRY(A((void**)&m.c2n, sizeof(int) * std::max(1, off[T])));
+    OFX_TRY(A((void**)&m.lomask, sizeof(int) * C));
     m.ellw = maxf;
     std::vector<int> ellf((size_t)maxf * C, 0), elln((size_t)maxf * C, 0);
     for (int i = 0; i < C; ++i)
@@ -528,6 +668,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(upload(c, m.c2fOff, off.data(), sizeof(int) * (T + 1)));
     OFX_TRY(upload(c, m.c2f, c2f.data(), sizeof(int) * off[T]));
     OFX_TRY(upload(c, m.c2n, c2n.data(), sizeof(int) * off[T]));
+    OFX_TRY(upload(c, m.lomask, lomask.data(), sizeof(int) * C));
     OFX_TRY(upload(c, m.ellf, ellf.data(), sizeof(int) * ellf.size()));
     OFX_TRY(upload(c, m.elln, elln.data(), sizeof(int) * elln.size()));
     OFX_TRY(upload(c, m.loOrder, loOrd.data(), sizeof(int) * C));

@@ -1062,11 +1062,13 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
+    const int lomask = m.lomask[i];
     bool act[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         int f = ent[j] >= 0 ? ent[j] : ~ent[j];
-        act[j] = j < n && f >= m.B && (UPPER ? nb[j] > i : nb[j] < i);
+        const bool lower = (lomask >> j) & 1;  // neighbour's id below the cell's
+        act[j] = j < n && f >= m.B && (UPPER ? !lower : lower);
     }
     double rl = d.q[i], ul = d.q[T + i], vl = d.q[2 * T + i], wl = d.q[3 * T + i], pl = d.q[4 * T + i];
     double xi = 0.0, yi = 0.0, zi = 0.0;

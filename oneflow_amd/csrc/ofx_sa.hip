@@ -417,11 +417,13 @@ __device__ __forceinline__ void sa_lusgs_cell(const DevMesh& m, const DevFields&
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
+    const int lomask = m.lomask[i];
     bool act[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         int f = ent[j] >= 0 ? ent[j] : ~ent[j];
-        act[j] = j < n && f >= m.B && (UPPER ? nb[j] > i : nb[j] < i);
+        const bool lower = (lomask >> j) & 1;  // neighbour's id below the cell's
+        act[j] = j < n && f >= m.B && (UPPER ? !lower : lower);
     }
     double mat[MAXF], dqn[MAXF];
 #pragma unroll

@@ -59,6 +59,7 @@ def load_library():
     L.ofx_mesh_upload.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ip, ip, ip, ip, ip]
     L.ofx_mesh_info.argtypes = [P, ip]
+    L.ofx_set_storage_order.argtypes = [P, ctypes.c_int]
     L.ofx_config_set.argtypes = [P, ctypes.c_void_p]
     for name in ("ofx_init_freestream", "ofx_ns_boundary", "ofx_sa_boundary", "ofx_residual_ns",
                  "ofx_residual_sa", "ofx_lusgs_ns", "ofx_lusgs_sa", "ofx_update_ns", "ofx_update_sa", "ofx_sync"):
@@ -102,12 +103,14 @@ def _ip(a):
 class Solver:
     """One zone + case resident on one GPU (device buffers owned by the context)."""
 
-    def __init__(self, grid: Grid, cfg: CaseConfig, device: int = 0):
+    def __init__(self, grid: Grid, cfg: CaseConfig, device: int = 0, storage_order=None):
         L = load_library()
         self._L = L
         h = ctypes.c_void_p()
         self._check(L.ofx_ctx_create(ctypes.byref(h), device))
         self.h = h
+        if storage_order is not None:  # 0 the grid's order, 1 spatial (the library's default)
+            self._check(L.ofx_set_storage_order(h, storage_order))
         g = grid
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z))
         fnc, fnn, lc, rc, bct = (np.ascontiguousarray(a, dtype=np.int32) for a in
@@ -314,10 +317,10 @@ class ZoneSolver(Solver):
     per-case freestream constants and reductions.  Each case's results equal
     a Solver on that case alone, bit for bit."""
 
-    def __init__(self, grids, cfgs, device: int = 0):
+    def __init__(self, grids, cfgs, device: int = 0, storage_order=None):
         from .meshgen import concat_grids
         g, cs, bs = concat_grids(grids)
-        super().__init__(g, cfgs[0], device)
+        super().__init__(g, cfgs[0], device, storage_order)
         self.n_zones = len(grids)
         self.cell_start, self.bface_start = cs, bs
         self.zone_grids = list(grids)

@@ -33,7 +33,7 @@ def test_oracle_exports_mirror():
     for n in declared():
         if n in ("ofx_ctx_create", "ofx_ctx_destroy", "ofx_mesh_upload", "ofx_sync",
                  "ofx_time_kernel", "ofx_copy_bench", "ofx_set_face_tile", "ofx_set_kernel_variant", "ofx_get_kernel_variant",
-                 "ofx_autotune") or \
+                 "ofx_autotune", "ofx_set_storage_order") or \
                 n.startswith("ofx_batch_") or n.startswith("ofx_mlp_") or n.startswith("ofx_zones_"):
             continue  # context plumbing, GPU launch tuning, batching, the predictor: no CPU counterpart
         assert hasattr(L, n.replace("ofx_", "ofo_")), n
