@@ -7,6 +7,8 @@
 // codes/multigrid/src/Multigrid.cpp:339-375).
 #include "ofx_common.h"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -57,7 +59,6 @@ static int alloc_fields(ofx_ctx* c) {
     OFX_TRY(A(&d.turbdq, T));
     OFX_TRY(A(&d.grad, 18 * T));
     OFX_TRY(A(&d.tgrad, 3 * T));
-    OFX_TRY(A(&d.minmax, 10 * C));
     OFX_TRY(A(&d.limiter, 5 * T));
     OFX_TRY(A(&d.timestep, T));
     OFX_TRY(A(&d.invsr, T));
@@ -138,6 +139,7 @@ static void set_consts(ofx_ctx* c) {
 
 int init_freestream(ofx_ctx* c) {
     OFX_TRY(ensure_dist(c));
+    ++c->qgen;
     const long T = c->m.T;
     if (c->m.nz > 1) {
         // each zone's cells and ghosts take that zone's freestream
@@ -273,6 +275,7 @@ int ofx_config_set(ofx_ctx* c, const ofx_config* cfg) {
     c->cfg = *cfg;
     set_consts(c);
     c->has_cfg = true;
+    c->qext_gen = -1;
     ++c->gen;
     return 0;
 }
@@ -416,6 +419,7 @@ int ofx_set_field(ofx_ctx* c, int field, const double* host) {
         OFX_HIP(hipMemcpyAsync(fb.p[b], src, fb.len * sizeof(double), hipMemcpyHostToDevice, c->stream));
         if (perm) OFX_HIP(hipStreamSynchronize(c->stream));
     }
+    if (field == OFX_F_Q) ++c->qgen;
     OFX_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -483,7 +487,7 @@ int step_launch(ofx_ctx* c, int iter) {
         // the SA solver registers no CALC_TIME_STEP and uses the NS time step
         OFX_TRY(ns_timestep(c, iter));
         for (int s = 0; s < c->cfg.rk_stage; ++s) {
-            OFX_TRY(ns_residual(c));
+            OFX_TRY(ns_residual(c, true));  // stage 0 reuses the time step's extrema
             OFX_TRY(lhs(c, 0, 1.0 / (s + 1)));
             OFX_TRY(ns_update_fused(c));
         }
@@ -502,7 +506,7 @@ int step_launch(ofx_ctx* c, int iter) {
     const int L0 = fusable_l0(c);
     const int ns_l0 = c->gather_variant == 1 ? L0 : 0;
     c->fuse_l0 = ns_l0;
-    int rc = ns_residual(c);
+    int rc = ns_residual(c, true);
     c->fuse_l0 = 0;
     OFX_TRY(rc);
     OFX_TRY(ns_lusgs(c, ns_l0 > 0 ? 1 : 0));
@@ -612,8 +616,12 @@ int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
                   "variant 0 or 1");
         return OFX_EINVAL;
     }
-    (kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : kernel == 2 ? c->gather_variant : c->face_split) =
-        variant;
+    if (kernel <= 1 && variant != 0) {
+        set_error("kernel variant: the gradient and limiter run as one kernel with a single form (variant 0)");
+        return OFX_EINVAL;
+    }
+    if (kernel <= 1) return 0;
+    (kernel == 2 ? c->gather_variant : c->face_split) = variant;
     ++c->gen;
     return 0;
 }
@@ -621,10 +629,26 @@ int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
     if (!variant || kernel < 0 || kernel > 3) { set_error("kernel variant: kernel 0..3 and non-null out"); return OFX_EINVAL; }
-    *variant = kernel == 0 ? c->grad_variant : kernel == 1 ? c->lim_variant : kernel == 2 ? c->gather_variant
-                                                                                          : c->face_split;
+    *variant = kernel <= 1 ? 0 : kernel == 2 ? c->gather_variant : c->face_split;
     return 0;
 }
+
+}  // extern "C"
+
+namespace {
+// Median over 5 repetitions of the average launch time of one residual part
+// (`iters` back-to-back launches each): a single short timing is at the mercy
+// of clocks and co-running work (a profiler, for one).
+int median_ms(ofx_ctx* c, int which, int iters, double* ms) {
+    double t[5];
+    for (double& x : t) OFX_TRY(ofx_time_kernel(c, which, iters, &x));
+    std::sort(t, t + 5);
+    *ms = t[2];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
 
 int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     CHECK_READY(c);
@@ -635,35 +659,31 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
     c->face_split = 0;
     for (int t : cand) {
         c->face_tb = t;
-        OFX_TRY(ofx_time_kernel(c, 2, iters, &ms));
+        OFX_TRY(median_ms(c, 2, iters, &ms));
         if (ms < best) { best = ms; pick = t; }
     }
     c->face_tb = pick;
     // the split face kernel (viscous FP64 runs), against the best tile
     if (c->k.vismodel != 0 && !c->k.flux_fp32) {
         c->face_split = 1;
-        OFX_TRY(ofx_time_kernel(c, 2, iters, &ms));
+        OFX_TRY(median_ms(c, 2, iters, &ms));
         c->face_split = ms < best ? 1 : 0;
     }
-    // NS gradient, limiter and gather variants (kernel parts 0, 1, 3)
-    int* var[3] = {&c->grad_variant, &c->lim_variant, &c->gather_variant};
-    const int part[3] = {0, 1, 3};
-    for (int q = 0; q < 3; ++q) {
-        best = 1e300;
-        pick = *var[q];
-        for (int v = 0; v < 2; ++v) {
-            *var[q] = v;
-            OFX_TRY(ofx_time_kernel(c, part[q], iters, &ms));
-            if (ms < best) { best = ms; pick = v; }
-        }
-        *var[q] = pick;
+    // the residual gather's two forms (kernel part 3)
+    best = 1e300;
+    pick = c->gather_variant;
+    for (int v = 0; v < 2; ++v) {
+        c->gather_variant = v;
+        OFX_TRY(median_ms(c, 3, iters, &ms));
+        if (ms < best) { best = ms; pick = v; }
     }
+    c->gather_variant = pick;
     if (c->k.vismodel == 3) {
         best = 1e300;
         pick = c->saface_tb;
         for (int t : cand) {
             c->saface_tb = t;
-            OFX_TRY(ofx_time_kernel(c, 5, iters, &ms));
+            OFX_TRY(median_ms(c, 5, iters, &ms));
             if (ms < best) { best = ms; pick = t; }
         }
         c->saface_tb = pick;
@@ -746,6 +766,7 @@ int ofx_zones_set(ofx_ctx* c, int n, const int* cellStart, const int* bfaceStart
     OFX_HIP(hipGetLastError());
     OFX_TRY(A((void**)&d.zred, sizeof(double) * (size_t)n * RQ_N));
     c->par_dt = c->par_lim = c->par_fix = 0;
+    c->qext_gen = -1;
     m.nz = n;
     c->zc.assign(cellStart, cellStart + n + 1);
     c->zb.assign(bfaceStart, bfaceStart + n + 1);

@@ -138,7 +138,6 @@ struct DevFields {
     double *turbdq = nullptr;   // [T]
     double *grad = nullptr;     // [3][6][T]: x,y,z of rho,u,v,w,p,T
     double *tgrad = nullptr;    // [3][T]: SA gradient
-    double *minmax = nullptr;   // [2][5][C] Venkat neighbour min / max differences
     double *limiter = nullptr;  // [5][T]
     double *timestep = nullptr; // [T]
     double *invsr = nullptr;    // [T]
@@ -174,9 +173,11 @@ struct ofx_ctx {
     ofx::DevFields d;
     std::vector<void*> allocs;
     int par_dt = 0, par_lim = 0, par_fix = 0;  // parity of the reduction buffers in use
+    // qgen counts host calls that may change the primitive state; qext_gen is
+    // its value when the limiter's grid extrema were last reduced (by the
+    // time step or k_ns_qext), so a residual of the same state reuses them
+    int qgen = 0, qext_gen = -1;
     int face_tb = 256, saface_tb = 256;        // face-kernel tile (threads per block)
-    int grad_variant = 0;  // NS gradient: 0 one thread per cell, 1 one wave per quantity
-    int lim_variant = 1;     // limiter: 0 lanes per cell, 1 one thread per cell
     int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
     int face_split = 0;      // viscous NS face kernel: 1 inviscid and viscous parts on separate waves
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
@@ -568,8 +569,8 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
 int ns_boundary(ofx_ctx* c);
 int sa_boundary(ofx_ctx* c);
 int ns_timestep(ofx_ctx* c, int iter);
-int ns_residual(ofx_ctx* c);
-int ns_residual_parts(ofx_ctx* c, unsigned parts);
+int ns_residual(ofx_ctx* c, bool reuse_ext = false);
+int ns_residual_parts(ofx_ctx* c, unsigned parts, bool reuse_ext = true);
 int step_launch(ofx_ctx* c, int iter);
 int sa_residual(ofx_ctx* c);
 int sa_residual_parts(ofx_ctx* c, unsigned parts);  // bits: gradient, face terms, cell terms

@@ -112,6 +112,7 @@ __global__ void __launch_bounds__(TB) k_ns_bc(DevMesh m, DevFields d, Consts k0)
 }
 
 int ns_boundary(ofx_ctx* c) {
+    ++c->qgen;  // ghost states change
     k_ns_cellstate<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->k);
     OFX_HIP(hipGetLastError());
     if (c->m.B) k_ns_bc<<<nblk(c->m.B), TB, 0, c->stream>>>(c->m, c->d, c->k);
@@ -126,14 +127,20 @@ int ns_boundary(ofx_ctx* c) {
 // UTimeStep::CalcInvSpectrumField / CalcVisSpectrumField / CalcLocalTimeStep
 // (codes/uns/src/UTimeStep.cpp:190-416, codes/ns/src/TimeStep.cpp:43-133):
 // per cell, the face spectral radii are summed in ascending face order.
-template <int DIM, int MAXF>
-__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k0, int par) {
+// QEXT: the same pass also reduces the extrema of the neighbour differences
+// that fix the Venkatakrishnan epsilon (Limiter::CalcMinMaxDiff, the global
+// part, codes/usolver/src/ULimiter.cpp:597-644): per primitive the largest
+// q_i - min(q_i, q_j) and max(q_i, q_j) - q_i over interior faces, into the
+// limiter's slots of parity par_lim; the residual's gradient kernel reads them.
+template <int DIM, int MAXF, bool QEXT>
+__global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k0, int par, int par_lim) {
     int i = cell_block(m, TB) * TB + threadIdx.x;
     const int T = m.T;
     const int zone = (i < m.C) ? cell_zone(m, i) : -1;
     const Consts k = zoned(k0, m, zone);
     const bool visc = k.vismodel > 0;
     double dtv = LARGE;
+    double ext[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < m.C) {
         // every load of the cell's faces is issued before the sums; each face
         // is then evaluated with its own left / right roles, in face order.
@@ -154,6 +161,17 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
                 ar[j] = m.area[f];
                 if (visc) { vln[j] = d.visl[c]; vtn[j] = d.vist[c]; }
             }
+        }
+        if (QEXT) {
+            double mn[5], mx[5];
+            for (int e = 0; e < 5; ++e) { mn[e] = qi[e]; mx[e] = qi[e]; }
+#pragma unroll
+            for (int j = 0; j < MAXF; ++j) {
+                int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+                if (j < n && f >= m.B)
+                    for (int e = 0; e < 5; ++e) { mn[e] = dmin(mn[e], qn[j][e]); mx[e] = dmax(mx[e], qn[j][e]); }
+            }
+            for (int e = 0; e < 5; ++e) { mn[e] -= qi[e]; mx[e] -= qi[e]; ext[e] = 0.0 - mn[e]; ext[5 + e] = mx[e]; }
         }
         double invsr = 0.0, vissr = 0.0;
 #pragma unroll
@@ -211,6 +229,41 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
     }
     if (m.nz > 1) zslot_reduce<1, false>(&dtv, zone, d.slots, par, RQ_N, RQ_DTMIN, m.nz);
     else slot_reduce<1, false>(&dtv, d.slots, par, RQ_N, RQ_DTMIN);
+    if (QEXT) {
+        if (m.nz > 1) zslot_reduce<10, true>(ext, zone, d.slots, par_lim, RQ_N, RQ_QMIN, m.nz);
+        else slot_reduce<10, true>(ext, d.slots, par_lim, RQ_N, RQ_QMIN);
+    }
+}
+
+// The limiter's extrema alone (the QEXT part of k_ns_timestep), for a residual
+// evaluated on a state no time step has seen.
+template <int DIM, int MAXF>
+__global__ void __launch_bounds__(TB) k_ns_qext(DevMesh m, DevFields d, int par_lim) {
+    int i = cell_block(m, TB) * TB + threadIdx.x;
+    const int T = m.T;
+    const int zone = (i < m.C) ? cell_zone(m, i) : -1;
+    double ext[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < m.C) {
+        const int n = m.c2fOff[i + 1] - m.c2fOff[i];
+        int ent[MAXF], nb[MAXF];
+        cell_faces<MAXF>(m, i, ent, nb);
+        double qi[5], qn[MAXF][5];
+        for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j)
+            for (int e = 0; e < 5; ++e) qn[j][e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + nb[j]];
+        double mn[5], mx[5];
+        for (int e = 0; e < 5; ++e) { mn[e] = qi[e]; mx[e] = qi[e]; }
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            int f = ent[j] >= 0 ? ent[j] : ~ent[j];
+            if (j < n && f >= m.B)
+                for (int e = 0; e < 5; ++e) { mn[e] = dmin(mn[e], qn[j][e]); mx[e] = dmax(mx[e], qn[j][e]); }
+        }
+        for (int e = 0; e < 5; ++e) { mn[e] -= qi[e]; mx[e] -= qi[e]; ext[e] = 0.0 - mn[e]; ext[5 + e] = mx[e]; }
+    }
+    if (m.nz > 1) zslot_reduce<10, true>(ext, zone, d.slots, par_lim, RQ_N, RQ_QMIN, m.nz);
+    else slot_reduce<10, true>(ext, d.slots, par_lim, RQ_N, RQ_QMIN);
 }
 
 
@@ -262,18 +315,31 @@ int ns_timestep(ofx_ctx* c, int iter) {
     c->par_dt ^= 1;
     const int mf = c->m.maxFacesPerCell;
     const DevMesh& m = c->m;
+    // the Venkatakrishnan extrema of this state come with the time step
+    const bool qext = c->k.ilim == 2;
+    if (qext) c->par_lim ^= 1;
+    const int pd = c->par_dt, pl = c->par_lim;
+#define OFX_TS(D, MF)                                                                                             \
+    (qext ? (k_ns_timestep<D, MF, true><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0)             \
+          : (k_ns_timestep<D, MF, false><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0))
     if (m.dim == 2) {
-        if (mf <= 4) k_ns_timestep<2, 4><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
-        else if (mf <= 8) k_ns_timestep<2, 8><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+        if (mf <= 4) OFX_TS(2, 4);
+        else if (mf <= 8) OFX_TS(2, 8);
         else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
     } else {
-        if (mf <= 6) k_ns_timestep<3, 6><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
-        else if (mf <= 16) k_ns_timestep<3, 16><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+        if (mf <= 6) OFX_TS(3, 6);
+        else if (mf <= 16) OFX_TS(3, 16);
         else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
     }
-    if (c->m.nz > 1)
+#undef OFX_TS
+    if (c->m.nz > 1) {
         k_zone_extract<false><<<c->m.nz, 64, 0, c->stream>>>(c->d.slots, c->par_dt, RQ_N, RQ_DTMIN, 1, c->m.nz, c->d.zred);
+        if (qext)
+            k_zone_extract<true><<<c->m.nz, 256, 0, c->stream>>>(c->d.slots, c->par_lim, RQ_N, RQ_QMIN, 10, c->m.nz,
+                                                                 c->d.zred);
+    }
     OFX_HIP(hipGetLastError());
+    if (qext) c->qext_gen = c->qgen;
     return 0;
 }
 
@@ -281,271 +347,108 @@ int ns_timestep(ofx_ctx* c, int iter) {
 
 // Green-Gauss gradient with inverse-distance face weights of rho,u,v,w,p and T
 // (CalcGradGGCellWeight, codes/usolver/src/UGrad.cpp:88-149) as a per-cell
-// gather; optionally the neighbour min/max differences of the limiter
-// (Limiter::CalcMinMaxDiff, codes/usolver/src/ULimiter.cpp:597-644).
-// The face loop is unrolled to MAXF: all loads of the cell's faces are issued
-// before the contributions are added in ascending face order.  DIM == 2 skips
-// the z components and w, which are identically zero in 2-D.
-template <int DIM, int MAXF, bool MINMAX>
-__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k0, int par) {
-    int i = cell_block(m, TB) * TB + threadIdx.x;
-    const int T = m.T;
-    const int zone = (i < m.C) ? cell_zone(m, i) : -1;
-    const Consts k = zoned(k0, m, zone);
-    double mn[5], mx[5];
-    bool live = i < m.C;
-    if (live) {
-        const int n = m.c2fOff[i + 1] - m.c2fOff[i];
-        int ent[MAXF], nb[MAXF];
-        cell_faces<MAXF>(m, i, ent, nb);
-        double qi[5];
-        for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
-        dvec4 gw[MAXF];
-        double gz[MAXF], qn[MAXF][5];
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            if (j < n) {
-                int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-                gw[j] = m.fgg[f];
-                gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
-                for (int e2 = 0; e2 < 5; ++e2) qn[j][e2] = (DIM == 2 && e2 == 3) ? 0.0 : d.q[e2 * T + c];
-            }
-        }
-        for (int e = 0; e < 5; ++e) { mn[e] = qi[e]; mx[e] = qi[e]; }
-        double ti = temperature(qi[0], qi[4], k.statecoef);
-        double acc[6][3];
-        for (int s = 0; s < 6; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.0;
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            if (j < n) {
-                bool own = ent[j] >= 0;  // this cell is the face's left cell
-                double cl = gw[j].x, cr = gw[j].y;
-                double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
-                double tn = temperature(qn[j][0], qn[j][4], k.statecoef);
-                for (int s = 0; s < 6; ++s) {
-                    double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
-                    double value = own ? (cl * vi + cr * vn) : (cl * vn + cr * vi);
-                    if (own) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
-                    else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
-                }
-                int f = own ? ent[j] : ~ent[j];
-                if (MINMAX && f >= m.B)
-                    for (int s = 0; s < 5; ++s) { mn[s] = dmin(mn[s], qn[j][s]); mx[s] = dmax(mx[s], qn[j][s]); }
-            }
-        }
-        // In 2-D the z components and w's gradient stay at the zeros the
-        // arrays were allocated with (w = 0 and z-extent 0 make them exactly 0).
-        double ovol = 1.0 / m.vol[i];
-        for (int s = 0; s < 6; ++s) {
-            if (DIM == 2 && s == 3) continue;
-            for (int a = 0; a < DIM; ++a) d.grad[(a * 6 + s) * T + i] = acc[s][a] * ovol;
-        }
-        if (MINMAX) {
-            for (int s = 0; s < 5; ++s) {
-                mn[s] -= qi[s]; mx[s] -= qi[s];
-                if (DIM == 2 && s == 3) continue;
-                d.minmax[s * m.C + i] = mn[s];
-                d.minmax[(5 + s) * m.C + i] = mx[s];
-            }
-        }
-    }
-    if (MINMAX) {
-        double v[10];
-        for (int s = 0; s < 5; ++s) { v[s] = live ? 0.0 - mn[s] : 0.0; v[5 + s] = live ? mx[s] : 0.0; }
-        if (m.nz > 1) zslot_reduce<10, true>(v, zone, d.slots, par, RQ_N, RQ_QMIN, m.nz);
-        else slot_reduce<10, true>(v, d.slots, par, RQ_N, RQ_QMIN);
-    }
-}
-
-// The same gradient with one wave per quantity: a block covers 64 cells and
-// holds one wave for each of rho, u, v, (w,) p and T, so every thread gathers
-// a single quantity over its cell's faces.  Six (3-D) or five (2-D) times the
-// waves of k_ns_grad and a fraction of its registers: more loads in flight on
-// grids too small to fill the chip with one thread per cell.  Each quantity's
-// sums are formed as in k_ns_grad, bit for bit; the neighbour min/max of the
-// five primitives are reduced per wave.
-template <int DIM, int MAXF, bool MINMAX>
-__global__ void __launch_bounds__(384) k_ns_grad_q(DevMesh m, DevFields d, Consts k0, int par) {
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    // wave -> quantity: rho, u, v, (w,) p, T (index 5)
-    const int s = (DIM == 2) ? (w < 3 ? w : w + 1) : w;
-    const int i = cell_block(m, 64) * 64 + lane;
-    const int T = m.T;
-    const bool live = i < m.C;
-    const int zone = live ? cell_zone(m, i) : -1;
-    const Consts k = zoned(k0, m, zone);
-    double mn = 0.0, mx = 0.0;
-    if (live) {
-        const int n = m.c2fOff[i + 1] - m.c2fOff[i];
-        int ent[MAXF], nb[MAXF];
-        cell_faces<MAXF>(m, i, ent, nb);
-        double vi = (s < 5) ? d.q[s * T + i] : temperature(d.q[i], d.q[4 * T + i], k.statecoef);
-        dvec4 gw[MAXF];
-        double gz[MAXF], vn[MAXF];
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            if (j < n) {
-                int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-                gw[j] = m.fgg[f];
-                gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
-                vn[j] = (s < 5) ? d.q[s * T + c] : temperature(d.q[c], d.q[4 * T + c], k.statecoef);
-            }
-        }
-        mn = vi; mx = vi;
-        double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int j = 0; j < MAXF; ++j) {
-            if (j < n) {
-                bool own = ent[j] >= 0;
-                double cl = gw[j].x, cr = gw[j].y;
-                double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
-                double value = own ? (cl * vi + cr * vn[j]) : (cl * vn[j] + cr * vi);
-                if (own) { acc[0] += fnxa * value; acc[1] += fnya * value; acc[2] += fnza * value; }
-                else { acc[0] -= fnxa * value; acc[1] -= fnya * value; acc[2] -= fnza * value; }
-                int f = own ? ent[j] : ~ent[j];
-                if (MINMAX && s < 5 && f >= m.B) { mn = dmin(mn, vn[j]); mx = dmax(mx, vn[j]); }
-            }
-        }
-        double ovol = 1.0 / m.vol[i];
-        for (int a = 0; a < DIM; ++a) d.grad[(a * 6 + s) * T + i] = acc[a] * ovol;
-        if (MINMAX && s < 5) {
-            mn -= vi; mx -= vi;
-            d.minmax[s * m.C + i] = mn;
-            d.minmax[(5 + s) * m.C + i] = mx;
-        }
-    }
-    if (MINMAX && s < 5) {  // wave-uniform: s depends on the wave only
-        double lo = live ? 0.0 - mn : 0.0, hi = live ? mx : 0.0;
-        zslot_reduce<1, true>(&lo, zone, d.slots, par, RQ_N, RQ_QMIN + s, m.nz);
-        zslot_reduce<1, true>(&hi, zone, d.slots, par, RQ_N, RQ_QMAX + s, m.nz);
-    }
-}
-
-// Venkatakrishnan / Barth limiter per cell: minimum over the cell's faces
-// (Limiter::CalcVencatLimiter / CalcBarthLimiter, codes/usolver/src/ULimiter.cpp:464-595).
-// LPC lanes per cell; the minimum is exact in any order.
-template <int DIM, int LPC, int LIM>
-__global__ void __launch_bounds__(TB) k_ns_limiter(DevMesh m, DevFields d, Consts k, int par) {
-    __shared__ double sext[10];
-    {
-        // the 10 limiter extrema: one wave per quantity group, one latency round
-        const unsigned long long* mine = d.slots + (size_t)par * m.nz * RQ_N * NSLOT;
-        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        double x[3];
-        for (int r = 0; r < 3; ++r) {
-            int q = w + 4 * r;
-            x[r] = (q < 10) ? __longlong_as_double((long long)mine[(RQ_QMIN + q) * NSLOT + lane]) : 0.0;
-        }
-        for (int r = 0; r < 3; ++r) {
-            int q = w + 4 * r;
-            double y = x[r];
-            for (int o = 32; o > 0; o >>= 1) { double z = __shfl_xor(y, o); y = (y > z) ? y : z; }
-            if (q < 10 && lane == 0) sext[q] = y;
-        }
-        __syncthreads();
-    }
-    const int T = m.T;
-    int gid = cell_block(m, blockDim.x / LPC) * blockDim.x + threadIdx.x;
-    int i = gid / LPC, j = gid % LPC;
-    bool live = i < m.C;
-    int n = 0;
-    if (live) n = m.c2fOff[i + 1] - m.c2fOff[i];
-    // entry j of the cell's face list, loaded without waiting for the count
-    const int ej = (live && j < m.ellw) ? m.ellf[(size_t)j * m.C + i] : 0;
-    bool has = live && j < n;
-    double dx = 0, dy = 0, dz = 0, dot = 0;
-    if (has) {
-        int e = ej;
-        int f = e >= 0 ? e : ~e;
-        dvec4 rc4 = m.frec[f];  // face centre relative to the left / right cell centre
-        dx = e >= 0 ? rc4.x : rc4.z;
-        dy = e >= 0 ? rc4.y : rc4.w;
-        dz = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
-        if (LIM == 1) {
-            double ds = sqrt(sqr3(dx, dy, dz));
-            double nz = (DIM == 3) ? m.zfn[f] : 0.0;
-            dot = (m.xfn[f] * dx + m.yfn[f] * dy + nz * dz) / (ds + SMALL);
-            if (e < 0) dot = -dot;
-        }
-    }
-    for (int s = 0; s < 5; ++s) {
-        double lim = 1.0;
-        bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
-        if (has && !skip) {
-            double gx = d.grad[(0 * 6 + s) * T + i];
-            double gy = d.grad[(1 * 6 + s) * T + i];
-            double gz = (DIM == 3) ? d.grad[(2 * 6 + s) * T + i] : 0.0;
-            double mn = d.minmax[s * m.C + i];
-            double mx = d.minmax[(5 + s) * m.C + i];
-            double dq = gx * dx + gy * dy + gz * dz;
-            if (LIM == 2) {
-                double lo = sext[s], hi = sext[5 + s];
-                if (m.nz > 1) {
-                    const double* zr = d.zred + (size_t)m.zcell[i] * RQ_N;
-                    lo = zr[RQ_QMIN + s];
-                    hi = zr[RQ_QMAX + s];
-                }
-                double eps = k.vencat_coef * dmax(hi - (-lo), 1.0);
-                double eps1 = eps * eps + SMALL;
-                lim = vencat_fn(dq, mn, mx, eps1);
-            } else {
-                lim = barth_fn(dq, mn, mx, dot);
-            }
-        }
-        lim = group_min<LPC>(lim);
-        if (live && j == 0) d.limiter[s * T + i] = dmin(1.0, lim);
-    }
-}
-
-// The same limiter with one thread per cell: the cell's gradients and
-// neighbour extents are loaded once and its faces walked in a loop (the
-// minimum is exact in any order).
+// gather, and with LIM = 1 (Barth) or 2 (Venkatakrishnan) the cell's limiter
+// in the same pass (Limiter::CalcLimiter with CalcMinMaxDiff and
+// CalcBarthLimiter / CalcVencatLimiter, codes/usolver/src/ULimiter.cpp:53-97,
+// 377-644): the neighbour min/max differences are the ones the gradient's face
+// loop already holds, the limiter of a face needs only the cell's own
+// gradient, and the Venkatakrishnan epsilon comes from the grid extrema that
+// the time-step kernel (or k_ns_qext) reduced on this state into the slots of
+// parity `par`.  The face loop is unrolled to MAXF: all loads of the cell's
+// faces are issued before the contributions are added in ascending face
+// order.  DIM == 2 skips the z components and w, identically zero in 2-D.
 template <int DIM, int MAXF, int LIM>
-__global__ void __launch_bounds__(TB) k_ns_limiter_c(DevMesh m, DevFields d, Consts k, int par) {
-    const int T = m.T;
-    // Every lane stays active to the end (the extrema are reduced per wave, with
-    // shuffles): lanes past the last cell evaluate the last cell and store nothing.
+__global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k0, int par) {
+    // Every lane stays active to the end (the extrema are read per wave):
+    // lanes past the last cell evaluate it and store nothing.
     const int i0 = cell_block(m, TB) * TB + threadIdx.x;
     const bool live = i0 < m.C;
     const int i = live ? i0 : m.C - 1;
+    const int T = m.T;
+    const Consts k = zoned(k0, m, cell_zone(m, i));
     double ext[10];
     if (LIM == 2 && m.nz == 1) wave_slot_result<10, true>(d.slots, par, m.nz * RQ_N, RQ_QMIN, ext);
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
-    // all entries are loaded (padding names face 0) and masked in the minimum
-    double dx[MAXF], dy[MAXF], dz[MAXF], dot[MAXF];
+    double qi[5];
+    for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
+    dvec4 gw[MAXF];
+    double gz[MAXF], qn[MAXF][5], dx[MAXF], dy[MAXF], dz[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        int e = ent[j], f = e >= 0 ? e : ~e;
-        dvec4 rc4 = m.frec[f];
-        dx[j] = e >= 0 ? rc4.x : rc4.z;
-        dy[j] = e >= 0 ? rc4.y : rc4.w;
-        dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
-        if (LIM == 1) {
-            double ds = sqrt(sqr3(dx[j], dy[j], dz[j]));
-            double nz = (DIM == 3) ? m.zfn[f] : 0.0;
-            dot[j] = (m.xfn[f] * dx[j] + m.yfn[f] * dy[j] + nz * dz[j]) / (ds + SMALL);
-            if (e < 0) dot[j] = -dot[j];
+        if (j < n) {
+            int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
+            gw[j] = m.fgg[f];
+            gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
+            for (int e2 = 0; e2 < 5; ++e2) qn[j][e2] = (DIM == 2 && e2 == 3) ? 0.0 : d.q[e2 * T + c];
+            if (LIM != 0) {
+                // face centre relative to this cell's centre
+                dvec4 rc4 = m.frec[f];
+                dx[j] = e >= 0 ? rc4.x : rc4.z;
+                dy[j] = e >= 0 ? rc4.y : rc4.w;
+                dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
+            }
         }
     }
-    double gxs[5], gys[5], gzs[5], mns[5], mxs[5];
+    double mn[5], mx[5];
+    for (int e = 0; e < 5; ++e) { mn[e] = qi[e]; mx[e] = qi[e]; }
+    double ti = temperature(qi[0], qi[4], k.statecoef);
+    double acc[6][3];
+    for (int s = 0; s < 6; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.0;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        const bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
-        gxs[s] = skip ? 0.0 : d.grad[(0 * 6 + s) * T + i];
-        gys[s] = skip ? 0.0 : d.grad[(1 * 6 + s) * T + i];
-        gzs[s] = (DIM == 3 && !skip) ? d.grad[(2 * 6 + s) * T + i] : 0.0;
-        mns[s] = skip ? 0.0 : d.minmax[s * m.C + i];
-        mxs[s] = skip ? 0.0 : d.minmax[(5 + s) * m.C + i];
+    for (int j = 0; j < MAXF; ++j) {
+        if (j < n) {
+            bool own = ent[j] >= 0;  // this cell is the face's left cell
+            double cl = gw[j].x, cr = gw[j].y;
+            double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
+            double tn = temperature(qn[j][0], qn[j][4], k.statecoef);
+            for (int s = 0; s < 6; ++s) {
+                double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
+                double value = own ? (cl * vi + cr * vn) : (cl * vn + cr * vi);
+                if (own) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
+                else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
+            }
+            int f = own ? ent[j] : ~ent[j];
+            if (LIM != 0 && f >= m.B)
+                for (int s = 0; s < 5; ++s) { mn[s] = dmin(mn[s], qn[j][s]); mx[s] = dmax(mx[s], qn[j][s]); }
+        }
+    }
+    // In 2-D the z components and w's gradient stay at the zeros the arrays
+    // were allocated with (w = 0 and z-extent 0 make them exactly 0).
+    double ovol = 1.0 / m.vol[i];
+    double g[6][3];
+    for (int s = 0; s < 6; ++s)
+        for (int a = 0; a < 3; ++a) g[s][a] = acc[s][a] * ovol;
+    if (live)
+        for (int s = 0; s < 6; ++s) {
+            if (DIM == 2 && s == 3) continue;
+            for (int a = 0; a < DIM; ++a) d.grad[(a * 6 + s) * T + i] = g[s][a];
+        }
+    if (LIM == 0) return;
+    double dot[MAXF];
+    if (LIM == 1) {
+#pragma unroll
+        for (int j = 0; j < MAXF; ++j) {
+            if (j < n) {
+                int e = ent[j], f = e >= 0 ? e : ~e;
+                double ds = sqrt(sqr3(dx[j], dy[j], dz[j]));
+                double nzf = (DIM == 3) ? m.zfn[f] : 0.0;
+                dot[j] = (m.xfn[f] * dx[j] + m.yfn[f] * dy[j] + nzf * dz[j]) / (ds + SMALL);
+                if (e < 0) dot[j] = -dot[j];
+            }
+        }
     }
     double lims[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
         double lim = 1.0;
-        const bool skip = (LIM == 2 && DIM == 2 && s == 3);
+        const bool skip = (LIM == 2 && DIM == 2 && s == 3);  // Venkat on w in 2-D: every face gives 1
         if (!skip) {
-            double gx = gxs[s], gy = gys[s], gz = gzs[s], mn = mns[s], mx = mxs[s];
+            // w in 2-D (Barth): gradient and extents are exactly zero, as stored
+            const bool w2 = DIM == 2 && s == 3;
+            double gx = w2 ? 0.0 : g[s][0], gy = w2 ? 0.0 : g[s][1], gzs = (DIM == 3) ? g[s][2] : 0.0;
+            double lo_d = w2 ? 0.0 : mn[s] - qi[s], hi_d = w2 ? 0.0 : mx[s] - qi[s];
             double eps1 = 0.0;
             if (LIM == 2) {
                 double lo, hi;
@@ -562,9 +465,11 @@ __global__ void __launch_bounds__(TB) k_ns_limiter_c(DevMesh m, DevFields d, Con
             }
 #pragma unroll
             for (int j = 0; j < MAXF; ++j) {
-                double dq = gx * dx[j] + gy * dy[j] + gz * dz[j];
-                double phi = (LIM == 2) ? vencat_fn(dq, mn, mx, eps1) : barth_fn(dq, mn, mx, dot[j]);
-                lim = (j < n) ? dmin(lim, phi) : lim;
+                if (j < n) {
+                    double dq = gx * dx[j] + gy * dy[j] + gzs * dz[j];
+                    double phi = (LIM == 2) ? vencat_fn(dq, lo_d, hi_d, eps1) : barth_fn(dq, lo_d, hi_d, dot[j]);
+                    lim = dmin(lim, phi);
+                }
             }
         }
         lims[s] = dmin(1.0, lim);
@@ -870,41 +775,41 @@ static void launch_face_dim(ofx_ctx* c, bool visc) {
     else { if (slau2) launch_face<DIM, 1, true>(c, visc); else launch_face<DIM, 1, false>(c, visc); }
 }
 
-// NS residual kernels; `parts` selects them (bit 0 gradient + min/max, bit 1
-// limiter, bit 2 face fluxes, bit 3 gather) so each can be timed alone.
-// MAXF bounds the faces per cell (unrolled loops); the lanes-per-cell kernels
-// use LPC, the next power of two (their group reductions are butterflies).
+// The limiter's grid extrema for the current state, unless the time step of
+// this state produced them (k_ns_timestep<.., QEXT>); `force` launches the
+// pass regardless (timing).
 template <int DIM, int MAXF>
-static void launch_cells(ofx_ctx* c, unsigned parts) {
+static void launch_qext(ofx_ctx* c, bool force) {
+    if (c->k.ilim != 2 || (!force && c->qext_gen == c->qgen)) return;
+    c->par_lim ^= 1;
+    k_ns_qext<DIM, MAXF><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, c->d, c->par_lim);
+    if (c->m.nz > 1)
+        k_zone_extract<true><<<c->m.nz, 256, 0, c->stream>>>(c->d.slots, c->par_lim, RQ_N, RQ_QMIN, 10, c->m.nz,
+                                                             c->d.zred);
+    c->qext_gen = c->qgen;
+}
+
+// NS residual kernels; `parts` selects them (bit 0 gradient + limiter, bit 1
+// the limiter's grid extrema, bit 2 face fluxes, bit 3 gather) so each can be
+// timed alone.  MAXF bounds the faces per cell (unrolled loops); the
+// lanes-per-cell gather uses LPC, the next power of two (its group
+// reductions are butterflies).
+template <int DIM, int MAXF>
+static void launch_cells(ofx_ctx* c, unsigned parts, bool reuse_ext) {
     constexpr int LPC = MAXF <= 4 ? 4 : (MAXF <= 8 ? 8 : 16);
     const Consts& k = c->k;
     const int C1 = c->m.C;         // threads: one per cell
     const int C = c->m.C * LPC;    // threads: LPC lanes per cell
     bool visc = k.vismodel != 0;
-    bool lim = k.ilim == 1 || k.ilim == 2;
+    if (parts & 2u) launch_qext<DIM, MAXF>(c, true);
     if (parts & 1u) {
-        const bool split = c->grad_variant == 1;
-        const int qw = (DIM == 2) ? 5 : 6;  // waves (quantities) per block of k_ns_grad_q
-        const int nbq = (C1 + 63) / 64;
-        if (lim) {
-            c->par_lim ^= 1;
-            if (split) k_ns_grad_q<DIM, MAXF, true><<<nbq, 64 * qw, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-            else k_ns_grad<DIM, MAXF, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-            if (c->m.nz > 1)
-                k_zone_extract<true><<<c->m.nz, 256, 0, c->stream>>>(c->d.slots, c->par_lim, RQ_N, RQ_QMIN, 10,
-                                                                     c->m.nz, c->d.zred);
+        if (k.ilim == 2) {
+            if (!(parts & 2u)) launch_qext<DIM, MAXF>(c, !reuse_ext);
+            k_ns_grad<DIM, MAXF, 2><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+        } else if (k.ilim == 1) {
+            k_ns_grad<DIM, MAXF, 1><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
         } else {
-            if (split) k_ns_grad_q<DIM, MAXF, false><<<nbq, 64 * qw, 0, c->stream>>>(c->m, c->d, k, 0);
-            else k_ns_grad<DIM, MAXF, false><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
-        }
-    }
-    if ((parts & 2u) && lim) {
-        if (c->lim_variant == 1) {
-            if (k.ilim == 2) k_ns_limiter_c<DIM, MAXF, 2><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-            else k_ns_limiter_c<DIM, MAXF, 1><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-        } else {
-            if (k.ilim == 2) k_ns_limiter<DIM, LPC, 2><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
-            else k_ns_limiter<DIM, LPC, 1><<<nblk(C), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            k_ns_grad<DIM, MAXF, 0><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
         }
     }
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
@@ -936,23 +841,26 @@ static void launch_cells(ofx_ctx* c, unsigned parts) {
     }
 }
 
-int ns_residual_parts(ofx_ctx* c, unsigned parts) {
+// reuse_ext: the limiter's extrema of this state may come from its time step
+// (inside a pseudo-step); otherwise the residual reduces them itself, so a
+// standalone UPDATE_RESIDUALS does the whole limiter's work.
+int ns_residual_parts(ofx_ctx* c, unsigned parts, bool reuse_ext) {
     int mf = c->m.maxFacesPerCell;
     if (c->m.dim == 2) {
-        if (mf <= 4) launch_cells<2, 4>(c, parts);
-        else if (mf <= 8) launch_cells<2, 8>(c, parts);
+        if (mf <= 4) launch_cells<2, 4>(c, parts, reuse_ext);
+        else if (mf <= 8) launch_cells<2, 8>(c, parts, reuse_ext);
         else { set_error("cells with more than 8 faces are not supported"); return OFX_EINVAL; }
     } else {
-        if (mf <= 6) launch_cells<3, 6>(c, parts);
-        else if (mf <= 8) launch_cells<3, 8>(c, parts);
-        else if (mf <= 16) launch_cells<3, 16>(c, parts);
+        if (mf <= 6) launch_cells<3, 6>(c, parts, reuse_ext);
+        else if (mf <= 8) launch_cells<3, 8>(c, parts, reuse_ext);
+        else if (mf <= 16) launch_cells<3, 16>(c, parts, reuse_ext);
         else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
     }
     OFX_HIP(hipGetLastError());
     return 0;
 }
 
-int ns_residual(ofx_ctx* c) { return ns_residual_parts(c, 0xFu); }
+int ns_residual(ofx_ctx* c, bool reuse_ext) { return ns_residual_parts(c, 0xFu & ~2u, reuse_ext); }
 
 // Ghost-cell gradients / limiter for field download (Grad::CalcGrad tail,
 // codes/usolver/src/UGrad.cpp:142-148)
@@ -1317,6 +1225,7 @@ __global__ void __launch_bounds__(TB) k_ns_update_apply(DevMesh m, DevFields d, 
 
 int ns_update(ofx_ctx* c) {
     const DevFields d = update_fields(c);
+    ++c->qgen;
     c->par_fix ^= 1;
     k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
     k_ns_update_apply<false><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
@@ -1326,6 +1235,7 @@ int ns_update(ofx_ctx* c) {
 
 int ns_update_fused(ofx_ctx* c) {
     const DevFields d = update_fields(c);
+    ++c->qgen;
     c->par_fix ^= 1;
     k_ns_update_check<<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);
     k_ns_update_apply<true><<<nblk(c->m.C), TB, 0, c->stream>>>(c->m, d, c->k, c->par_fix);

@@ -239,17 +239,14 @@ class Solver:
         """Threads per block of the NS / SA face-flux kernels (64..512)."""
         self._check(self._L.ofx_set_face_tile(self.h, ns_tile, sa_tile))
 
-    def set_grad_variant(self, variant):
-        """NS gradient kernel: 0 one thread per cell, 1 one wave per quantity."""
-        self._check(self._L.ofx_set_kernel_variant(self.h, 0, variant))
-
     def kernel_variant(self, kernel):
         v = np.zeros(1, np.int32)
         self._check(self._L.ofx_get_kernel_variant(self.h, kernel, _ip(v)))
         return int(v[0])
 
     def set_kernel_variant(self, kernel, variant):
-        """kernel 0 gradient, 1 limiter, 2 gather, 3 NS face split; variant 0 or 1 (results identical)."""
+        """kernel 2 gather, 3 NS face split: variant 0 or 1 (results identical); kernels 0 and 1
+        (the fused gradient and limiter) have the single form 0."""
         self._check(self._L.ofx_set_kernel_variant(self.h, kernel, variant))
 
     def autotune(self, iters=20):

@@ -134,11 +134,11 @@ def test_fifty_steps_against_goldens(case):
     gpu, ora = make(case)
     gpu.init_freestream()
     ora.init_freestream()
-    norms, aero, rows = [], [], []
+    norms, aero, aero_rows = [], [], []
     for it in range(1, 51):
         norms.append(gpu.iterate(it, 1)[0])
         aero.append(gpu.aero())
-        rows.append(gpu.aero_row())
+        aero_rows.append(gpu.aero_row())
     norms = np.array(norms)
     onorms = ora.iterate(1, 50)
     e = report("norms", norms, onorms)
@@ -159,7 +159,7 @@ def test_fifty_steps_against_goldens(case):
             assert harness_equal("%.4e" % mine, "%.4e" % float(ref)), (case, it + 1, aero[it], row[3:5])
     # the whole aero.dat file (15 columns) from the GPU's rows
     from tests.test_oracle_golden import compare_aero_file
-    compare_aero_file(case, aero_lines(1, rows), (GOLDEN / case / "aero.dat").read_text().splitlines())
+    compare_aero_file(case, aero_lines(1, aero_rows), (GOLDEN / case / "aero.dat").read_text().splitlines())
     # wallaero.dat after step 50: per wall face -cp and cf at 11 digits
     wall = gpu.wall_data()
     ew = report("wall", wall, ora.wall_data())

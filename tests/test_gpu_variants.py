@@ -1,9 +1,8 @@
 """Launch variants give identical bits.
 
-The NS gradient (one thread per cell; one wave per quantity), the limiter
-and the residual gather (one lane per face; one thread per cell) have two
-kernels each, the viscous face kernel a split form (inviscid and viscous
-parts on separate waves), and the face kernels take any face tile.  Every combination must give the
+The residual gather has two kernels (one lane per face; one thread per
+cell), the viscous face kernel a split form (inviscid and viscous parts on
+separate waves), and the face kernels take any face tile.  Every combination must give the
 same state, gradients, limiter and residual bit for bit, here after twelve
 pseudo-time steps from freestream (through the CFL ramp) on 2-D and 3-D grids
 and several option sets.  Variant 0 with the default tile is checked against
@@ -21,15 +20,14 @@ FIELDS = ["Q", "TURBQ", "DQDX", "LIMITER", "RES", "TIMESTEP", "DQ"]
 
 @pytest.mark.parametrize("grid", ["quad", "tri", "mixed", "wing"])
 @pytest.mark.parametrize("opt", ["roe_venkat_sa_test", "roe_barth_lam_std", "slau2_o1_lam_aver_iso"])
-def test_grad_variant_and_tiles_bitwise(grid, opt):
+def test_variants_and_tiles_bitwise(grid, opt):
     from oneflow_amd.solver import Solver
     base, _, g = make(grid, opt)
     runs = [base]
-    # (gradient, limiter, gather, face split) variants and face tiles
-    for variants, tiles in (((1, 0, 0, 1), (256, 256)), ((1, 1, 1, 0), (64, 128)),
-                            ((0, 1, 0, 1), (512, 64)), ((0, 0, 1, 0), (128, 512))):
+    # (gather, face split) variants and face tiles
+    for variants, tiles in (((0, 1), (256, 256)), ((1, 0), (64, 128)), ((0, 1), (512, 64)), ((1, 0), (128, 512))):
         s = Solver(g, base.cfg)
-        for kernel, v in enumerate(variants):
+        for kernel, v in zip((2, 3), variants):
             s.set_kernel_variant(kernel, v)
         s.set_face_tile(*tiles)
         runs.append(s)
@@ -43,3 +41,13 @@ def test_grad_variant_and_tiles_bitwise(grid, opt):
             assert np.array_equal(a, s.get(n)), (grid, opt, n, k)
     print(f"\n[variants] {grid} {opt}: {len(runs)} launch configurations agree", flush=True)
 
+
+
+def test_gradient_and_limiter_have_one_form():
+    from oneflow_amd.solver import OfxError
+    base, _, g = make("quad", "roe_venkat_sa_test")
+    base.set_kernel_variant(0, 0)
+    with pytest.raises(OfxError):
+        base.set_kernel_variant(0, 1)
+    with pytest.raises(OfxError):
+        base.set_kernel_variant(1, 1)

@@ -51,12 +51,13 @@ def boxes():
 
 @pytest.mark.parametrize("which,variant", [("natural", 0), ("mc", 0), ("boxes3d", 0), ("mc", 1), ("boxes3d", 1)])
 def test_zones_equal_single_cases(which, variant):
+    # variant: the residual gather's form (kernel 2)
     from oneflow_amd.solver import Solver, ZoneSolver
     grids, cfgs = boxes() if which == "boxes3d" else airfoils(which)
-    print(f"\n[zones] {which} gradient variant {variant}: cells {[g.n_cells for g in grids]}", flush=True)
+    print(f"\n[zones] {which} gather variant {variant}: cells {[g.n_cells for g in grids]}", flush=True)
     single = [Solver(g, c) for g, c in zip(grids, cfgs)]
     multi = ZoneSolver(grids, cfgs)
-    multi.set_grad_variant(variant)
+    multi.set_kernel_variant(2, variant)
     for z, s in enumerate(single):
         assert np.array_equal(multi.get("DIST")[multi.zone_slice(z)[0]], s.get("DIST")), ("DIST", z)
     for s in single + [multi]:
