@@ -888,6 +888,10 @@ int fill_ghost_grad(ofx_ctx* c) {
 // has put i on the left): the split flux increment of the neighbour r and,
 // with viscosity, its viscous radius.  Own state (rl..pl) and neighbour data
 // are passed in, loaded by the caller.
+// VM: the viscous term of the face, 0 none (inviscid), 1 spectral-radius
+// model 1 (neighbour volume), 2 models 2 and 3 (centre distance) -- a
+// compile-time choice, so a face evaluates only its own model.
+template <int VM>
 __device__ __forceinline__ void ns_lusgs_face(const Consts& k, double xfn, double yfn, double zfn, double vfn,
                                               double farea, double rl, double ul, double vl, double wl, double pl,
                                               const double* pj, const double* dqj, double visl, double vist,
@@ -936,9 +940,9 @@ __device__ __forceinline__ void ns_lusgs_face(const Consts& k, double xfn, doubl
     dfj[3] = lmd1 * dqj[3] - pj[3] * term1 + zfn * term2;
     dfj[4] = lmd1 * dqj[4] - hmj * term1 + vn_fluid * term2;
     for (int e = 0; e < 5; ++e) dfj[e] *= farea;
-    if (k.vismodel != 0) {
+    if (VM != 0) {
         double visrad;
-        if (k.visSRModel == 1) {
+        if (VM == 1) {
             double density = 0.5 * (rl + rr);
             double c1 = 4.0 / 3.0 * (visl + vist);
             double cc2 = k.gama * (visl * k.oprl + vist * k.oprt);
@@ -960,13 +964,13 @@ __device__ __forceinline__ void ns_lusgs_face(const Consts& k, double xfn, doubl
 // upper-sweep contribution through them is exactly +0.  The cell's faces are
 // unrolled to MAXF: all neighbour loads are issued first, then the face
 // terms are added in ascending face order.
-template <bool UPPER, int MAXF>
+template <bool UPPER, int MAXF, int VM>
 __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int i, double mindt1,
                                               bool store = true) {
     const int T = m.T;
     const Consts k = zoned(k0, m, cell_zone(m, i));
     const double mindt = zone_mindt(m, d, i, mindt1);
-    const bool visc = k.vismodel != 0, sr1 = k.visSRModel == 1;
+    constexpr bool visc = VM != 0, sr1 = VM == 1;
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
     cell_faces<MAXF>(m, i, ent, nb);
@@ -1009,8 +1013,8 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
                 if (sr1) volr = rx[j];
                 else dist = dabs(xfn * (rx[j] - xi) + yfn * (ry[j] - yi) + zfn * (rz[j] - zi));
             }
-            ns_lusgs_face(k, xfn, yfn, zfn, vfn, ar[j], rl, ul, vl, wl, pl, pj[j], dqj[j], vl_[j], vt_[j], volr, dist,
-                          rhs0);
+            ns_lusgs_face<VM>(k, xfn, yfn, zfn, vfn, ar[j], rl, ul, vl, wl, pl, pj[j], dqj[j], vl_[j], vt_[j], volr,
+                              dist, rhs0);
         }
     }
     double rad = ns_impsr(m, d, k, i, mindt);
@@ -1024,7 +1028,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
 
 // Whole sweep in one workgroup: levels in order, a workgroup barrier between
 // levels (every cell of a level depends only on earlier levels).
-template <bool UPPER, int MAXF>
+template <bool UPPER, int MAXF, int VM>
 __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k, int par) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     const int* off = UPPER ? m.upOff : m.loOff;
@@ -1032,14 +1036,14 @@ __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, C
     int nl = UPPER ? m.nLevUp : m.nLevLo;
     for (int L = 0; L < nl; ++L) {
         for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
-            ns_lusgs_cell<UPPER, MAXF>(m, d, k, ord[idx], mindt);
+            ns_lusgs_cell<UPPER, MAXF, VM>(m, d, k, ord[idx], mindt);
         __syncthreads();
     }
 }
 
 // One level per launch (wide levels on large meshes).
 // base >= 0: the level's cells are base, base + 1, ... (no order lookup).
-template <bool UPPER, int MAXF>
+template <bool UPPER, int MAXF, int VM>
 __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int base,
                                                        int par) {
     // Every lane stays active (the minimum time step is reduced per wave, its
@@ -1051,7 +1055,7 @@ __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, C
     double mindt;
     wave_slot_result<1, false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN, &mindt);
     int i = base >= 0 ? base + (idx - b) : (UPPER ? m.upOrder : m.loOrder)[idx];
-    ns_lusgs_cell<UPPER, MAXF>(m, d, k, i, mindt, live);
+    ns_lusgs_cell<UPPER, MAXF, VM>(m, d, k, i, mindt, live);
 }
 
 // Launch policy: few wide levels (multicolour numbering) -> one grid-wide
@@ -1062,15 +1066,15 @@ bool use_level_launches(const std::vector<int>& off) {
     return nl <= 64;
 }
 
-template <int MAXF>
+template <int MAXF, int VM>
 static void launch_lusgs(ofx_ctx* c, int lo_first) {
     const DevMesh& m = c->m;
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
         const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
         if (!use_level_launches(off)) {
-            if (up) k_ns_sweep_1wg<true, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
-            else k_ns_sweep_1wg<false, MAXF><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            if (up) k_ns_sweep_1wg<true, MAXF, VM><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            else k_ns_sweep_1wg<false, MAXF, VM><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
         } else {
             const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
             // upper level 0 holds the cells without a higher neighbour: their
@@ -1079,12 +1083,21 @@ static void launch_lusgs(ofx_ctx* c, int lo_first) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
                 if (up)
-                    k_ns_sweep_level<true, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L], c->par_dt);
+                    k_ns_sweep_level<true, MAXF, VM><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L],
+                                                                                       c->par_dt);
                 else
-                    k_ns_sweep_level<false, MAXF><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L], c->par_dt);
+                    k_ns_sweep_level<false, MAXF, VM><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L],
+                                                                                        c->par_dt);
             }
         }
     }
+}
+
+template <int MAXF>
+static void launch_lusgs_vm(ofx_ctx* c, int lo_first) {
+    if (c->k.vismodel == 0) launch_lusgs<MAXF, 0>(c, lo_first);
+    else if (c->k.visSRModel == 1) launch_lusgs<MAXF, 1>(c, lo_first);
+    else launch_lusgs<MAXF, 2>(c, lo_first);
 }
 
 // The first level of the lower sweep that may be fused into the residual
@@ -1098,10 +1111,10 @@ int fusable_l0(const ofx_ctx* c) {
 
 int ns_lusgs(ofx_ctx* c, int lo_first) {
     int mf = c->m.maxFacesPerCell;
-    if (mf <= 4) launch_lusgs<4>(c, lo_first);
-    else if (mf <= 6) launch_lusgs<6>(c, lo_first);
-    else if (mf <= 8) launch_lusgs<8>(c, lo_first);
-    else if (mf <= 16) launch_lusgs<16>(c, lo_first);
+    if (mf <= 4) launch_lusgs_vm<4>(c, lo_first);
+    else if (mf <= 6) launch_lusgs_vm<6>(c, lo_first);
+    else if (mf <= 8) launch_lusgs_vm<8>(c, lo_first);
+    else if (mf <= 16) launch_lusgs_vm<16>(c, lo_first);
     else { set_error("cells with more than 16 faces are not supported"); return OFX_EINVAL; }
     OFX_HIP(hipGetLastError());
     return 0;

