@@ -611,9 +611,9 @@ int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
 
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel < 0 || kernel > 3 || variant < 0 || variant > 1) {
-        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather) or 3 (NS face split), "
-                  "variant 0 or 1");
+    if (kernel < 0 || kernel > 4 || variant < 0 || variant > 1) {
+        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather), 3 (NS face split) or "
+                  "4 (SA face geometry), variant 0 or 1");
         return OFX_EINVAL;
     }
     if (kernel <= 1 && variant != 0) {
@@ -621,15 +621,15 @@ int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
         return OFX_EINVAL;
     }
     if (kernel <= 1) return 0;
-    (kernel == 2 ? c->gather_variant : c->face_split) = variant;
+    (kernel == 2 ? c->gather_variant : kernel == 3 ? c->face_split : c->sa_geom_variant) = variant;
     ++c->gen;
     return 0;
 }
 
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
-    if (!variant || kernel < 0 || kernel > 3) { set_error("kernel variant: kernel 0..3 and non-null out"); return OFX_EINVAL; }
-    *variant = kernel <= 1 ? 0 : kernel == 2 ? c->gather_variant : c->face_split;
+    if (!variant || kernel < 0 || kernel > 4) { set_error("kernel variant: kernel 0..4 and non-null out"); return OFX_EINVAL; }
+    *variant = kernel <= 1 ? 0 : kernel == 2 ? c->gather_variant : kernel == 3 ? c->face_split : c->sa_geom_variant;
     return 0;
 }
 
@@ -687,6 +687,17 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
             if (ms < best) { best = ms; pick = t; }
         }
         c->saface_tb = pick;
+        // the TEST diffusion's face geometry: stored records or evaluated (part 7, the SA face terms)
+        if (c->k.iturb_visflux == 2) {
+            best = 1e300;
+            pick = c->sa_geom_variant;
+            for (int v = 0; v < 2; ++v) {
+                c->sa_geom_variant = v;
+                OFX_TRY(median_ms(c, 7, iters, &ms));
+                if (ms < best) { best = ms; pick = v; }
+            }
+            c->sa_geom_variant = pick;
+        }
     }
     ++c->gen;
     // leave the residual fields as one evaluation of the current state leaves them

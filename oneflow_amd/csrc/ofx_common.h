@@ -179,6 +179,7 @@ struct ofx_ctx {
     int qgen = 0, qext_gen = -1;
     int face_tb = 256, saface_tb = 256;        // face-kernel tile (threads per block)
     int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
+    int sa_geom_variant = 0;  // SA diffusion, TEST method: 0 stored face records, 1 evaluated from the centres
     int face_split = 0;      // viscous NS face kernel: 1 inviscid and viscous parts on separate waves
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
     bool dist_valid = false;  // wall distance computed for the current mesh / zones

@@ -145,7 +145,10 @@ __device__ __forceinline__ double sa_face_dqdn(const DevMesh& m, int f, double x
 // codes/turb/src/TurbInvFlux.cpp:78-98) and diffusion
 // (UTurbVisFlux::CalcFaceVisFlux1Equ, codes/uturb/src/UTurbVisFlux.cpp:247-369).
 // saface rows: 0 radius, 1 matrix_l, 2 matrix_r, 3 convection, 4 diffusion.
-template <int DIM>
+// GEO 1 (TEST method only): the face-gradient geometry is evaluated from the
+// cell and face centres (vis_geom, the expressions the mesh upload used for
+// the stored 160-byte record, so the same bits) instead of read from the record.
+template <int DIM, int GEO>
 __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts k0, int tb) {
     int f = face_block(m, tb) * tb + threadIdx.x;
     if (f >= m.F) return;
@@ -168,7 +171,7 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     dvec4 rec[5];
     double g1[3], g2[3];
     auto diffusion_loads = [&]() {
-        vis_rec_load(m, k0.iturb_visflux, f, rec);
+        if (GEO == 0) vis_rec_load(m, k0.iturb_visflux, f, rec);
         g1[0] = d.tgrad[l]; g1[1] = d.tgrad[T + l]; g1[2] = (DIM == 3) ? d.tgrad[2 * T + l] : 0.0;
         g2[0] = d.tgrad[rg]; g2[1] = d.tgrad[T + rg]; g2[2] = (DIM == 3) ? d.tgrad[2 * T + rg] : 0.0;
     };
@@ -214,7 +217,12 @@ __global__ void __launch_bounds__(512) k_sa_face(DevMesh m, DevFields d, Consts 
     // diffusion (face-gradient method uniform: one dispatch, separate instantiations)
     if (DIM == 3) diffusion_loads();
     double dqdn;
-    if (k0.iturb_visflux == 0) dqdn = sa_face_dqdn<0>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
+    if (GEO == 1) {
+        VisGeom G;
+        vis_geom(m, f, l, r, 2, G);
+        double qf, gf[3];
+        vis_grad<2>(G, nl, nr, g1, g2, qf, gf, dqdn);
+    } else if (k0.iturb_visflux == 0) dqdn = sa_face_dqdn<0>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
     else if (k0.iturb_visflux == 1) dqdn = sa_face_dqdn<1>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
     else dqdn = sa_face_dqdn<2>(m, f, xfn, yfn, gzf, rec, nl, nr, g1, g2);
     double visl_nv = 0.5 * (visll * orl + vislr * orr);
@@ -382,9 +390,13 @@ template <int DIM, int MAXF>
 static void launch_sa(ofx_ctx* c, unsigned parts) {
     const DevMesh& m = c->m;
     if (parts & 1u) k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
-    if (parts & 2u)
-        k_sa_face<DIM><<<(m.F + c->saface_tb - 1) / c->saface_tb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k,
-                                                                                                c->saface_tb);
+    if (parts & 2u) {
+        const int nb = (m.F + c->saface_tb - 1) / c->saface_tb;
+        if (c->sa_geom_variant == 1 && c->k.iturb_visflux == 2)
+            k_sa_face<DIM, 1><<<nb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k, c->saface_tb);
+        else
+            k_sa_face<DIM, 0><<<nb, c->saface_tb, 0, c->stream>>>(m, c->d, c->k, c->saface_tb);
+    }
     if (parts & 4u) k_sa_cell<MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k, c->par_dt, c->fuse_l0);
 }
 

@@ -2,7 +2,8 @@
 
 The residual gather has two kernels (one lane per face; one thread per
 cell), the viscous face kernel a split form (inviscid and viscous parts on
-separate waves), and the face kernels take any face tile.  Every combination must give the
+separate waves), the SA face kernel reads or evaluates the TEST method's
+face geometry, and the face kernels take any face tile.  Every combination must give the
 same state, gradients, limiter and residual bit for bit, here after twelve
 pseudo-time steps from freestream (through the CFL ramp) on 2-D and 3-D grids
 and several option sets.  Variant 0 with the default tile is checked against
@@ -15,7 +16,7 @@ from tests.test_gpu_options import GRIDS, OPTS, make
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ["Q", "TURBQ", "DQDX", "LIMITER", "RES", "TIMESTEP", "DQ"]
+FIELDS = ["Q", "TURBQ", "DQDX", "LIMITER", "RES", "TURBRES", "TIMESTEP", "DQ"]
 
 
 @pytest.mark.parametrize("grid", ["quad", "tri", "mixed", "wing"])
@@ -24,10 +25,11 @@ def test_variants_and_tiles_bitwise(grid, opt):
     from oneflow_amd.solver import Solver
     base, _, g = make(grid, opt)
     runs = [base]
-    # (gather, face split) variants and face tiles
-    for variants, tiles in (((0, 1), (256, 256)), ((1, 0), (64, 128)), ((0, 1), (512, 64)), ((1, 0), (128, 512))):
+    # (gather, face split, SA face geometry) variants and face tiles
+    for variants, tiles in (((0, 1, 1), (256, 256)), ((1, 0, 1), (64, 128)), ((0, 1, 0), (512, 64)),
+                            ((1, 0, 1), (128, 512))):
         s = Solver(g, base.cfg)
-        for kernel, v in zip((2, 3), variants):
+        for kernel, v in zip((2, 3, 4), variants):
             s.set_kernel_variant(kernel, v)
         s.set_face_tile(*tiles)
         runs.append(s)

@@ -37,16 +37,17 @@ def test_learned_start_saves_steps_at_equal_forces():
     print(f"[warmstart] grid cells={g.n_cells}", flush=True)
 
     # 1. training fields: the nine cases converged together as zones, each to
-    #    a 1e-9 drop from its peak density residual over the first 1000 steps
+    #    a 1e-8 drop from its peak density residual over the first 1000 steps
     zt = ZoneSolver([g] * len(train), [cfg_of(c) for c in train])
     zt.init_freestream()
     peak = np.zeros(len(train))
     for it in range(1, 1001, 100):
         zt.iterate(it, 100, with_norms=False)
         peak = np.maximum(peak, zt.zone_norms()[:, 0])
-    _, steps, _, status = ws.solve_zones_to_tolerance(zt, list(peak), tol=1e-9, chunk=2000, max_steps=80_000,
+    _, steps, _, status = ws.solve_zones_to_tolerance(zt, list(peak), tol=TOL, chunk=2000, max_steps=250_000,
                                                       first_iter=1001)
-    print(f"[warmstart] training fields: steps {steps}, status {status}", flush=True)
+    drop = zt.zone_norms()[:, 0] / peak
+    print(f"[warmstart] training fields: steps {steps}, status {status}, drop {drop}", flush=True)
     assert status == ["converged"] * len(train)
     views = [ws.ZoneView(zt, z) for z in range(len(train))]
     X = np.concatenate([ws.cell_features(v, c) for v, c in zip(views, train)])
@@ -73,7 +74,7 @@ def test_learned_start_saves_steps_at_equal_forces():
         z.init_freestream()
         if learned:
             ws.apply_predictions_zones(z, test, net)
-        out = ws.solve_zones_to_tolerance(z, refs, tol=TOL, chunk=200, max_steps=80_000)
+        out = ws.solve_zones_to_tolerance(z, refs, tol=TOL, chunk=200, max_steps=250_000)
         z.close()
         return out
     _, st_plain, aero_plain, stat_plain = run(False)
