@@ -342,6 +342,16 @@ int ofx_mlp_get_params(ofx_mlp* net, float* host);
 int ofx_mlp_set_params(ofx_mlp* net, const float* host);
 /* y[n][5] = net(x[n][9]), n <= 2^20 per call. */
 int ofx_mlp_forward(ofx_mlp* net, const float* x, int n, float* y);
+/* ofx_mlp_forward without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops).  x_dev [n][9] and y_dev [n][5] are DEVICE pointers
+ * owned by the caller; the work is launched on `stream` (a hipStream_t; NULL:
+ * the network's own stream) with no host synchronisation, copy, memset or
+ * allocation.  n <= 8192: the work arrays are one fixed 10.25 MiB device
+ * array per GPU, held until the process ends and shared by every call on
+ * that GPU, so calls that overlap on one GPU race.  Larger n is rejected
+ * (OFX_EINVAL).  Results equal ofx_mlp_forward's bit for bit. */
+int ofx_mlp_forward_noalloc(ofx_mlp* net, const float* x_dev, int n, float* y_dev, void* stream);
 /* Mean-squared-error loss over the 5 outputs and its gradient with respect to
  * every parameter (same layout as the parameter vector); no update. */
 int ofx_mlp_gradients(ofx_mlp* net, const float* x, const float* t, int n, float* loss, float* grads);

@@ -16,6 +16,8 @@
 // loss.
 #include "ofx_common.h"
 
+#include <mutex>
+
 namespace ofx {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int ld
                     Cf[(size_t)blockIdx.z * M * ldc + o] = v;
                 } else if (EPI == EPI_BIAS_SILU) {
                     float z = v + bias[col];
-                    zout[o] = z;
+                    if (zout) zout[o] = z;  // pre-activations kept for a backward pass only
                     Cb[o] = f2bf(silu(z));
                 } else if (EPI == EPI_BIAS) {
                     Cf[o] = v + bias[col];
@@ -332,8 +334,8 @@ int mlp_reserve(ofx_mlp* n, int samples) {
 }
 
 template <int EPI>
-int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, int nsplit, float* Cf, bf16* Cb,
-         int ldc, const float* bias, const float* zin, float* zout) {
+int gemm_on(hipStream_t st, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, int nsplit, float* Cf,
+            bf16* Cb, int ldc, const float* bias, const float* zin, float* zout) {
     if (M % 64 || N % 64 || K % 16 || K % nsplit || (K / nsplit) % 16) {
         set_error("mlp: GEMM shape not tiled");
         return OFX_EINVAL;
@@ -341,11 +343,17 @@ int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* B, int ldb, int M, int 
     dim3 grid(N / 64, M / 64, nsplit);
     int kc = K / nsplit;
     if (kc % 64 == 0)
-        k_gemm<EPI, 64><<<grid, 256, 0, n->stream>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+        k_gemm<EPI, 64><<<grid, 256, 0, st>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
     else
-        k_gemm<EPI, 16><<<grid, 256, 0, n->stream>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+        k_gemm<EPI, 16><<<grid, 256, 0, st>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
     OFX_HIP(hipGetLastError());
     return 0;
+}
+
+template <int EPI>
+int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, int nsplit, float* Cf, bf16* Cb,
+         int ldc, const float* bias, const float* zin, float* zout) {
+    return gemm_on<EPI>(n->stream, A, lda, B, ldb, M, N, K, nsplit, Cf, Cb, ldc, bias, zin, zout);
 }
 
 int transpose(ofx_mlp* n, const bf16* in, int R, int C, bf16* out, int Cp) {
@@ -428,6 +436,46 @@ int load_batch(ofx_mlp* n, const float* x, const float* t, int nsamp, bool on_de
     return 0;
 }
 
+// ---- allocation-free forward pass (ofx_mlp_forward_noalloc)
+// Its work arrays live in one file-scope device array, `scratch`, of a fixed
+// size: packed inputs [MAXN][16] bf16, two activation buffers [MAXN][256]
+// bf16 (layers alternate between them) and the padded outputs [MAXN][64]
+// fp32 -- 10.25 MiB, held by every GPU the library runs on until the process
+// ends.  Every call on one GPU uses that GPU's copy; each element a call reads
+// it has written first, so nothing needs clearing, and calls that overlap on
+// one GPU race.
+constexpr int NOALLOC_MAXN = 8192;
+constexpr size_t NOALLOC_IN = sizeof(bf16) * NOALLOC_MAXN * NIN_P;
+constexpr size_t NOALLOC_ACT = sizeof(bf16) * NOALLOC_MAXN * NHID;
+constexpr size_t NOALLOC_OUT = sizeof(float) * NOALLOC_MAXN * NOUT_P;
+constexpr size_t NOALLOC_BYTES = NOALLOC_IN + 2 * NOALLOC_ACT + NOALLOC_OUT;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // device id -> address of its `scratch`
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (host code must never pass the array's name itself).
+int scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        OFX_HIP(hipGetDevice(&cur));
+        OFX_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            set_error(std::string("mlp: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return OFX_EHIP;
+        }
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -448,6 +496,8 @@ int ofx_mlp_create(ofx_mlp** out, int device, unsigned long long seed) {
         n->kin[l] = (l == 0) ? NIN_P : NHID;
         n->kout[l] = (l == NLAYER - 1) ? NOUT_P : NHID;
     }
+    unsigned char* sp = nullptr;  // the allocation-free forward's work array on this device
+    rc = scratch_address(device, &sp);
     // parameters first (mlp_reserve keeps the first NLAYER * 12 allocations)
     std::vector<float> hw, hb;
     uint64_t s = seed * 6364136223846793005ull + 1442695040888963407ull;
@@ -556,6 +606,37 @@ int ofx_mlp_forward(ofx_mlp* n, const float* x, int nsamp, float* y) {
     OFX_HIP(hipGetLastError());
     OFX_HIP(hipMemcpyAsync(y, n->tgt, sizeof(float) * cnt, hipMemcpyDeviceToHost, n->stream));
     OFX_HIP(hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+int ofx_mlp_forward_noalloc(ofx_mlp* n, const float* x_dev, int nsamp, float* y_dev, void* stream) {
+    if (!n || !x_dev || !y_dev || nsamp < 1) { set_error("mlp_forward_noalloc: bad arguments"); return OFX_EINVAL; }
+    if (nsamp > NOALLOC_MAXN) {
+        set_error("mlp_forward_noalloc: at most 8192 samples per call (fixed work array)");
+        return OFX_EINVAL;
+    }
+    OFX_HIP(hipSetDevice(n->device));
+    unsigned char* base = nullptr;
+    OFX_TRY(scratch_address(n->device, &base));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : n->stream;
+    bf16* in = reinterpret_cast<bf16*>(base);
+    bf16* act[2] = {reinterpret_cast<bf16*>(base + NOALLOC_IN), reinterpret_cast<bf16*>(base + NOALLOC_IN + NOALLOC_ACT)};
+    float* yp = reinterpret_cast<float*>(base + NOALLOC_IN + 2 * NOALLOC_ACT);
+    const int Mp = (nsamp + 63) / 64 * 64;
+    k_pack_inputs<<<(Mp * NIN_P + 255) / 256, 256, 0, st>>>(x_dev, nsamp, Mp, in);
+    OFX_HIP(hipGetLastError());
+    const bf16* h = in;
+    for (int l = 0; l < NLAYER - 1; ++l) {
+        OFX_TRY(gemm_on<EPI_BIAS_SILU>(st, h, n->kin[l], n->wb[l], n->kout[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
+                                       act[l & 1], n->kout[l], n->b[l], nullptr, nullptr));
+        h = act[l & 1];
+    }
+    const int L = NLAYER - 1;
+    OFX_TRY(gemm_on<EPI_BIAS>(st, h, n->kin[L], n->wb[L], n->kout[L], Mp, n->kout[L], n->kin[L], 1, yp, nullptr,
+                              n->kout[L], n->b[L], nullptr, nullptr));
+    size_t cnt = (size_t)nsamp * NOUT;
+    k_pack_outputs<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(yp, nsamp, y_dev);
+    OFX_HIP(hipGetLastError());
     return 0;
 }
 

@@ -29,6 +29,7 @@ def _lib():
         L.ofx_mlp_get_params.argtypes = [P, fp]
         L.ofx_mlp_set_params.argtypes = [P, fp]
         L.ofx_mlp_forward.argtypes = [P, fp, ctypes.c_int, fp]
+        L.ofx_mlp_forward_noalloc.argtypes = [P, P, ctypes.c_int, P, P]
         L.ofx_mlp_gradients.argtypes = [P, fp, fp, ctypes.c_int, fp, fp]
         L.ofx_mlp_train.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_ulonglong, fp]
