@@ -244,16 +244,16 @@ int ofx_wall_data(ofx_ctx* ctx, double* out);
 /* Kernel timing (no counterpart in the reference; measurement only):
  * launches one part of the residual `iters` times back to back on the
  * context's stream between two HIP events, after one warm-up launch, and
- * returns the average ms per launch.  which: 0 NS gradient + limiter min/max,
- * 1 limiter, 2 NS face fluxes (MUSCL + Roe/SLAU2 + viscous), 3 NS ordered
- * gather, 4 whole NS residual, 5 whole SA residual, 6 SA gradient (+ ghost
- * values), 7 SA face terms, 8 SA cell terms (source + ordered gather),
- * 9 NS + SA residual (one cell-residual evaluation), 10 one whole
- * pseudo-time step past the CFL ramp (as ofx_iterate; this one advances the
- * state).  Parts 0-9 are
- * idempotent on the current state, so the fields are left as one
- * ofx_residual_ns / ofx_residual_sa call leaves them (which 0 also leaves the
- * limiter stale until the next limiter launch). */
+ * returns the average ms per launch.  which: 0 NS gradient and limiter (one
+ * kernel), 1 the grid extrema of the Venkatakrishnan epsilon (a pass of its
+ * own only when no time step has reduced them on this state), 2 NS face
+ * fluxes (MUSCL + Roe/SLAU2 + viscous), 3 NS ordered gather, 4 whole NS
+ * residual, 5 whole SA residual, 6 SA gradient (+ ghost values), 7 SA face
+ * terms, 8 SA cell terms (source + ordered gather), 9 NS + SA residual (one
+ * cell-residual evaluation), 10 one whole pseudo-time step past the CFL ramp
+ * (as ofx_iterate; this one advances the state).  Parts 0-9 are idempotent
+ * on the current state, so the fields are left as one ofx_residual_ns /
+ * ofx_residual_sa call leaves them. */
 int ofx_time_kernel(ofx_ctx* ctx, int which, int iters, double* ms_out);
 
 /* Bandwidth reference (no counterpart in the reference): average duration
@@ -287,20 +287,22 @@ int ofx_zones_aero(ofx_ctx* ctx, double* out);
 int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
 
 /* Kernel variant (no counterpart in the reference; results do not depend on
- * it).  kernel 0, the NS gradient: variant 0 (default) one thread per cell,
- * variant 1 one wave per quantity (more waves in flight on small grids).
- * kernel 1, the limiter, and kernel 2, the residual gather: variant 0 one
- * lane per face of a cell, variant 1 (default) one thread per cell.
- * kernel 3, the viscous FP64 NS face kernel: variant 0 (default) one thread
- * per face, variant 1 two waves per 64 faces (inviscid / viscous part). */
+ * it).  Kernels 0 and 1, the NS gradient and the limiter, run as one kernel
+ * of a single form: only variant 0 is accepted.  kernel 2, the residual
+ * gather: variant 0 one lane per face of a cell, variant 1 (default) one
+ * thread per cell.  kernel 3, the viscous FP64 NS face kernel: variant 0
+ * (default) one thread per face, variant 1 two waves per 64 faces (inviscid
+ * / viscous part).  kernel 4, the SA face kernel with the TEST diffusion
+ * (iturb_visflux 2): variant 0 (default) reads the face's stored geometry
+ * record, variant 1 evaluates it from the cell and face centres. */
 int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
 int ofx_get_kernel_variant(ofx_ctx* ctx, int kernel, int* variant);
 
 /* Autotuner (no counterpart in the reference): times the NS face kernel for
- * each face tile, the NS gradient, limiter and gather for each variant and
- * the SA residual for each SA face tile on the current state over `iters`
- * launches, keeps the fastest, and reports the face tiles in chosen2[0..1]
- * (may be NULL).
+ * each face tile and split, the gather for each variant, and the SA residual
+ * for each SA face tile and geometry variant on the current state (each the
+ * median of five timings of `iters` launches), keeps the fastest, and
+ * reports the face tiles in chosen2[0..1] (may be NULL).
  * Leaves res / turbres as one evaluation of the current state. */
 int ofx_autotune(ofx_ctx* ctx, int iters, int* chosen2);
 
