@@ -304,8 +304,10 @@ def warmstart_main(args):
     phase = args.phase
     wsdir = Path(args.ws_dir)
     wsdir.mkdir(parents=True, exist_ok=True)
-    train = airfoil_sweep(args.n_train, seed=args.seed + 1, min_cells=20_000, max_cells=50_000)
-    test = airfoil_sweep(args.n_test, seed=args.seed, min_cells=20_000, max_cells=60_000)
+    train = airfoil_sweep(args.n_train, seed=args.seed + 1, min_cells=20_000, max_cells=args.train_max_cells)
+    test = airfoil_sweep(args.n_test, seed=args.seed, min_cells=20_000, max_cells=args.test_max_cells)
+    if args.test_ids:
+        test = [test[int(i)] for i in args.test_ids.split(",")]
     cfg_of = lambda c: c.config(ilim=-1, cfled=args.cfl)
     if phase == "combine":
         return _warmstart_combine(args, wsdir, train, test)
@@ -321,18 +323,23 @@ def warmstart_main(args):
     info = {}
     if phase in ("all", "train"):
         gtrain = grids_of(train)
-        # 1. training fields: the training cases as zones of one context, a fixed number of steps
+        # 1. training fields: the training cases as zones of one context, each
+        #    converged to the tolerance (a tol drop from its peak density
+        #    residual over the first 1000 steps); converged zones keep
+        #    iterating in place until the last one converges
         zt = ZoneSolver(gtrain, [cfg_of(c) for c in train])
         zt.init_freestream()
         t0 = time.perf_counter()
-        done = 0
-        while done < args.train_steps:
-            k = min(5000, args.train_steps - done)
-            zt.iterate(1 + done, k, with_norms=False)
-            zt.sync()
-            done += k
-            log(f"training fields: {done} steps, {time.perf_counter() - t0:.1f} s")
+        peak = np.zeros(len(train))
+        for it in range(1, 1001, 100):
+            zt.iterate(it, 100, with_norms=False)
+            peak = np.maximum(peak, zt.zone_norms()[:, 0])
+        _, tsteps, _, tstat = ws.solve_zones_to_tolerance(
+            zt, list(peak), tol=args.tol, chunk=2000, max_steps=args.train_steps, first_iter=1001,
+            progress=lambda i, n: log(f"  training case {i} done at step {n}"))
         t_gen = time.perf_counter() - t0
+        drop = zt.zone_norms()[:, 0] / peak
+        log(f"training fields: steps {min(tsteps)}-{max(tsteps)}, {t_gen:.1f} s, final drops {drop.min():.1e}-{drop.max():.1e}")
         views = [ws.ZoneView(zt, z) for z in range(zt.n_zones)]
         X = np.concatenate([ws.cell_features(v, c) for v, c in zip(views, train)])
         Y = np.concatenate([ws.cell_targets(v, c, v.cfg) for v, c in zip(views, train)])
@@ -351,7 +358,9 @@ def warmstart_main(args):
         t0 = time.perf_counter()
         net.forward(xt)
         t_fwd = time.perf_counter() - t0
-        info = {"train_cases": len(train), "train_samples": int(X.shape[0]), "train_steps": args.train_steps,
+        info = {"train_cases": len(train), "train_samples": int(X.shape[0]), "train_max_steps": args.train_steps,
+                "train_case_steps": [int(v) for v in tsteps], "train_case_status": tstat,
+                "train_final_drop": [float(v) for v in drop],
                 "training_fields_s": t_gen, "training_s": t_train,
                 "final_loss": float(losses[-50:].mean()), "first_loss": float(losses[:10].mean()),
                 "train_tflops": flops_step * args.mlp_steps / t_train / 1e12, "forward_1M_ms": 1e3 * t_fwd,
@@ -464,11 +473,18 @@ def _warmstart_result(args, train, test, info, seqs, z):
     credited to the network.  Forces are compared on the cases both runs
     converged; cases that hit max_steps or diverged are counted, not timed as
     converged."""
-    t_case = [t for s in seqs for t in s["seconds_per_case"]]
-    st_seq = [v for s in seqs for v in s["steps"]]
-    stat_seq = [v for s in seqs for v in s.get("status", ["converged"] * len(s["steps"]))]
-    aero_seq = [a for s in seqs for a in s["aero"]]
-    t_seq = float(sum(t_case))
+    seqs = [s for s in seqs if s]
+    if seqs:
+        t_case = [t for s in seqs for t in s["seconds_per_case"]]
+        st_seq = [v for s in seqs for v in s["steps"]]
+        stat_seq = [v for s in seqs for v in s.get("status", ["converged"] * len(s["steps"]))]
+        aero_seq = [a for s in seqs for a in s["aero"]]
+        t_seq = float(sum(t_case))
+    else:  # no case-after-case run: forces against the plain zones (the same solver)
+        st_seq = z["plain_zones_steps"]
+        stat_seq = z.get("status_plain_zones", ["converged"] * len(test))
+        aero_seq = z["aero_plain_zones"]
+        t_seq = None
     t_zp, t_zl, t_inf = z["plain_zones_s"], z["learned_zones_s"], z["inference_s"]
     stat_zp = z.get("status_plain_zones", ["converged"] * len(test))
     stat_zl = z.get("status_learned_zones", ["converged"] * len(test))
@@ -488,11 +504,13 @@ def _warmstart_result(args, train, test, info, seqs, z):
         "higher_is_better": True,
         "dtype": "f64 solver, bf16 network",
         "data": "synthetic",
-        "config": {"workload": f"{len(test)} unseen NACA 4-digit airfoils (20k-60k cells), Roe + SA, first order",
+        "config": {"workload": f"{len(test)} unseen NACA 4-digit airfoils (20k-{args.test_max_cells // 1000}k cells), "
+                               "Roe + SA, first order",
                    "train_cases": len(train), "tol": args.tol, "cfl": args.cfl, "mach_gate": args.mach_gate,
                    "compact": args.compact},
         "seconds": {"plain_sequential": t_seq, "plain_zones": t_zp, "learned_zones": t_zl, "inference": t_inf},
-        "speedup": {"zones": t_seq / t_zp, "warm_start": t_zp / (t_inf + t_zl), "total": t_seq / (t_inf + t_zl)},
+        "speedup": {"zones": t_seq / t_zp if t_seq else None, "warm_start": t_zp / (t_inf + t_zl),
+                    "total": t_seq / (t_inf + t_zl) if t_seq else None},
         "status": {"plain_sequential": count(stat_seq), "plain_zones": count(stat_zp),
                    "learned_zones": count(stat_zl)},
         "aero_rel_diff_learned_vs_plain": {
@@ -500,6 +518,10 @@ def _warmstart_result(args, train, test, info, seqs, z):
             "cases_within_1e-6": int(np.sum((dcl <= 1e-6) & (dcd <= 1e-6))), "cases_compared": len(ok),
             "cases": len(test)},
         "steps": {"plain_per_case": st_seq, "learned_zones_per_case": z["learned_zones_steps"]},
+        "per_case": [{"mach": c.mach, "aoa": c.aoa, "t": c.t, "cells": c.target_cells,
+                      "dCL": float(abs(a[0] - b[0]) / max(abs(b[0]), 1e-12)) if a and b else None,
+                      "dCD": float(abs(a[1] - b[1]) / max(abs(b[1]), 1e-12)) if a and b else None}
+                     for c, a, b in zip(test, z["aero_learned_zones"], aero_seq)],
         "training": info,
     }
     if "aero_plain_tight" in z:
@@ -513,7 +535,7 @@ def _warmstart_combine(args, wsdir, train, test):
     info = json.loads((wsdir / "train.json").read_text())
     seqs = sorted((json.loads(p.read_text()) for p in wsdir.glob("seq_*.json")), key=lambda s: s["range"][0])
     covered = [i for s in seqs for i in range(*s["range"])]
-    if covered != list(range(len(test))):
+    if seqs and covered != list(range(len(test))):
         raise SystemExit(f"seq phases cover {covered[:3]}..{covered[-3:]} of {len(test)} cases; run the missing ranges")
     z = json.loads((wsdir / "zones.json").read_text())
     print(json.dumps(_warmstart_result(args, train, test, info, seqs, z)))
@@ -534,15 +556,23 @@ def main():
                          "(default: cell for c1, spatial for c2)")
     ap.add_argument("--kernel-iters", type=int, default=200, help="back-to-back launches per kernel timing")
     ap.add_argument("--storage", type=int, default=None, choices=[0, 1],
-                    help="device storage order: 0 the grid's, 1 spatial (library default)")
+                    help="device storage order: 0 the grid's (library default), 1 spatial")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "batch", "warmstart"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "batch", "warmstart", "mlp"],
                     help="c2: the 3-D swept wing (default, the largest single-GPU configuration); "
                          "c1: the 2-D NACA0012 grid; batch: the 64-case sweep; "
-                         "warmstart: time to 1e-8, learned vs plain")
+                         "warmstart: time to 1e-8, learned vs plain; mlp: the warm-start network's "
+                         "forward pass and training steps alone")
+    ap.add_argument("--mlp-samples", type=int, default=1 << 20, help="mlp: samples per forward pass")
+    ap.add_argument("--mlp-batch", type=int, default=4096, help="mlp: training batch")
     ap.add_argument("--n-train", type=int, default=48)
     ap.add_argument("--n-test", type=int, default=16)
-    ap.add_argument("--train-steps", type=int, default=30000, help="solver steps for the training fields")
+    ap.add_argument("--train-steps", type=int, default=200_000,
+                    help="warmstart: step cap while the training fields converge to --tol")
+    ap.add_argument("--train-max-cells", type=int, default=50_000, help="warmstart: largest training grid")
+    ap.add_argument("--test-max-cells", type=int, default=200_000,
+                    help="warmstart: largest held-out grid (the batched sweep's range is 20k-200k)")
+    ap.add_argument("--test-ids", default="", help="warmstart: comma-separated subset of the held-out cases")
     ap.add_argument("--mlp-steps", type=int, default=3000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cfl", type=float, default=10.0, help="final CFL of the warm-start cases")
@@ -567,8 +597,50 @@ def main():
         return batch_main(args)
     if args.workload == "warmstart":
         return warmstart_main(args)
+    if args.workload == "mlp":
+        return mlp_main(args)
 
     return grid_main(args)
+
+
+# Multiply-adds of one sample through the padded network (16 -> 256 x 5 -> 64)
+MLP_MACS = 16 * 256 + 4 * 256 * 256 + 256 * 64
+
+
+def mlp_main(args):
+    """The warm-start network alone: `steps` forward passes of --mlp-samples
+    samples (host timed, PCIe copies of inputs and outputs included) and
+    --mlp-steps training steps of --mlp-batch samples (forward, backward and
+    Adam; one call).  Kernel durations come from rocprofv3 --kernel-trace;
+    FLOP counts use the padded layer shapes."""
+    from oneflow_amd.mlp import Mlp
+    rng = np.random.default_rng(args.seed)
+    n = args.mlp_samples
+    x = rng.normal(0.0, 1.0, (n, 9)).astype(np.float32)
+    net = Mlp(seed=1)
+    net.forward(x[:4096])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.forward(x)
+    fwd_s = (time.perf_counter() - t0) / args.steps
+    nt = 1 << 16
+    xt = x[:nt]
+    yt = np.tanh(xt[:, :5]).astype(np.float32)
+    net.train(xt, yt, batch=args.mlp_batch, steps=10)
+    t0 = time.perf_counter()
+    losses = net.train(xt, yt, batch=args.mlp_batch, steps=args.mlp_steps)
+    train_s = (time.perf_counter() - t0) / args.mlp_steps
+    net.close()
+    flop_fwd = 2.0 * MLP_MACS * n
+    flop_step = 3 * 2.0 * MLP_MACS * args.mlp_batch  # forward + two backward products per layer
+    print(json.dumps({
+        "metric": "mlp forward samples/s", "value": n / fwd_s, "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": 1e3 * fwd_s, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "mlp", "samples": n, "train_batch": args.mlp_batch},
+        "forward_tflops_host": flop_fwd / fwd_s / 1e12,
+        "train": {"ms_per_step": 1e3 * train_s, "tflops_host": flop_step / train_s / 1e12,
+                  "loss_first": float(losses[0]), "loss_last": float(losses[-1])}}), flush=True)
 
 
 def grid_main(args):
@@ -624,7 +696,7 @@ def grid_main(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "numbering": args.numbering, "face_order": args.face_order,
-                   "storage_order": "grid" if args.storage == 0 else "spatial",
+                   "storage_order": "spatial" if args.storage == 1 else "grid",
                    "cells": grid.n_cells, "faces": grid.n_faces, "nTCell": grid.n_tcell,
                    "mach": cfg.mach_ref, "aoa_deg": cfg.aoa_degree, "reynolds": cfg.reynolds},
         "roofline": r64["roofline"],
@@ -696,6 +768,10 @@ def residual_bench(s, grid, args, flux_bytes=8):
              "face": "k_ns_face",
              "gather": ("k_ns_gather", "k_ns_gather_c")[s.kernel_variant(2)],
              "sa_grad": "k_sa_grad", "sa_face": "k_sa_face", "sa_cell": "k_sa_cell"}
+    # keys of the counter summary: the instantiation this solver runs
+    fp32 = "_fp32" if flux_bytes == 4 else ""
+    geo = "_geo" if s.kernel_variant(4) == 1 and s.cfg.iturb_visflux == 2 else ""
+    tkey = {"face": "k_ns_face" + fp32, "gather": names["gather"] + fp32, "sa_face": "k_sa_face" + geo}
     prof = {key: s.time_kernel(key, iters=args.kernel_iters) for key in names}
     s.residual_ns()  # leave a consistent residual behind
     s.residual_sa()
@@ -705,7 +781,7 @@ def residual_bench(s, grid, args, flux_bytes=8):
         ms = prof[key]
         kernels[kname] = {"avg_us": 1e3 * ms, "bytes_per_launch": kb[key],
                           "achieved_gbs": kb[key] / (ms * 1e-3) / 1e9 if ms > 0 else None,
-                          "traffic_bytes": read_traffic(kname, grid.dim)}
+                          "traffic_bytes": read_traffic(tkey.get(key, kname), grid.dim)}
     dom = max(names, key=lambda k: prof[k])
     ach = kb[dom] / (prof[dom] * 1e-3) / 1e9
     dominant = {"kernel": names[dom], "avg_us": 1e3 * prof[dom], "bytes_per_launch": kb[dom], "achieved": ach,

@@ -132,11 +132,13 @@ int ofx_mesh_upload(ofx_ctx* ctx, int dim, int nNodes,
                     const int* lc, const int* rc, const int* bcType);
 
 /* Storage order of the device arrays (no counterpart in the reference; a
- * layout choice, results do not depend on it): 0 the grid's own cell and
- * face order, 1 (default) spatial -- the cells of each connected part of the
+ * layout choice, results do not depend on it): 0 (default) the grid's own
+ * cell and face order, 1 spatial -- the cells of each connected part of the
  * grid in Morton order of their centroids and the interior faces sorted by
  * their cells' positions, so that neighbours share cache lines whatever the
- * numbering (e.g. a colouring chosen for the LU-SGS sweeps).  Call before
+ * numbering.  On a grid numbered by colour (for the LU-SGS sweeps) the
+ * spatial order scatters each sweep level over the whole array, so the
+ * grid's own order is the default.  Call before
  * ofx_mesh_upload.  Every entry point keeps the grid's numbering: fields are
  * read and written in grid order, each cell sums its faces in ascending face
  * id and the LU-SGS sweeps split neighbours by cell id. */

@@ -159,12 +159,12 @@ struct ofx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool has_mesh = false, has_cfg = false;
-    // Storage order (chosen before the mesh upload): 0 the grid's own order,
-    // 1 spatial -- cells of each connected part in Morton order of their
+    // Storage order (chosen before the mesh upload): 0 (default) the grid's
+    // own order, 1 spatial -- cells of each connected part in Morton order of their
     // centroids, interior faces by their lower cell, boundary faces and ghosts
     // unchanged.  cperm / fperm map a grid id to its storage position (empty:
     // identity); host copies of fields are permuted on the way in and out.
-    int storage_order = 1;
+    int storage_order = 0;
     std::vector<int> cperm, fperm;
     ofx_config cfg{};
     ofx::Consts k{};

@@ -6,11 +6,12 @@
 //
 // Numerics: weights are kept in FP32 (Adam master copy) and used in BF16;
 // activations between layers are BF16; every product is accumulated in FP32
-// by v_mfma_f32_16x16x16_bf16.  Pre-activations are kept in FP32 for the
+// by v_mfma_f32_16x16x32_bf16.  Pre-activations are kept in FP32 for the
 // backward pass.
 //
 // Layout: activations row-major [samples][features]; weights [in][out]
-// row-major.  Feature counts are padded: input 9 -> 16, output 5 -> 64 (the
+// row-major, with a transposed bf16 copy [out][in] for the forward products
+// (the GEMM reads both operands along k).  Feature counts are padded: input 9 -> 16, output 5 -> 64 (the
 // padding weights are zero and stay zero: their gradients are exactly zero);
 // the sample count is padded to a multiple of 64 with zero rows that carry no
 // loss.
@@ -20,8 +21,9 @@
 
 namespace ofx {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef uint16_t bf16;
 
 constexpr int NIN = 9, NIN_P = 16, NHID = 256, NLAYER = 6, NOUT = 5, NOUT_P = 64;
@@ -42,42 +44,45 @@ __device__ __forceinline__ float dsilu(float z) {
 
 enum { EPI_PARTIAL = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3 };
 
-// C[M x N] = A[M x K] B[K x N]; A, B bf16 row-major (lda, ldb); M and N
-// multiples of 64; blockIdx.z selects the K range [z*kchunk, (z+1)*kchunk)
-// (kchunk a multiple of KC).  256 threads per 64 x 64 tile; wave w owns rows
-// 32*(w>>1).. and columns 32*(w&1).. as 2 x 2 MFMA 16x16x16 tiles.  MFMA
-// operands: lane l supplies A[l%16][4*(l/16)+0..3] and B[4*(l/16)+0..3][l%16]
-// and receives D[4*(l/16)+r][l%16], r = 0..3.
-// K advances KC at a time through LDS (one barrier pair per KC/16 MFMA
-// steps); the next chunk's global loads are issued into registers before the
-// current chunk's MFMAs, so their latency overlaps the arithmetic.  The MFMA
-// chain runs over k in ascending order of 16-wide steps for any KC.
+// C[M x N] = A[M x K] Bt[N x K]^T: both operands bf16 with k contiguous
+// (row strides lda, ldb), so every global and LDS access moves 8 elements
+// (16 bytes) at a time.  M and N multiples of 64; blockIdx.z selects the K
+// range [z*kchunk, (z+1)*kchunk), a multiple of 16 (an 8-element vector at or
+// past its end is zero-filled, so K = 16 runs through the 32-deep MFMA with
+// zero products added).  256 threads per 64 x 64 tile; wave w owns rows
+// 32*(w>>1).. and columns 32*(w&1).. as 2 x 2 tiles of gfx950's
+// v_mfma_f32_16x16x32_bf16.  MFMA operands: lane l supplies A[l%16][8*(l/16)
+// + 0..7] and Bt[l%16][8*(l/16) + 0..7] and receives D[4*(l/16)+r][l%16],
+// r = 0..3.  K advances KC at a time through LDS (one barrier pair per KC/32
+// MFMA steps); the next chunk's global loads are issued into registers before
+// the current chunk's MFMAs, so their latency overlaps the arithmetic.  The
+// MFMA chain runs over k in ascending order of 32-wide steps.
 template <int EPI, int KC>
-__global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+__global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ Bt, int ldb,
                                               int kchunk, int M, float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc,
                                               const float* __restrict__ bias, const float* __restrict__ zin,
                                               float* __restrict__ zout) {
-    constexpr int PAD = 8;            // bf16 elements of row padding (LDS bank spread)
-    constexpr int NV = KC / 16;       // 4-element vectors per thread per operand tile
+    constexpr int PAD = 8;              // bf16 elements of row padding (LDS bank spread)
+    constexpr int VR = KC / 8;          // 8-element vectors per tile row
+    constexpr int NV = 64 * VR / 256;   // vectors per thread per operand tile
+    static_assert(KC % 32 == 0 && NV >= 1, "KC must be a multiple of 32");
     __shared__ __align__(16) bf16 sA[64][KC + PAD];
-    __shared__ __align__(16) bf16 sB[64][KC + PAD];  // transposed: [n][k]
+    __shared__ __align__(16) bf16 sB[64][KC + PAD];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
     const int kb = blockIdx.z * kchunk, ke = kb + kchunk;
     v4f acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    // A tile 64 x KC: vector e = u*256 + t is row e / (KC/4), k 4*(e % (KC/4))
-    // B tile KC x 64: vector e is k e / 16, n 4*(e % 16)
-    v4s ra[NV], rb[NV];
+    // vector e = u*256 + t of a tile: row e / VR, k offset 8*(e % VR)
+    v4u ra[NV], rb[NV];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < NV; ++u) {
-            int e = u * 256 + t;
-            int r = e / (KC / 4), kq = e % (KC / 4);
-            ra[u] = *(const v4s*)&A[(size_t)(m0 + r) * lda + k0 + 4 * kq];
-            int kk = e >> 4, nq = e & 15;
-            rb[u] = *(const v4s*)&B[(size_t)(k0 + kk) * ldb + n0 + 4 * nq];
+            const int e = u * 256 + t, r = e / VR, kv = 8 * (e % VR);
+            const bool ok = k0 + kv < ke;
+            ra[u] = ok ? *(const v4u*)&A[(size_t)(m0 + r) * lda + k0 + kv] : v4u{0u, 0u, 0u, 0u};
+            rb[u] = ok ? *(const v4u*)&Bt[(size_t)(n0 + r) * ldb + k0 + kv] : v4u{0u, 0u, 0u, 0u};
         }
     };
     gload(kb);
@@ -85,25 +90,20 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int ld
         __syncthreads();  // the previous chunk's MFMAs have read LDS
 #pragma unroll
         for (int u = 0; u < NV; ++u) {
-            int e = u * 256 + t;
-            int r = e / (KC / 4), kq = e % (KC / 4);
-            *(v4s*)&sA[r][4 * kq] = ra[u];
-            int kk = e >> 4, nq = e & 15;
-            sB[4 * nq + 0][kk] = (bf16)rb[u].x;
-            sB[4 * nq + 1][kk] = (bf16)rb[u].y;
-            sB[4 * nq + 2][kk] = (bf16)rb[u].z;
-            sB[4 * nq + 3][kk] = (bf16)rb[u].w;
+            const int e = u * 256 + t, r = e / VR, kv = 8 * (e % VR);
+            *(v4u*)&sA[r][kv] = ra[u];
+            *(v4u*)&sB[r][kv] = rb[u];
         }
         __syncthreads();
         if (k0 + KC < ke) gload(k0 + KC);
 #pragma unroll
-        for (int ks = 0; ks < KC; ks += 16) {
-            v4s a[2], b[2];
-            for (int i = 0; i < 2; ++i) a[i] = *(const v4s*)&sA[32 * (w >> 1) + 16 * i + (lane & 15)][ks + 4 * (lane >> 4)];
-            for (int j = 0; j < 2; ++j) b[j] = *(const v4s*)&sB[32 * (w & 1) + 16 * j + (lane & 15)][ks + 4 * (lane >> 4)];
+        for (int ks = 0; ks < KC; ks += 32) {
+            v8bf a[2], b[2];
+            for (int i = 0; i < 2; ++i) a[i] = *(const v8bf*)&sA[32 * (w >> 1) + 16 * i + (lane & 15)][ks + 8 * (lane >> 4)];
+            for (int j = 0; j < 2; ++j) b[j] = *(const v8bf*)&sB[32 * (w & 1) + 16 * j + (lane & 15)][ks + 8 * (lane >> 4)];
             for (int i = 0; i < 2; ++i)
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
     for (int i = 0; i < 2; ++i)
@@ -155,28 +155,35 @@ __global__ void k_splitk_sum(const float* part, int nsplit, size_t n, float* out
 }
 
 // Column sums of an M x N fp32 matrix (bias gradients), two stages in a fixed
-// order: block (x, y) sums columns 64x..64x+63 over rows 256y..256y+255
-// (4 row lanes of 64 rows, then the 4 lanes in order) into part[y][N]; then
-// one thread per column adds the row-chunk partials in order.
-constexpr int COLSUM_ROWS = 256;
+// order: block (x, y) sums columns 64x..64x+63 over rows COLSUM_ROWS*y ..
+// (4 row lanes of COLSUM_ROWS/4 rows, then the 4 lanes in order) into
+// part[y][N]; then one block per 64 columns adds the row-chunk partials: row
+// lane r takes chunks r, r+4, r+8, ... in order, then the 4 lanes in order.
+constexpr int COLSUM_ROWS = 64;
 __global__ void __launch_bounds__(256) k_colsum_part(const float* g, int M, int N, float* part) {
     __shared__ float sm[4][64];
     int c = blockIdx.x * 64 + (threadIdx.x & 63), lane4 = threadIdx.x >> 6;
     int r0 = blockIdx.y * COLSUM_ROWS + lane4 * (COLSUM_ROWS / 4);
     float s = 0.f;
     if (c < N)
-        for (int r = r0; r < r0 + COLSUM_ROWS / 4 && r < M; ++r) s += g[(size_t)r * N + c];
+#pragma unroll
+        for (int r = r0; r < r0 + COLSUM_ROWS / 4; ++r)
+            if (r < M) s += g[(size_t)r * N + c];
     sm[lane4][threadIdx.x & 63] = s;
     __syncthreads();
     if (threadIdx.x < 64 && c < N)
         part[(size_t)blockIdx.y * N + c] = ((sm[0][threadIdx.x] + sm[1][threadIdx.x]) + sm[2][threadIdx.x]) + sm[3][threadIdx.x];
 }
-__global__ void k_colsum_final(const float* part, int chunks, int N, float* out) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
+__global__ void __launch_bounds__(256) k_colsum_final(const float* part, int chunks, int N, float* out) {
+    __shared__ float sm[4][64];
+    int c = blockIdx.x * 64 + (threadIdx.x & 63), lane4 = threadIdx.x >> 6;
     float s = 0.f;
-    for (int k = 0; k < chunks; ++k) s += part[(size_t)k * N + c];
-    out[c] = s;
+    if (c < N)
+        for (int k = lane4; k < chunks; k += 4) s += part[(size_t)k * N + c];
+    sm[lane4][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < N)
+        out[c] = ((sm[0][threadIdx.x] + sm[1][threadIdx.x]) + sm[2][threadIdx.x]) + sm[3][threadIdx.x];
 }
 
 // Inputs fp32 [n][NIN] -> bf16 [Mp][NIN_P] (zero padded).
@@ -285,7 +292,7 @@ struct ofx_mlp {
     float* y = nullptr;
     float* tgt = nullptr;
     float* xin = nullptr;
-    bf16 *dyb = nullptr, *ht = nullptr;
+    bf16 *dyb = nullptr, *ht = nullptr, *gt = nullptr;  // ht, gt: transposed factors of a weight gradient
     float *dyf = nullptr, *gz = nullptr;  // fp32 gradient of the current layer's pre-activation
     bf16 *gzb = nullptr, *gzb2 = nullptr;
     float *part = nullptr, *lpart = nullptr, *loss = nullptr, *cpart = nullptr;
@@ -319,6 +326,7 @@ int mlp_reserve(ofx_mlp* n, int samples) {
     OFX_TRY(A((void**)&n->dyb, sizeof(bf16) * (size_t)Mp * NOUT_P));
     OFX_TRY(A((void**)&n->dyf, sizeof(float) * (size_t)Mp * NOUT_P));
     OFX_TRY(A((void**)&n->ht, sizeof(bf16) * (size_t)Mp * NHID));
+    OFX_TRY(A((void**)&n->gt, sizeof(bf16) * (size_t)Mp * NHID));
     OFX_TRY(A((void**)&n->gz, sizeof(float) * (size_t)Mp * NHID));
     OFX_TRY(A((void**)&n->gzb, sizeof(bf16) * (size_t)Mp * NHID));
     OFX_TRY(A((void**)&n->gzb2, sizeof(bf16) * (size_t)Mp * NHID));
@@ -333,8 +341,9 @@ int mlp_reserve(ofx_mlp* n, int samples) {
     return 0;
 }
 
+// C = A Bt^T (see k_gemm): A [M][K], Bt [N][K].
 template <int EPI>
-int gemm_on(hipStream_t st, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, int nsplit, float* Cf,
+int gemm_on(hipStream_t st, const bf16* A, int lda, const bf16* Bt, int ldb, int M, int N, int K, int nsplit, float* Cf,
             bf16* Cb, int ldc, const float* bias, const float* zin, float* zout) {
     if (M % 64 || N % 64 || K % 16 || K % nsplit || (K / nsplit) % 16) {
         set_error("mlp: GEMM shape not tiled");
@@ -343,17 +352,17 @@ int gemm_on(hipStream_t st, const bf16* A, int lda, const bf16* B, int ldb, int 
     dim3 grid(N / 64, M / 64, nsplit);
     int kc = K / nsplit;
     if (kc % 64 == 0)
-        k_gemm<EPI, 64><<<grid, 256, 0, st>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+        k_gemm<EPI, 64><<<grid, 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
     else
-        k_gemm<EPI, 16><<<grid, 256, 0, st>>>(A, lda, B, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+        k_gemm<EPI, 32><<<grid, 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
     OFX_HIP(hipGetLastError());
     return 0;
 }
 
 template <int EPI>
-int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, int nsplit, float* Cf, bf16* Cb,
+int gemm(ofx_mlp* n, const bf16* A, int lda, const bf16* Bt, int ldb, int M, int N, int K, int nsplit, float* Cf, bf16* Cb,
          int ldc, const float* bias, const float* zin, float* zout) {
-    return gemm_on<EPI>(n->stream, A, lda, B, ldb, M, N, K, nsplit, Cf, Cb, ldc, bias, zin, zout);
+    return gemm_on<EPI>(n->stream, A, lda, Bt, ldb, M, N, K, nsplit, Cf, Cb, ldc, bias, zin, zout);
 }
 
 int transpose(ofx_mlp* n, const bf16* in, int R, int C, bf16* out, int Cp) {
@@ -363,15 +372,17 @@ int transpose(ofx_mlp* n, const bf16* in, int R, int C, bf16* out, int Cp) {
     return 0;
 }
 
-// Weight gradient dW[kin x kout] = H^T[kin x Mp] G[Mp x kout], split over samples.
+// Weight gradient dW[kin x kout] = H^T[kin x Mp] G[Mp x kout], split over
+// samples: both factors transposed so that k (the sample) is contiguous.
 int weight_grad(ofx_mlp* n, int l, const bf16* H, const bf16* G, int Mp) {
     int ki = n->kin[l], ko = n->kout[l];
     int rows = (ki + 63) / 64 * 64;  // the input layer's 16 rows padded to one tile
     OFX_TRY(transpose(n, H, Mp, ki, n->ht, rows));
+    OFX_TRY(transpose(n, G, Mp, ko, n->gt, ko));
     int nsplit = 1;
     while (Mp / nsplit > SPLITK_CHUNK && Mp % (nsplit * 2) == 0 && (Mp / (nsplit * 2)) % 16 == 0) nsplit *= 2;
     if (nsplit > n->max_split || Mp > n->cap) { set_error("mlp: split-K buffer too small"); return OFX_EINVAL; }
-    OFX_TRY(gemm<EPI_PARTIAL>(n, n->ht, Mp, G, ko, rows, ko, Mp, nsplit, n->part, nullptr, ko, nullptr, nullptr, nullptr));
+    OFX_TRY(gemm<EPI_PARTIAL>(n, n->ht, Mp, n->gt, Mp, rows, ko, Mp, nsplit, n->part, nullptr, ko, nullptr, nullptr, nullptr));
     size_t cnt = (size_t)rows * ko;
     // rows beyond kin are zero inputs (their sums are zero); keep the first kin rows
     k_splitk_sum<<<(unsigned)((cnt + 255) / 256), 256, 0, n->stream>>>(n->part, nsplit, cnt, n->part + (size_t)nsplit * cnt);
@@ -385,18 +396,26 @@ int colsum(ofx_mlp* n, const float* g, int M, int N, float* out) {
     int chunks = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
     if ((size_t)chunks * N > (size_t)n->cap / COLSUM_ROWS * NHID + NHID) { set_error("mlp: column-sum buffer too small"); return OFX_EINVAL; }
     k_colsum_part<<<dim3((N + 63) / 64, chunks), 256, 0, n->stream>>>(g, M, N, n->cpart);
-    k_colsum_final<<<(N + 63) / 64, 64, 0, n->stream>>>(n->cpart, chunks, N, out);
+    k_colsum_final<<<(N + 63) / 64, 256, 0, n->stream>>>(n->cpart, chunks, N, out);
     OFX_HIP(hipGetLastError());
     return 0;
 }
 
-int forward(ofx_mlp* n, int Mp) {
+// keep_z: store the pre-activations (a backward pass follows).
+int forward(ofx_mlp* n, int Mp, bool keep_z) {
     for (int l = 0; l < NLAYER - 1; ++l)
-        OFX_TRY(gemm<EPI_BIAS_SILU>(n, n->h[l], n->kin[l], n->wb[l], n->kout[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
-                                    n->h[l + 1], n->kout[l], n->b[l], nullptr, n->z[l]));
+        OFX_TRY(gemm<EPI_BIAS_SILU>(n, n->h[l], n->kin[l], n->wt[l], n->kin[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
+                                    n->h[l + 1], n->kout[l], n->b[l], nullptr, keep_z ? n->z[l] : nullptr));
     int L = NLAYER - 1;
-    OFX_TRY(gemm<EPI_BIAS>(n, n->h[L], n->kin[L], n->wb[L], n->kout[L], Mp, n->kout[L], n->kin[L], 1, n->y, nullptr,
+    OFX_TRY(gemm<EPI_BIAS>(n, n->h[L], n->kin[L], n->wt[L], n->kin[L], Mp, n->kout[L], n->kin[L], 1, n->y, nullptr,
                            n->kout[L], n->b[L], nullptr, nullptr));
+    return 0;
+}
+
+// The transposed bf16 weights [kout][kin] that the forward products read,
+// refreshed whenever the bf16 weights change.
+int refresh_wt(ofx_mlp* n) {
+    for (int l = 0; l < NLAYER; ++l) OFX_TRY(transpose(n, n->wb[l], n->kin[l], n->kout[l], n->wt[l], n->kout[l]));
     return 0;
 }
 
@@ -412,9 +431,9 @@ int backward(ofx_mlp* n, int nsamp, int Mp) {
     const bf16* g = n->dyb;
     bf16* gnext = n->gzb;
     for (int l = L; l >= 1; --l) {
-        // gradient of layer l-1's pre-activation: (G W_l^T) * silu'(z_{l-1})
-        OFX_TRY(transpose(n, n->wb[l], n->kin[l], n->kout[l], n->wt[l], n->kout[l]));
-        OFX_TRY(gemm<EPI_DSILU>(n, g, n->kout[l], n->wt[l], n->kin[l], Mp, n->kin[l], n->kout[l], 1, n->gz, gnext,
+        // gradient of layer l-1's pre-activation: (G W_l^T) * silu'(z_{l-1});
+        // the second factor read along k is W_l itself
+        OFX_TRY(gemm<EPI_DSILU>(n, g, n->kout[l], n->wb[l], n->kout[l], Mp, n->kin[l], n->kout[l], 1, n->gz, gnext,
                                 n->kin[l], nullptr, n->z[l - 1], nullptr));
         OFX_TRY(weight_grad(n, l - 1, n->h[l - 1], gnext, Mp));
         OFX_TRY(colsum(n, n->gz, Mp, NHID, n->gb[l - 1]));
@@ -535,6 +554,7 @@ int ofx_mlp_create(ofx_mlp** out, int device, unsigned long long seed) {
         }
         if (!rc && hipStreamSynchronize(n->stream) != hipSuccess) rc = OFX_EHIP;
     }
+    if (!rc) rc = refresh_wt(n);
     if (rc) {
         ofx_mlp_destroy(n);
         return rc;
@@ -579,6 +599,7 @@ static int mlp_copy(ofx_mlp* n, float* host, bool grads, bool to_device) {
         }
         o += nw + n->kout[l];
     }
+    if (to_device && !grads) OFX_TRY(refresh_wt(n));
     OFX_HIP(hipStreamSynchronize(n->stream));
     return 0;
 }
@@ -599,7 +620,7 @@ int ofx_mlp_forward(ofx_mlp* n, const float* x, int nsamp, float* y) {
     OFX_HIP(hipSetDevice(n->device));
     OFX_TRY(load_batch(n, x, nullptr, nsamp, false));
     int Mp = (nsamp + 63) / 64 * 64;
-    OFX_TRY(forward(n, Mp));
+    OFX_TRY(forward(n, Mp, false));
     // tgt (Mp x NOUT floats) is free in a forward-only call: pack there, copy once
     size_t cnt = (size_t)nsamp * NOUT;
     k_pack_outputs<<<(unsigned)((cnt + 255) / 256), 256, 0, n->stream>>>(n->y, nsamp, n->tgt);
@@ -627,12 +648,12 @@ int ofx_mlp_forward_noalloc(ofx_mlp* n, const float* x_dev, int nsamp, float* y_
     OFX_HIP(hipGetLastError());
     const bf16* h = in;
     for (int l = 0; l < NLAYER - 1; ++l) {
-        OFX_TRY(gemm_on<EPI_BIAS_SILU>(st, h, n->kin[l], n->wb[l], n->kout[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
+        OFX_TRY(gemm_on<EPI_BIAS_SILU>(st, h, n->kin[l], n->wt[l], n->kin[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
                                        act[l & 1], n->kout[l], n->b[l], nullptr, nullptr));
         h = act[l & 1];
     }
     const int L = NLAYER - 1;
-    OFX_TRY(gemm_on<EPI_BIAS>(st, h, n->kin[L], n->wb[L], n->kout[L], Mp, n->kout[L], n->kin[L], 1, yp, nullptr,
+    OFX_TRY(gemm_on<EPI_BIAS>(st, h, n->kin[L], n->wt[L], n->kin[L], Mp, n->kout[L], n->kin[L], 1, yp, nullptr,
                               n->kout[L], n->b[L], nullptr, nullptr));
     size_t cnt = (size_t)nsamp * NOUT;
     k_pack_outputs<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(yp, nsamp, y_dev);
@@ -645,7 +666,7 @@ int ofx_mlp_gradients(ofx_mlp* n, const float* x, const float* t, int nsamp, flo
     OFX_HIP(hipSetDevice(n->device));
     OFX_TRY(load_batch(n, x, t, nsamp, false));
     int Mp = (nsamp + 63) / 64 * 64;
-    OFX_TRY(forward(n, Mp));
+    OFX_TRY(forward(n, Mp, true));
     OFX_TRY(backward(n, nsamp, Mp));
     if (loss) OFX_HIP(hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
     OFX_HIP(hipStreamSynchronize(n->stream));
@@ -655,7 +676,7 @@ int ofx_mlp_gradients(ofx_mlp* n, const float* x, const float* t, int nsamp, flo
 
 static int train_step(ofx_mlp* n, int nsamp, float lr) {
     int Mp = (nsamp + 63) / 64 * 64;
-    OFX_TRY(forward(n, Mp));
+    OFX_TRY(forward(n, Mp, true));
     OFX_TRY(backward(n, nsamp, Mp));
     ++n->step;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -669,7 +690,7 @@ static int train_step(ofx_mlp* n, int nsamp, float lr) {
                                                                (size_t)n->kout[l], lr, b1, b2, eps, c1, c2);
     }
     OFX_HIP(hipGetLastError());
-    return 0;
+    return refresh_wt(n);
 }
 
 int ofx_mlp_train(ofx_mlp* n, const float* x, const float* t, int nsamp, int batch, int steps, float lr,

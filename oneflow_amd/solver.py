@@ -109,7 +109,7 @@ class Solver:
         h = ctypes.c_void_p()
         self._check(L.ofx_ctx_create(ctypes.byref(h), device))
         self.h = h
-        if storage_order is not None:  # 0 the grid's order, 1 spatial (the library's default)
+        if storage_order is not None:  # 0 the grid's order (the library's default), 1 spatial
             self._check(L.ofx_set_storage_order(h, storage_order))
         g = grid
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z))
