@@ -53,7 +53,7 @@ TRAFFIC_SUMMARY = {2: ROOT / "profiles" / "naca0012_c1_hbm_traffic.txt",
 VIS_RECORD_BYTES = {0: 32, 1: 0, 2: 160}  # per-face viscous geometry record read by face-gradient method
 
 
-def kernel_bytes(C, B, F, dim, flux_bytes=8, sa_vis_method=2, ns_vis_method=0):
+def kernel_bytes(C, B, F, dim, flux_bytes=8, sa_vis_method=2, ns_vis_method=0, sa_geo=0):
     """Algorithmic bytes per launch of the NS residual kernels: every array the
     kernel must read or write counted once (FP64 values, int32 indices; the
     face fluxes are FP32 in the FP32-flux variant, flux_bytes = 4).  In 2-D the
@@ -63,7 +63,8 @@ def kernel_bytes(C, B, F, dim, flux_bytes=8, sa_vis_method=2, ns_vis_method=0):
     (4 doubles per face, 3-D + 1), face centre relative to both cell centres
     (4 doubles, 3-D + 2), and the viscous face-gradient record of the NS and
     SA methods (ns_vis_method, sa_vis_method: STD 4 doubles, AVER none,
-    TEST 20 doubles)."""
+    TEST 20 doubles; with sa_geo = 1 the SA kernel evaluates the TEST
+    geometry from the face centre and both cell centres instead)."""
     T = C + B
     nnz = 2 * F - B                  # entries of the cell -> face lists
     nq = 4 if dim == 2 else 5        # primitive components carried
@@ -88,7 +89,8 @@ def kernel_bytes(C, B, F, dim, flux_bytes=8, sa_vis_method=2, ns_vis_method=0):
         "sa_grad": 8 * T + idx + 4 * nnz + gg + 8 * C + 4 * B + 8 * B + 8 * D * C,
         # lc/rc, normal/area, viscous record, rho + velocity, nu_tilde, mu_l,
         # volume, grad nu_tilde, wall-face types -> 5 face terms
-        "sa_face": 8 * F + 8 * (D + 1) * F + VIS_RECORD_BYTES[sa_vis_method] * F + 8 * (D + 1) * T
+        "sa_face": 8 * F + 8 * (D + 1) * F + (8 * D * F + 8 * D * T if sa_geo and sa_vis_method == 2
+                                               else VIS_RECORD_BYTES[sa_vis_method] * F) + 8 * (D + 1) * T
                    + 8 * 3 * T + 8 * D * C + 4 * B + 40 * F,
         # face lists, 3 face terms, velocity gradient, rho, nu_tilde, mu_l, wall
         # distance, grad nu_tilde, volume, time step -> diagonal and residual
@@ -273,6 +275,17 @@ def batch_main(args):
     print(json.dumps(out))
 
 
+def _every(n, fn):
+    """A chunk callback that calls fn(step, running) once per n steps."""
+    last = [0]
+
+    def f(it, nl):
+        if it >= last[0] + n:
+            last[0] = it
+            fn(it, nl)
+    return f
+
+
 def _sweep_grid_o1(case):
     return case.grid("multicolor")
 
@@ -336,7 +349,8 @@ def warmstart_main(args):
             peak = np.maximum(peak, zt.zone_norms()[:, 0])
         _, tsteps, _, tstat = ws.solve_zones_to_tolerance(
             zt, list(peak), tol=args.tol, chunk=2000, max_steps=args.train_steps, first_iter=1001,
-            progress=lambda i, n: log(f"  training case {i} done at step {n}"))
+            progress=lambda i, n: log(f"  training case {i} done at step {n}"),
+            on_chunk=_every(10000, lambda it, nl: log(f"  training fields: step {it}, {nl} running")))
         t_gen = time.perf_counter() - t0
         drop = zt.zone_norms()[:, 0] / peak
         log(f"training fields: steps {min(tsteps)}-{max(tsteps)}, {t_gen:.1f} s, final drops {drop.min():.1e}-{drop.max():.1e}")
@@ -412,11 +426,18 @@ def warmstart_main(args):
             return
 
     if phase == "zones":
-        net = Mlp(seed=args.seed)
-        net.set_params(np.load(wsdir / "net.npy"))
+        net = None
+        if args.zones_run != "plain":
+            net = Mlp(seed=args.seed)
+            net.set_params(np.load(wsdir / "net.npy"))
     zres = _warmstart_zones(args, test, gtest, cfg_of, refs, peaks, ws, ZoneSolver, log, net)
     if phase == "zones":
-        (wsdir / "zones.json").write_text(json.dumps(zres))
+        ids = [int(i) for i in args.test_ids.split(",")] if args.test_ids else list(range(len(test)))
+        zres["test_ids"] = ids
+        name = "zones" if args.zones_run == "both" else f"zones_{args.zones_run}"
+        if args.test_ids:
+            name += f"_{ids[0]}-{ids[-1]}"
+        (wsdir / f"{name}.json").write_text(json.dumps(zres))
         return
     print(json.dumps(_warmstart_result(args, train, test, info, [seq], zres)))
 
@@ -424,6 +445,7 @@ def warmstart_main(args):
 def _warmstart_zones(args, test, gtest, cfg_of, refs, peaks, ws, ZoneSolver, log, net):
     """Plain and learned zones to the tolerance (and optionally past it)."""
     prog = lambda i, n: log(f"  zone {i} done at step {n}")
+    chunk_log = _every(10000, lambda it, nl: log(f"  step {it}, {nl} running"))
 
     def zones():
         z = ZoneSolver(gtest, [cfg_of(c) for c in test])
@@ -433,23 +455,35 @@ def _warmstart_zones(args, test, gtest, cfg_of, refs, peaks, ws, ZoneSolver, log
     rebuild = None
     if args.compact > 0:
         rebuild = lambda ids: ZoneSolver([gtest[i] for i in ids], [cfg_of(test[i]) for i in ids])
-    zp = zones()
-    t_zp, st_zp, aero_zp, stat_zp = ws.solve_zones_to_tolerance(zp, refs, tol=args.tol, progress=prog,
-                                                                rebuild=rebuild, compact=args.compact)
-    log(f"plain, zones: {t_zp:.1f} s")
-    zl = zones()
-    zl.sync()
-    t0 = time.perf_counter()
-    ws.apply_predictions_zones(zl, test, net, mach_gate=args.mach_gate if args.mach_gate > 0 else None)
-    zl.sync()
-    t_inf = time.perf_counter() - t0
-    t_zl, st_zl, aero_zl, stat_zl = ws.solve_zones_to_tolerance(zl, refs, tol=args.tol, progress=prog,
-                                                                rebuild=rebuild, compact=args.compact)
-    log(f"learned, zones: {t_inf:.2f} s inference + {t_zl:.1f} s")
-    out = {"plain_zones_s": t_zp, "learned_zones_s": t_zl, "inference_s": t_inf,
-           "plain_zones_steps": [int(v) for v in st_zp], "learned_zones_steps": [int(v) for v in st_zl],
-           "aero_plain_zones": aero_zp, "aero_learned_zones": aero_zl,
-           "status_plain_zones": stat_zp, "status_learned_zones": stat_zl}
+    out = {}
+    which = args.zones_run
+    if which in ("plain", "both"):
+        zp = zones()
+        t_zp, st_zp, aero_zp, stat_zp = ws.solve_zones_to_tolerance(zp, refs, tol=args.tol, progress=prog,
+                                                                    rebuild=rebuild, compact=args.compact,
+                                                                    on_chunk=chunk_log, max_steps=args.max_steps)
+        log(f"plain, zones: {t_zp:.1f} s")
+        out.update({"plain_zones_s": t_zp, "plain_zones_steps": [int(v) for v in st_zp],
+                    "aero_plain_zones": aero_zp, "status_plain_zones": stat_zp})
+    if which in ("learned", "both"):
+        zl = zones()
+        zl.sync()
+        t0 = time.perf_counter()
+        ws.apply_predictions_zones(zl, test, net, mach_gate=args.mach_gate if args.mach_gate > 0 else None)
+        zl.sync()
+        t_inf = time.perf_counter() - t0
+        t_zl, st_zl, aero_zl, stat_zl = ws.solve_zones_to_tolerance(zl, refs, tol=args.tol, progress=prog,
+                                                                    rebuild=rebuild, compact=args.compact,
+                                                                    on_chunk=chunk_log, max_steps=args.max_steps)
+        log(f"learned, zones: {t_inf:.2f} s inference + {t_zl:.1f} s")
+        out.update({"learned_zones_s": t_zl, "inference_s": t_inf, "learned_zones_steps": [int(v) for v in st_zl],
+                    "aero_learned_zones": aero_zl, "status_learned_zones": stat_zl})
+    if which != "both":
+        if which == "plain":
+            zp.close()
+        else:
+            zl.close()
+        return out
     if args.tight and args.compact <= 0:
         # both zone runs continued (untimed) until every equation's norm (SA
         # included) is 100x below tol times its peak: the forces both starts reach
@@ -537,7 +571,28 @@ def _warmstart_combine(args, wsdir, train, test):
     covered = [i for s in seqs for i in range(*s["range"])]
     if seqs and covered != list(range(len(test))):
         raise SystemExit(f"seq phases cover {covered[:3]}..{covered[-3:]} of {len(test)} cases; run the missing ranges")
-    z = json.loads((wsdir / "zones.json").read_text())
+    if (wsdir / "zones.json").exists():
+        z = json.loads((wsdir / "zones.json").read_text())
+    else:  # the zone runs made by separate processes, possibly over parts of the cases
+        z = {}
+        for run in ("plain", "learned"):
+            parts = [json.loads(p.read_text()) for p in sorted(wsdir.glob(f"zones_{run}*.json"))]
+            n = len(test)
+            merged = {}
+            for part in parts:
+                for key, val in part.items():
+                    if key == "test_ids":
+                        continue
+                    if isinstance(val, list):
+                        merged.setdefault(key, [None] * n)
+                        for i, v in zip(part["test_ids"], val):
+                            merged[key][i] = v
+                    else:  # seconds: the parts ran one after another
+                        merged[key] = merged.get(key, 0.0) + val
+            missing = [i for i in range(n) if merged.get(f"status_{run}_zones", [None] * n)[i] is None]
+            if missing:
+                raise SystemExit(f"zones_{run}: cases {missing[:5]}... not run")
+            z.update(merged)
     print(json.dumps(_warmstart_result(args, train, test, info, seqs, z)))
 
 
@@ -573,6 +628,10 @@ def main():
     ap.add_argument("--test-max-cells", type=int, default=200_000,
                     help="warmstart: largest held-out grid (the batched sweep's range is 20k-200k)")
     ap.add_argument("--test-ids", default="", help="warmstart: comma-separated subset of the held-out cases")
+    ap.add_argument("--max-steps", type=int, default=250_000, help="warmstart: step cap of the held-out zone runs")
+    ap.add_argument("--zones-run", default="both", choices=["both", "plain", "learned"],
+                    help="warmstart zones phase: both zone runs, or one of them (zones_plain.json / "
+                         "zones_learned.json, merged by the combine phase)")
     ap.add_argument("--mlp-steps", type=int, default=3000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cfl", type=float, default=10.0, help="final CFL of the warm-start cases")
@@ -762,7 +821,7 @@ def residual_bench(s, grid, args, flux_bytes=8):
     copy_ms = s.copy_bench(args.copy_bytes, iters=20)
     copy_gbs = 2.0 * args.copy_bytes / (copy_ms * 1e-3) / 1e9
     kb = kernel_bytes(grid.n_cells, grid.n_bfaces, grid.n_faces, grid.dim, flux_bytes, s.cfg.iturb_visflux,
-                      s.cfg.ivischeme)
+                      s.cfg.ivischeme, s.kernel_variant(4))
     names = {"grad": "k_ns_grad",
              "limiter": "k_ns_qext",
              "face": "k_ns_face",
@@ -802,7 +861,8 @@ def residual_bench(s, grid, args, flux_bytes=8):
               "cell_steps_per_s": grid.n_cells / (ps_ms * 1e-3)}
     return {"value": grid.n_cells * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline,
             "kernels": kernels, "dominant_kernel": dominant, "pseudo_step": pseudo, "copy_kernel_gbs": copy_gbs,
-            "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "face_split": s.kernel_variant(3)}}
+            "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "face_split": s.kernel_variant(3),
+                       "sa_geometry": s.kernel_variant(4)}}
 
 
 def oracle_error(grid, cfg, fp32=True):

@@ -46,71 +46,83 @@ enum { EPI_PARTIAL = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3 };
 
 // C[M x N] = A[M x K] Bt[N x K]^T: both operands bf16 with k contiguous
 // (row strides lda, ldb), so every global and LDS access moves 8 elements
-// (16 bytes) at a time.  M and N multiples of 64; blockIdx.z selects the K
-// range [z*kchunk, (z+1)*kchunk), a multiple of 16 (an 8-element vector at or
-// past its end is zero-filled, so K = 16 runs through the 32-deep MFMA with
-// zero products added).  256 threads per 64 x 64 tile; wave w owns rows
-// 32*(w>>1).. and columns 32*(w&1).. as 2 x 2 tiles of gfx950's
+// (16 bytes) at a time.  BM x BN block tile (64 or 128 each; M and N
+// multiples of them); blockIdx.z selects the K range [z*kchunk,
+// (z+1)*kchunk), a multiple of 16 (an 8-element vector at or past its end is
+// zero-filled, so K = 16 runs through the 32-deep MFMA with zero products
+// added).  256 threads: wave w owns rows (BM/2)*(w>>1).. and columns
+// (BN/2)*(w&1).. as (BM/32) x (BN/32) tiles of gfx950's
 // v_mfma_f32_16x16x32_bf16.  MFMA operands: lane l supplies A[l%16][8*(l/16)
 // + 0..7] and Bt[l%16][8*(l/16) + 0..7] and receives D[4*(l/16)+r][l%16],
 // r = 0..3.  K advances KC at a time through LDS (one barrier pair per KC/32
 // MFMA steps); the next chunk's global loads are issued into registers before
 // the current chunk's MFMAs, so their latency overlaps the arithmetic.  The
-// MFMA chain runs over k in ascending order of 32-wide steps.
-template <int EPI, int KC>
+// MFMA chain runs over k in ascending order of 32-wide steps.  The larger
+// tiles re-read each operand from L2 fewer times (the forward of many samples).
+template <int EPI, int KC, int BM, int BN>
 __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ Bt, int ldb,
                                               int kchunk, int M, float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc,
                                               const float* __restrict__ bias, const float* __restrict__ zin,
                                               float* __restrict__ zout) {
     constexpr int PAD = 8;              // bf16 elements of row padding (LDS bank spread)
     constexpr int VR = KC / 8;          // 8-element vectors per tile row
-    constexpr int NV = 64 * VR / 256;   // vectors per thread per operand tile
-    static_assert(KC % 32 == 0 && NV >= 1, "KC must be a multiple of 32");
-    __shared__ __align__(16) bf16 sA[64][KC + PAD];
-    __shared__ __align__(16) bf16 sB[64][KC + PAD];
+    constexpr int NVA = BM * VR / 256;  // vectors per thread of the A tile
+    constexpr int NVB = BN * VR / 256;  // and of the Bt tile
+    constexpr int MT = BM / 32, NT = BN / 32;  // MFMA tiles per wave
+    static_assert(KC % 32 == 0 && NVA >= 1 && NVB >= 1, "tile shape");
+    __shared__ __align__(16) bf16 sA[BM][KC + PAD];
+    __shared__ __align__(16) bf16 sB[BN][KC + PAD];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (BM / 2) * (w >> 1), wn = (BN / 2) * (w & 1);
     const int kb = blockIdx.z * kchunk, ke = kb + kchunk;
-    v4f acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    v4f acc[MT][NT];
+    for (int i = 0; i < MT; ++i)
+        for (int j = 0; j < NT; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     // vector e = u*256 + t of a tile: row e / VR, k offset 8*(e % VR)
-    v4u ra[NV], rb[NV];
+    v4u ra[NVA], rb[NVB];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < NV; ++u) {
+        for (int u = 0; u < NVA; ++u) {
             const int e = u * 256 + t, r = e / VR, kv = 8 * (e % VR);
-            const bool ok = k0 + kv < ke;
-            ra[u] = ok ? *(const v4u*)&A[(size_t)(m0 + r) * lda + k0 + kv] : v4u{0u, 0u, 0u, 0u};
-            rb[u] = ok ? *(const v4u*)&Bt[(size_t)(n0 + r) * ldb + k0 + kv] : v4u{0u, 0u, 0u, 0u};
+            ra[u] = (k0 + kv < ke) ? *(const v4u*)&A[(size_t)(m0 + r) * lda + k0 + kv] : v4u{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < NVB; ++u) {
+            const int e = u * 256 + t, r = e / VR, kv = 8 * (e % VR);
+            rb[u] = (k0 + kv < ke) ? *(const v4u*)&Bt[(size_t)(n0 + r) * ldb + k0 + kv] : v4u{0u, 0u, 0u, 0u};
         }
     };
     gload(kb);
     for (int k0 = kb; k0 < ke; k0 += KC) {
         __syncthreads();  // the previous chunk's MFMAs have read LDS
 #pragma unroll
-        for (int u = 0; u < NV; ++u) {
+        for (int u = 0; u < NVA; ++u) {
             const int e = u * 256 + t, r = e / VR, kv = 8 * (e % VR);
             *(v4u*)&sA[r][kv] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NVB; ++u) {
+            const int e = u * 256 + t, r = e / VR, kv = 8 * (e % VR);
             *(v4u*)&sB[r][kv] = rb[u];
         }
         __syncthreads();
         if (k0 + KC < ke) gload(k0 + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 32) {
-            v8bf a[2], b[2];
-            for (int i = 0; i < 2; ++i) a[i] = *(const v8bf*)&sA[32 * (w >> 1) + 16 * i + (lane & 15)][ks + 8 * (lane >> 4)];
-            for (int j = 0; j < 2; ++j) b[j] = *(const v8bf*)&sB[32 * (w & 1) + 16 * j + (lane & 15)][ks + 8 * (lane >> 4)];
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j)
+            v8bf a[MT], b[NT];
+            for (int i = 0; i < MT; ++i) a[i] = *(const v8bf*)&sA[wm + 16 * i + (lane & 15)][ks + 8 * (lane >> 4)];
+            for (int j = 0; j < NT; ++j) b[j] = *(const v8bf*)&sB[wn + 16 * j + (lane & 15)][ks + 8 * (lane >> 4)];
+            for (int i = 0; i < MT; ++i)
+                for (int j = 0; j < NT; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < MT; ++i)
+        for (int j = 0; j < NT; ++j)
             for (int r = 0; r < 4; ++r) {
-                int row = m0 + 32 * (w >> 1) + 16 * i + 4 * (lane >> 4) + r;
-                int col = n0 + 32 * (w & 1) + 16 * j + (lane & 15);
+                int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                int col = n0 + wn + 16 * j + (lane & 15);
                 size_t o = (size_t)row * ldc + col;
                 float v = acc[i][j][r];
                 if (EPI == EPI_PARTIAL) {
@@ -349,12 +361,23 @@ int gemm_on(hipStream_t st, const bf16* A, int lda, const bf16* Bt, int ldb, int
         set_error("mlp: GEMM shape not tiled");
         return OFX_EINVAL;
     }
-    dim3 grid(N / 64, M / 64, nsplit);
     int kc = K / nsplit;
-    if (kc % 64 == 0)
-        k_gemm<EPI, 64><<<grid, 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
-    else
-        k_gemm<EPI, 32><<<grid, 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+    // 128-row tiles (and 128 columns where N allows) once there are rows
+    // enough to fill the GPU with them: fewer L2 re-reads of both operands
+    if (nsplit == 1 && M % 128 == 0 && M >= 65536 && kc % 64 == 0) {
+        if (N % 128 == 0)
+            k_gemm<EPI, 64, 128, 128><<<dim3(N / 128, M / 128, 1), 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc,
+                                                                                 bias, zin, zout);
+        else
+            k_gemm<EPI, 64, 128, 64><<<dim3(N / 64, M / 128, 1), 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc,
+                                                                               bias, zin, zout);
+    } else {
+        dim3 grid(N / 64, M / 64, nsplit);
+        if (kc % 64 == 0)
+            k_gemm<EPI, 64, 64, 64><<<grid, 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+        else
+            k_gemm<EPI, 32, 64, 64><<<grid, 256, 0, st>>>(A, lda, Bt, ldb, kc, M, Cf, Cb, ldc, bias, zin, zout);
+    }
     OFX_HIP(hipGetLastError());
     return 0;
 }

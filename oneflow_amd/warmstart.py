@@ -169,7 +169,7 @@ def _case_status(norm, limit, it, max_steps):
 
 
 def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None, rebuild=None,
-                             compact=0.5, first_iter=1):
+                             compact=0.5, first_iter=1, on_chunk=None):
     """All cases as zones of one context advance together until every
     zone's density residual norm is below tol * refs[case], its norm is not
     finite (diverged: the reference stops on NaN,
@@ -182,7 +182,8 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
     seconds including any rebuilds, step at which each case finished, forces
     [CL, CD, CDp, CDf] of each case at that step (None if diverged), status
     of each case: CONVERGED, MAX_STEPS or DIVERGED).  first_iter: the outer
-    step the context is at (steps already taken + 1)."""
+    step the context is at (steps already taken + 1).  on_chunk(step, n_running)
+    is called after every chunk of steps."""
     n = zs.n_zones
     steps = [0] * n
     aero = [None] * n
@@ -214,6 +215,8 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
                 if progress:
                     progress(cid, steps[cid])
         live = [cid for cid in ids if status[cid] is None]
+        if on_chunk:
+            on_chunk(it - 1, len(live))
         if not live:
             break
         if rebuild is not None and len(live) < len(ids) and (diverged or len(live) <= compact * len(ids)):
