@@ -545,8 +545,9 @@ def _warmstart_result(args, train, test, info, seqs, z):
         "seconds": {"plain_sequential": t_seq, "plain_zones": t_zp, "learned_zones": t_zl, "inference": t_inf},
         "speedup": {"zones": t_seq / t_zp if t_seq else None, "warm_start": t_zp / (t_inf + t_zl),
                     "total": t_seq / (t_inf + t_zl) if t_seq else None},
-        "status": {"plain_sequential": count(stat_seq), "plain_zones": count(stat_zp),
+        "status": {"plain_sequential": count(stat_seq) if seqs else None, "plain_zones": count(stat_zp),
                    "learned_zones": count(stat_zl)},
+        "forces_compared_against": "plain case after case" if seqs else "plain zones",
         "aero_rel_diff_learned_vs_plain": {
             "max_CL": float(dcl.max()) if len(ok) else None, "max_CD": float(dcd.max()) if len(ok) else None,
             "cases_within_1e-6": int(np.sum((dcl <= 1e-6) & (dcd <= 1e-6))), "cases_compared": len(ok),
