@@ -15,6 +15,9 @@
 
 #include "../../include/ofx.h"
 
+#define OFX_HD __host__ __device__ __forceinline__
+#include "ofx_math.h"
+
 namespace ofx {
 
 // codes/basic/include/HXTypeBasic.h:33-34, codes/basic/include/Constant.h

@@ -84,13 +84,13 @@ __device__ __forceinline__ void ns_bc_face(const DevMesh& m, const DevFields& d,
             double cb = 0.25 * (riemp - riemm) * gamm1;
             double vtx, vty, vtz, entr;
             if (vnb >= 0.0) {
-                entr = pin / pow(rin, gama);
+                entr = pin / ofxm::ofx_pow(rin, gama);
                 vtx = uin - xfn * vnin; vty = vin - yfn * vnin; vtz = win - zfn * vnin;
             } else {
-                entr = pref / pow(rref, gama);
+                entr = pref / ofxm::ofx_pow(rref, gama);
                 vtx = uref - xfn * vnref; vty = vref - yfn * vnref; vtz = wref - zfn * vnref;
             }
-            double rb = pow((cb * cb / (entr * gama)), 1.0 / gamm1);
+            double rb = ofxm::ofx_pow((cb * cb / (entr * gama)), 1.0 / gamm1);
             t[0] = rb; t[1] = vtx + xfn * vnb; t[2] = vty + yfn * vnb; t[3] = vtz + zfn * vnb;
             t[4] = cb * cb * rb / gama;
         }

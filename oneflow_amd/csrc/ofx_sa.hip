@@ -294,7 +294,7 @@ __device__ __forceinline__ void sa_source(const DevMesh& m, const DevFields& d, 
     double gg2 = gg * gg;
     double g6 = gg2 * gg2 * gg2;
     double or6 = one / six;
-    double fw = gg * pow((one + k.cw36) / (g6 + k.cw36), or6);
+    double fw = gg * ofxm::ofx_pow((one + k.cw36) / (g6 + k.cw36), or6);
     const double ft2 = 0.0;
     double grd2 = sqr3(d.tgrad[i], d.tgrad[T + i], d.tgrad[2 * T + i]);
     double prod = k.cb1 * (one - ft2) * stdv * nuet;

@@ -13,6 +13,8 @@
 
 #include "oracle.h"
 
+#include "../oneflow_amd/csrc/ofx_math.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -542,13 +544,13 @@ void ns_bc(ofo_case& c) {
                 double cb = fourth * (riemp - riemm) * gamm1;
                 double vtx, vty, vtz, entr;
                 if (vnb >= 0.0) {
-                    entr = pin / std::pow(rin, gama);
+                    entr = pin / ofxm::ofx_pow(rin, gama);
                     vtx = uin - xfn * vnin; vty = vin - yfn * vnin; vtz = win - zfn * vnin;
                 } else {
-                    entr = pref / std::pow(rref, gama);
+                    entr = pref / ofxm::ofx_pow(rref, gama);
                     vtx = uref - xfn * vnref; vty = vref - yfn * vnref; vtz = wref - zfn * vnref;
                 }
-                double rb = std::pow((cb * cb / (entr * gama)), 1.0 / gamm1);
+                double rb = ofxm::ofx_pow((cb * cb / (entr * gama)), 1.0 / gamm1);
                 double ub = vtx + xfn * vnb;
                 double vb = vty + yfn * vnb;
                 double wb = vtz + zfn * vnb;
@@ -1389,11 +1391,13 @@ void sa_source_cell(const ofo_case& c, int i, double& res_sa, double& spec_sa) {
     double ostd = one / (stdv + SMALL);
     double r = rs * ostd;
     r = MIN(r, ten);
-    double r5 = std::pow(r, 5);
+    double r2 = r * r;
+    double r5 = r2 * r2 * r;
     double gg = r + c.cw2 * (r5 * r - r);
-    double g6 = std::pow(gg, 6);
+    double gg2 = gg * gg;
+    double g6 = gg2 * gg2 * gg2;
     double or6 = one / six;
-    double fw = gg * std::pow((one + c.cw36) / (g6 + c.cw36), or6);
+    double fw = gg * ofxm::ofx_pow((one + c.cw36) / (g6 + c.cw36), or6);
     double ft2 = 0.0;  // ft2_flag is never set in the reference (zero-initialised global)
     double grd2 = SQR(c.turbdqdx[i], c.turbdqdy[i], c.turbdqdz[i]);
     double prod = c.cb1 * (one - ft2) * stdv * nuet;
@@ -1402,7 +1406,9 @@ void sa_source_cell(const ofo_case& c, int i, double& res_sa, double& spec_sa) {
     diff = MIN(c.cfg.rprod * prod, diff);
     double srcTerm = prod - dest;
     srcTerm += diff;
-    double dfv2dk = (three * c.cv13 * std::pow(fv1 / (xsi + SMALL), 2) - one) / std::pow(one + xsi * fv1, 2);
+    double p2 = fv1 / (xsi + SMALL);
+    double p3 = one + xsi * fv1;
+    double dfv2dk = (three * c.cv13 * (p2 * p2) - one) / (p3 * p3);
     double dsdnu = c.oreynolds * c.okarm2 * od2 * (fv2 + xsi * dfv2dk);
     double dfwdg = fw / (gg + SMALL) * (one - g6 / (g6 + c.cw36));
     double dgdr = one + c.cw2 * (six * r5 - one);
@@ -1927,6 +1933,11 @@ int ofo_face_flux(ofo_case* c, double* inv, double* vis) {
         std::copy(c->invflux[e].begin(), c->invflux[e].end(), inv + (size_t)e * F);
         std::copy(c->visflux[e].begin(), c->visflux[e].end(), vis + (size_t)e * F);
     }
+    return 0;
+}
+
+int ofo_pow(const double* x, const double* y, int n, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = ofxm::ofx_pow(x[i], y[i]);
     return 0;
 }
 

@@ -53,6 +53,8 @@ int  ofo_wall_count(ofo_case* c);
 int  ofo_wall_data(ofo_case* c, double* out);
 /* Face flux arrays of the last ofo_residual_ns: inv[5][F], vis[5][F]. */
 int  ofo_face_flux(ofo_case* c, double* inv, double* vis);
+/* The shared x^y routine (ofx_math.h) for n pairs: out[i] = x[i]^y[i]. */
+int  ofo_pow(const double* x, const double* y, int n, double* out);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,7 @@ def lib():
         L.ofo_wall_count.argtypes = [P]
         L.ofo_wall_data.argtypes = [P, dp]
         L.ofo_face_flux.argtypes = [P, dp, dp]
+        L.ofo_pow.argtypes = [dp, dp, ctypes.c_int, dp]
         _lib = L
     return _lib
 

@@ -1,8 +1,10 @@
 """GPU (libofx.so) vs the CPU oracle on the reference's own grids, task by task.
 
-Every floating-point result must agree with the oracle to 1e-12 normwise
-relative error (FP64 tolerance of the path); most agree bit for bit because
-both sides keep the reference's operation order with -ffp-contract=off.
+Every field must equal the oracle's bit for bit: both sides keep the
+reference's operation order with -ffp-contract=off, and the one function
+whose device and host library versions differ (pow) is the shared routine of
+oneflow_amd/csrc/ofx_math.h on both.  The residual norms are a reduction in a
+different order on the GPU (a deterministic tree) and are compared at 1e-12.
 Whole 50-step runs must also reproduce the reference goldens (res.dat,
 turbres.dat, aero.dat) under the reference harness rule.
 """
@@ -18,7 +20,8 @@ from tests.parity_util import report
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1.0e-12
+TOL = 0.0
+NORM_TOL = 1.0e-12  # the norms' reduction order differs
 CASES = ["plateuns2dslau2", "turbplateuns2droe_sa", "rae2822_roe_sa"]
 
 
@@ -142,8 +145,8 @@ def test_fifty_steps_against_goldens(case):
     norms = np.array(norms)
     onorms = ora.iterate(1, 50)
     e = report("norms", norms, onorms)
-    assert e <= 1.0e-10, e
-    cmp(gpu, ora, ["Q", "TURBQ"], tol=1.0e-10)
+    assert e <= NORM_TOL, e
+    cmp(gpu, ora, ["Q", "TURBQ"])
     gold = load_res(GOLDEN / case / "res.dat", 5)
     for it in range(50):
         for q in range(5):
@@ -163,7 +166,7 @@ def test_fifty_steps_against_goldens(case):
     # wallaero.dat after step 50: per wall face -cp and cf at 11 digits
     wall = gpu.wall_data()
     ew = report("wall", wall, ora.wall_data())
-    assert ew <= 1.0e-10, ew
+    assert ew == 0.0, ew
     mine = wallaero_lines(wall)
     gold_w = (GOLDEN / case / "wallaero.dat").read_text().splitlines()
     assert len(mine) == len(gold_w)
@@ -180,8 +183,8 @@ def test_synthetic_twenty_steps(case):
     ng = gpu.iterate(1, 20)
     no = ora.iterate(1, 20)
     e = report("norms", ng, no)
-    assert e <= 1.0e-10, e
-    cmp(gpu, ora, ["Q", "TURBQ"], tol=1.0e-10)
+    assert e <= NORM_TOL, e
+    cmp(gpu, ora, ["Q", "TURBQ"])
 
 
 def test_continue_from_reference_restart():
@@ -192,8 +195,8 @@ def test_continue_from_reference_restart():
     norms, aero = continued(Solver)
     onorms, oaero = continued(OracleCase)
     e = report("norms", norms, onorms)
-    assert e <= 1.0e-10, e
-    assert np.allclose(aero, oaero, rtol=1e-10, atol=1e-16)
+    assert e <= NORM_TOL, e
+    assert report("aero", aero, oaero) == 0.0
     check_against_goldens(norms, aero)
 
 
