@@ -317,7 +317,7 @@ def warmstart_main(args):
     phase = args.phase
     wsdir = Path(args.ws_dir)
     wsdir.mkdir(parents=True, exist_ok=True)
-    train = airfoil_sweep(args.n_train, seed=args.seed + 1, min_cells=20_000, max_cells=args.train_max_cells)
+    train = airfoil_sweep(args.n_train, seed=args.seed + 1, min_cells=args.train_min_cells, max_cells=args.train_max_cells)
     test = airfoil_sweep(args.n_test, seed=args.seed, min_cells=20_000, max_cells=args.test_max_cells)
     if args.test_ids:
         test = [test[int(i)] for i in args.test_ids.split(",")]
@@ -338,8 +338,9 @@ def warmstart_main(args):
         gtrain = grids_of(train)
         # 1. training fields: the training cases as zones of one context, each
         #    converged to the tolerance (a tol drop from its peak density
-        #    residual over the first 1000 steps); converged zones keep
-        #    iterating in place until the last one converges
+        #    residual over the first 1000 steps).  Converged cases leave the run
+        #    as the held-out runs do (--compact); each case's samples are read
+        #    from the last context that holds it, once, when that context is left.
         zt = ZoneSolver(gtrain, [cfg_of(c) for c in train])
         zt.init_freestream()
         t0 = time.perf_counter()
@@ -347,17 +348,30 @@ def warmstart_main(args):
         for it in range(1, 1001, 100):
             zt.iterate(it, 100, with_norms=False)
             peak = np.maximum(peak, zt.zone_norms()[:, 0])
+        xs, ys = {}, {}
+
+        def take(ctx, pairs):
+            fc = ws.FieldCache(ctx)
+            for z, cid in pairs:
+                v = ws.ZoneView(fc, z)
+                xs[cid] = ws.cell_features(v, train[cid])
+                ys[cid] = ws.cell_targets(v, train[cid], v.cfg)
+        rebuild_t = None
+        if args.compact > 0:
+            rebuild_t = lambda ids: ZoneSolver([gtrain[i] for i in ids], [cfg_of(train[i]) for i in ids])
         _, tsteps, _, tstat = ws.solve_zones_to_tolerance(
             zt, list(peak), tol=args.tol, chunk=2000, max_steps=args.train_steps, first_iter=1001,
             progress=lambda i, n: log(f"  training case {i} done at step {n}"),
-            on_chunk=_every(10000, lambda it, nl: log(f"  training fields: step {it}, {nl} running")))
+            on_chunk=_every(10000, lambda it, nl: log(f"  training fields: step {it}, {nl} running")),
+            rebuild=rebuild_t, compact=args.compact, on_leave=take)
         t_gen = time.perf_counter() - t0
-        drop = zt.zone_norms()[:, 0] / peak
-        log(f"training fields: steps {min(tsteps)}-{max(tsteps)}, {t_gen:.1f} s, final drops {drop.min():.1e}-{drop.max():.1e}")
-        views = [ws.ZoneView(zt, z) for z in range(zt.n_zones)]
-        X = np.concatenate([ws.cell_features(v, c) for v, c in zip(views, train)])
-        Y = np.concatenate([ws.cell_targets(v, c, v.cfg) for v, c in zip(views, train)])
+        used = [i for i in range(len(train)) if tstat[i] == ws.CONVERGED]
+        log(f"training fields: steps {min(tsteps)}-{max(tsteps)}, {t_gen:.1f} s, {len(used)} of {len(train)} converged")
+        X = np.concatenate([xs[i] for i in used])
+        Y = np.concatenate([ys[i] for i in used])
+        del xs, ys
         zt.close()
+        n_train_cells = int(sum(g.n_cells for g in gtrain))
         del gtrain
         log(f"training fields: {X.shape[0]} samples in {t_gen:.1f} s")
         # 2. train the network on the GPU
@@ -374,7 +388,7 @@ def warmstart_main(args):
         t_fwd = time.perf_counter() - t0
         info = {"train_cases": len(train), "train_samples": int(X.shape[0]), "train_max_steps": args.train_steps,
                 "train_case_steps": [int(v) for v in tsteps], "train_case_status": tstat,
-                "train_final_drop": [float(v) for v in drop],
+                "train_cases_used": len(used), "train_cells": n_train_cells,
                 "training_fields_s": t_gen, "training_s": t_train,
                 "final_loss": float(losses[-50:].mean()), "first_loss": float(losses[:10].mean()),
                 "train_tflops": flops_step * args.mlp_steps / t_train / 1e12, "forward_1M_ms": 1e3 * t_fwd,
@@ -625,6 +639,7 @@ def main():
     ap.add_argument("--n-test", type=int, default=16)
     ap.add_argument("--train-steps", type=int, default=200_000,
                     help="warmstart: step cap while the training fields converge to --tol")
+    ap.add_argument("--train-min-cells", type=int, default=20_000, help="warmstart: smallest training grid")
     ap.add_argument("--train-max-cells", type=int, default=50_000, help="warmstart: largest training grid")
     ap.add_argument("--test-max-cells", type=int, default=200_000,
                     help="warmstart: largest held-out grid (the batched sweep's range is 20k-200k)")

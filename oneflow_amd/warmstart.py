@@ -143,6 +143,23 @@ class ZoneView:
         return self.zs.get_zone(name, self.z)
 
 
+class FieldCache:
+    """A ZoneSolver whose fields are downloaded once each, for reading many
+    zones of one state (ZoneView over it)."""
+
+    def __init__(self, zs):
+        self.zs = zs
+        self.cell_start, self.zone_cfgs = zs.cell_start, zs.zone_cfgs
+        self._f = {}
+
+    def get_zone(self, name, z):
+        if name not in self._f:
+            self._f[name] = self.zs.get(name)
+        a = self._f[name]
+        cs, gs = self.zs.zone_slice(z)
+        return np.concatenate([a[..., cs], a[..., gs]], axis=-1)
+
+
 CONVERGED, MAX_STEPS, DIVERGED = "converged", "max_steps", "diverged"
 
 
@@ -169,7 +186,7 @@ def _case_status(norm, limit, it, max_steps):
 
 
 def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, progress=None, rebuild=None,
-                             compact=0.5, first_iter=1, on_chunk=None):
+                             compact=0.5, first_iter=1, on_chunk=None, on_leave=None):
     """All cases as zones of one context advance together until every
     zone's density residual norm is below tol * refs[case], its norm is not
     finite (diverged: the reference stops on NaN,
@@ -183,7 +200,11 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
     [CL, CD, CDp, CDf] of each case at that step (None if diverged), status
     of each case: CONVERGED, MAX_STEPS or DIVERGED).  first_iter: the outer
     step the context is at (steps already taken + 1).  on_chunk(step, n_running)
-    is called after every chunk of steps."""
+    is called after every chunk of steps.  on_leave(context, [(zone, case)])
+    is called once for every case, with the context that holds its last state,
+    just before that context is left (at a rebuild, for the cases that
+    finished in it; at the end, for all cases of the last context): the
+    finished cases' states are read there without a download per case."""
     n = zs.n_zones
     steps = [0] * n
     aero = [None] * n
@@ -232,9 +253,13 @@ def solve_zones_to_tolerance(zs, refs, tol=1e-8, chunk=500, max_steps=150_000, p
             new.set("TURBQ", nun)
             new.ns_boundary()
             new.sa_boundary()
+            if on_leave:
+                on_leave(cur, [(z, cid) for z, cid in enumerate(ids) if status[cid] is not None])
             if cur is not zs:
                 cur.close()
             cur, ids = new, live
+    if on_leave:
+        on_leave(cur, list(enumerate(ids)))
     if cur is not zs:
         cur.close()
     return time.perf_counter() - t0, steps, aero, status

@@ -95,3 +95,31 @@ def test_case_status_rules():
     assert ws._case_status(0.5, 1.0, 10, 100) == ws.CONVERGED
     assert ws._case_status(2.0, 1.0, 101, 100) == ws.MAX_STEPS
     assert ws._case_status(2.0, 1.0, 10, 100) is None
+
+
+def test_every_case_leaves_once_with_its_context():
+    """on_leave sees each case exactly once, in the context that holds its
+    last state (the first context for the cases finished before a rebuild)."""
+    seen = []
+
+    def rebuild(ids):
+        return FakeZones(schedules(), ids)
+    fz = FakeZones(schedules())
+    ws.solve_zones_to_tolerance(fz, [1.0] * 3, tol=1e-8, chunk=100, max_steps=1000, rebuild=rebuild, compact=0.0,
+                                on_leave=lambda ctx, pairs: seen.extend((ctx.ids[z], cid) for z, cid in pairs))
+    assert sorted(cid for _, cid in seen) == [0, 1, 2]
+    assert all(a == b for a, b in seen)
+
+
+def test_field_cache_reads_each_field_once():
+    calls = []
+
+    class Z(FakeZones):
+        zone_cfgs = [None] * 3
+
+        def get(self, name):
+            calls.append(name)
+            return np.arange(5 * self.n_zones, dtype=float).reshape(5, -1) if name == "Q" else np.zeros(self.n_zones)
+    fc = ws.FieldCache(Z(schedules()))
+    a = [ws.ZoneView(fc, z).get("Q") for z in range(3)]
+    assert calls == ["Q"] and a[2][0, 0] == 2.0
