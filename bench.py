@@ -25,11 +25,12 @@ evaluations; peak 8 TB/s.  traffic: the bytes that left L2 per evaluation
 copy kernel) from the committed counter summary of the same workload.
 `kernels` lists each kernel with its own algorithmic bytes and HIP-event
 time; `pseudo_step` the whole pseudo-time step (residual, LU-SGS, update and
-boundary conditions of both solvers); `oracle_error` the FP64 and FP32-flux
-residuals against the CPU oracle on a slice of the same grid.
+boundary conditions of both solvers).
 
-cpu_baseline: the CPU oracle (single thread, the reference's loops) timed on
-this host on a bounded sample of the same workload.
+The CPU leg (left out by --no-cpu) is the only part that runs the oracle:
+cpu_baseline, the CPU oracle (single thread, the reference's loops) timed on
+this host on a bounded sample of the same workload, and oracle_error, the
+GPU's FP64 and FP32-flux residuals checked against the oracle on that sample.
 """
 from __future__ import annotations
 
@@ -744,11 +745,14 @@ def grid_main(args):
         r32 = residual_bench(s, grid, args, flux_bytes=4)
         s.close()
         log(f"fp32 flux: {r32['ms_per_step']:.4f} ms per evaluation")
-    sample = c2_grid(n_span=2) if c2 else grid
-    err = oracle_error(sample, cfg, fp32=c2)
-    log(f"oracle error: {err}")
-    cpu = None
+    # The CPU leg, the only part that runs the oracle: on a bounded sample of
+    # the workload it is timed (the baseline) and checks the GPU's residuals
+    # on the same state (oracle_error).  --no-cpu leaves both out.
+    err, cpu = None, None
     if not args.no_cpu:
+        sample = c2_grid(n_span=2) if c2 else grid
+        err = oracle_error(sample, cfg, fp32=c2)
+        log(f"oracle error: {err}")
         n, dt = cpu_baseline(sample, cfg, args.cpu_budget)
         what = (f"a {sample.n_cells}-cell 2-layer span slice of the same wing grid" if c2
                 else f"the same {grid.n_cells}-cell grid")
