@@ -1,4 +1,5 @@
-"""The update's positivity fixes and the NaN stop on the GPU, against the oracle.
+"""The update's positivity fixes, the reconstruction's first-order fallback and
+the NaN stop on the GPU, against the oracle.
 
 The same forced states as tests/test_oracle_robustness.py (whose expected
 values are computed by hand from NsUpdate::CalcFlowField / WeekSolutionFix,
@@ -15,7 +16,7 @@ from oneflow_amd import meshgen
 from oneflow_amd.config import CaseConfig
 from oracle.pyoracle import OracleCase
 from tests.parity_util import report
-from tests.test_oracle_robustness import expected_update, prim_to_q
+from tests.test_oracle_robustness import expected_update, prim_to_q, spike_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -105,4 +106,22 @@ def test_nan_state_stops_the_step():
     s.set("Q", q2)
     with pytest.raises(OfxError, match="-5"):
         s.iterate(1, 1)
+    s.close()
+
+
+def test_reconstruction_fallback_bitwise():
+    """Unlimited MUSCL on the two-spike state of
+    tests/test_oracle_robustness.py, whose middle face falls back to first
+    order on both sides (LimField::CalcFaceValue, codes/usolver/src/ULimiter.cpp:218-254):
+    the GPU's NS residual equals the oracle's bit for bit."""
+    from oneflow_amd.solver import Solver
+    g, cfg, o, q, face, control = spike_pair()
+    s = Solver(g, cfg)
+    s.init_freestream()
+    s.set("Q", q)
+    s.ns_boundary()
+    for x in (s, o):
+        x.residual_ns()
+    C = g.n_cells
+    assert report("RES", s.get("RES")[:, :C], o.get("RES")[:, :C]) == 0.0
     s.close()

@@ -1,4 +1,5 @@
-"""The update's positivity fixes and the NaN stop, forced on the CPU oracle.
+"""The update's positivity fixes, the reconstruction's first-order fallback
+(at the end) and the NaN stop, forced on the CPU oracle.
 
 NsUpdate::CalcFlowField (codes/ns/src/NsUpdate.cpp:40-67) adds the LU-SGS
 increment to the conservative state and converts back (QToPrim takes |rho|).
@@ -163,3 +164,57 @@ def test_nan_state_stops_the_residual():
     o.timestep(1)
     with pytest.raises(RuntimeError, match="-5"):
         o.residual_ns()
+
+
+# ---- the reconstruction's first-order fallback (LimField::CalcFaceValue /
+# BcQlQrFix, codes/usolver/src/ULimiter.cpp:218-254): a face state with
+# rho <= 0 or p <= 0 from q + phi grad . dx is replaced by the cell's own state
+
+def spike_pair(nx=10, ny=5):
+    """Unlimited MUSCL (ilim 0) on a uniform-pressure box with two density
+    spikes two cells apart on the middle row: the two cells between them have
+    gradients pointing away from their common face, so both reconstructions
+    there are negative and both fall back to the cell states."""
+    g = meshgen.box2d(nx, ny)
+    cfg = CaseConfig(dim=2, ischeme=1, ilim=0, vismodel=0, mach_ref=0.5, aoa_degree=5.0)
+    o = OracleCase(g, cfg)
+    o.init_freestream()
+    q = o.get("Q")
+    xc = o.get("XCC")
+    C = g.n_cells
+    h = 1.0 / nx
+    row = np.abs(xc[1, :C] - 0.5) < 0.5 / ny
+    at = lambda x: int(np.flatnonzero(row & (np.abs(xc[0, :C] - x) < 0.5 * h))[0])
+    s1, a, b, s2 = at(2.5 * h), at(3.5 * h), at(4.5 * h), at(5.5 * h)
+    q[0, s1] = q[0, s2] = 50.0
+    o.set("Q", q)
+    o.ns_boundary()
+    face = next(f for f in range(g.n_bfaces, g.n_faces) if {int(g.lc[f]), int(g.rc[f])} == {a, b})
+    control = next(f for f in range(g.n_bfaces, g.n_faces) if {int(g.lc[f]), int(g.rc[f])} == {s1, a})
+    return g, cfg, o, q, face, control
+
+
+def reconstructed_rho(g, o, q, f):
+    d, fc, xc = o.get("DQDX"), o.get("FACE_CENTER"), o.get("XCC")
+    return [q[0, c] + d[0, 0, c] * (fc[0, f] - xc[0, c]) + d[1, 0, c] * (fc[1, f] - xc[1, c])
+            for c in (int(g.lc[f]), int(g.rc[f]))]
+
+
+def test_reconstruction_falls_back_to_first_order():
+    g, cfg, o, q, face, control = spike_pair()
+    o.residual_ns()
+    # the unlimited reconstructions at the face are negative on both sides
+    # (-11.25), valid beside the spike
+    assert all(r < 0 for r in reconstructed_rho(g, o, q, face))
+    assert all(r > 0 for r in reconstructed_rho(g, o, q, control))
+    inv2, _ = o.face_flux()
+    first = OracleCase(g, cfg.replace(ilim=-1))
+    first.init_freestream()
+    first.set("Q", q)
+    first.ns_boundary()
+    first.residual_ns()
+    inv1, _ = first.face_flux()
+    # both sides fell back: the unlimited flux is the first-order one, bit for bit
+    assert np.array_equal(inv2[:, face], inv1[:, face])
+    # beside a spike the reconstruction is valid and changes the flux
+    assert not np.array_equal(inv2[:, control], inv1[:, control])
