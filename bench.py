@@ -815,14 +815,17 @@ def residual_bench(s, grid, args, flux_bytes=8):
     s.init_freestream()
     s.iterate(1, args.pre_iters, with_norms=False)
     s.timestep(args.pre_iters + 1)
-    # launch configuration: the gather's two forms timed here for the record,
-    # then the autotuner picks the variants and face tiles on this state
+    # launch configuration: the two forms of the gather and of both gradients
+    # timed here for the record, then the autotuner picks the variants and
+    # face tiles on this state
     variant_us = {}
-    t = {}
-    for v in (0, 1):
-        s.set_kernel_variant(2, v)
-        t[v] = 1e3 * s.time_kernel("gather", iters=args.kernel_iters)
-    variant_us["gather"] = t
+    for name, kernel, part in (("gather", 2, "gather"), ("grad", 0, "grad"), ("sa_grad", 5, "sa_grad"),
+                               ("timestep", 6, "timestep"), ("lusgs_ns", 6, "lusgs_ns")):
+        t = {}
+        for v in (0, 1):
+            s.set_kernel_variant(kernel, v)
+            t[v] = 1e3 * s.time_kernel(part, iters=args.kernel_iters)
+        variant_us[name] = t
     tiles = s.autotune(iters=20)
     T, F = s.n_tcell, s.n_faces
     for _ in range(args.warmup):
@@ -850,7 +853,9 @@ def residual_bench(s, grid, args, flux_bytes=8):
     # keys of the counter summary: the instantiation this solver runs
     fp32 = "_fp32" if flux_bytes == 4 else ""
     geo = "_geo" if s.kernel_variant(4) == 1 and s.cfg.iturb_visflux == 2 else ""
-    tkey = {"face": "k_ns_face" + fp32, "gather": names["gather"] + fp32, "sa_face": "k_sa_face" + geo}
+    tkey = {"face": "k_ns_face" + fp32, "gather": names["gather"] + fp32, "sa_face": "k_sa_face" + geo,
+            "grad": "k_ns_grad" + ("_eg" if s.kernel_variant(0) == 1 else ""),
+            "sa_grad": "k_sa_grad" + ("_eg" if s.kernel_variant(5) == 1 else "")}
     prof = {key: s.time_kernel(key, iters=args.kernel_iters) for key in names}
     s.residual_ns()  # leave a consistent residual behind
     s.residual_sa()
@@ -882,7 +887,8 @@ def residual_bench(s, grid, args, flux_bytes=8):
     return {"value": grid.n_cells * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline,
             "kernels": kernels, "dominant_kernel": dominant, "pseudo_step": pseudo, "copy_kernel_gbs": copy_gbs,
             "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "face_split": s.kernel_variant(3),
-                       "sa_geometry": s.kernel_variant(4)}}
+                       "sa_geometry": s.kernel_variant(4), "grad_entry_terms": s.kernel_variant(0),
+                       "sa_grad_entry_terms": s.kernel_variant(5), "step_entry_normals": s.kernel_variant(6)}}
 
 
 def oracle_error(grid, cfg, fp32=True):

@@ -253,9 +253,11 @@ int ofx_wall_data(ofx_ctx* ctx, double* out);
  * residual, 5 whole SA residual, 6 SA gradient (+ ghost values), 7 SA face
  * terms, 8 SA cell terms (source + ordered gather), 9 NS + SA residual (one
  * cell-residual evaluation), 10 one whole pseudo-time step past the CFL ramp
- * (as ofx_iterate; this one advances the state).  Parts 0-9 are idempotent
- * on the current state, so the fields are left as one ofx_residual_ns /
- * ofx_residual_sa call leaves them. */
+ * (as ofx_iterate; this one advances the state), 11 the NS time step past
+ * the CFL ramp, 12 the NS LU-SGS sweeps (lower and upper, every level) on the
+ * current time step and residual.  Parts 0-9 are idempotent on the current
+ * state, so the fields are left as one ofx_residual_ns / ofx_residual_sa
+ * call leaves them; 11 and 12 rewrite the time step and dq of the state. */
 int ofx_time_kernel(ofx_ctx* ctx, int which, int iters, double* ms_out);
 
 /* Bandwidth reference (no counterpart in the reference): average duration
@@ -289,8 +291,14 @@ int ofx_zones_aero(ofx_ctx* ctx, double* out);
 int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
 
 /* Kernel variant (no counterpart in the reference; results do not depend on
- * it).  Kernels 0 and 1, the NS gradient and the limiter, run as one kernel
- * of a single form: only variant 0 is accepted.  kernel 2, the residual
+ * it).  Kernels 0 and 1 both name the NS gradient kernel (it evaluates the
+ * limiter in the same pass), kernel 5 the SA gradient: variant 0 (default)
+ * reads the face terms (weights, normal, face centre offsets) by face id,
+ * variant 1 reads the per-(cell, face) copies stored cell-major at mesh
+ * upload (64 bytes per cell face), contiguous per wave.  kernel 6, the NS
+ * time step and LU-SGS sweeps: variant 0 (default) reads the faces' normals
+ * and areas by face id, variant 1 per-(cell, face) copies of them (32 bytes
+ * per cell face).  kernel 2, the residual
  * gather: variant 0 one lane per face of a cell, variant 1 (default) one
  * thread per cell.  kernel 3, the viscous FP64 NS face kernel: variant 0
  * (default) one thread per face, variant 1 two waves per 64 faces (inviscid
@@ -301,8 +309,9 @@ int ofx_set_kernel_variant(ofx_ctx* ctx, int kernel, int variant);
 int ofx_get_kernel_variant(ofx_ctx* ctx, int kernel, int* variant);
 
 /* Autotuner (no counterpart in the reference): times the NS face kernel for
- * each face tile and split, the gather for each variant, and the SA residual
- * for each SA face tile and geometry variant on the current state (each the
+ * each face tile and split, the gather, both gradients and (LU-SGS runs)
+ * the NS time step with the sweeps for each variant, and the SA residual for
+ * each SA face tile and geometry variant on the current state (each the
  * median of five timings of `iters` launches), keeps the fastest, and
  * reports the face tiles in chosen2[0..1] (may be NULL).
  * Leaves res / turbres as one evaluation of the current state. */

@@ -569,8 +569,8 @@ int ofx_aero_row(ofx_ctx* c, double* out12) {
 
 int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
     CHECK_READY(c);
-    if (!ms_out || iters < 1 || which < 0 || which > 10) {
-        set_error("time_kernel: which must be 0..10 and iters >= 1");
+    if (!ms_out || iters < 1 || which < 0 || which > 12) {
+        set_error("time_kernel: which must be 0..12 and iters >= 1");
         return OFX_EINVAL;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -583,6 +583,8 @@ int ofx_time_kernel(ofx_ctx* c, int which, int iters, double* ms_out) {
         if (which == 5) return sa_residual(c);
         if (which == 9) { OFX_TRY(ns_residual(c)); return sa_residual(c); }
         if (which == 10) return step_launch(c, c->cfg.ncfl + 1);
+        if (which == 11) return ns_timestep(c, c->cfg.ncfl + 1);
+        if (which == 12) return ns_lusgs(c, 0);
         return sa_residual_parts(c, 1u << (which - 6));
     };
     rc = once();  // warm-up launch
@@ -609,27 +611,42 @@ int ofx_set_face_tile(ofx_ctx* c, int ns_tile, int sa_tile) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// The launch-variant switch of kernel id 0..6 (0 and 1 both name the NS
+// gradient kernel, which evaluates the limiter in the same pass; 6 the NS
+// time step and LU-SGS sweeps).
+int& variant_ref(ofx_ctx* c, int kernel) {
+    switch (kernel) {
+        case 2: return c->gather_variant;
+        case 3: return c->face_split;
+        case 4: return c->sa_geom_variant;
+        case 5: return c->sagrad_variant;
+        case 6: return c->normal_variant;
+        default: return c->grad_variant;
+    }
+}
+}  // namespace
+
+extern "C" {
+
 int ofx_set_kernel_variant(ofx_ctx* c, int kernel, int variant) {
     CHECK_CTX(c);
-    if (kernel < 0 || kernel > 4 || variant < 0 || variant > 1) {
-        set_error("kernel variant: kernel 0 (NS gradient), 1 (limiter), 2 (gather), 3 (NS face split) or "
-                  "4 (SA face geometry), variant 0 or 1");
+    if (kernel < 0 || kernel > 6 || variant < 0 || variant > 1) {
+        set_error("kernel variant: kernel 0 or 1 (NS gradient with limiter), 2 (gather), 3 (NS face split), "
+                  "4 (SA face geometry), 5 (SA gradient) or 6 (NS time step and sweeps), variant 0 or 1");
         return OFX_EINVAL;
     }
-    if (kernel <= 1 && variant != 0) {
-        set_error("kernel variant: the gradient and limiter run as one kernel with a single form (variant 0)");
-        return OFX_EINVAL;
-    }
-    if (kernel <= 1) return 0;
-    (kernel == 2 ? c->gather_variant : kernel == 3 ? c->face_split : c->sa_geom_variant) = variant;
+    variant_ref(c, kernel) = variant;
     ++c->gen;
     return 0;
 }
 
 int ofx_get_kernel_variant(ofx_ctx* c, int kernel, int* variant) {
     CHECK_CTX(c);
-    if (!variant || kernel < 0 || kernel > 4) { set_error("kernel variant: kernel 0..4 and non-null out"); return OFX_EINVAL; }
-    *variant = kernel <= 1 ? 0 : kernel == 2 ? c->gather_variant : kernel == 3 ? c->face_split : c->sa_geom_variant;
+    if (!variant || kernel < 0 || kernel > 6) { set_error("kernel variant: kernel 0..6 and non-null out"); return OFX_EINVAL; }
+    *variant = variant_ref(c, kernel);
     return 0;
 }
 
@@ -678,7 +695,39 @@ int ofx_autotune(ofx_ctx* c, int iters, int* chosen2) {
         if (ms < best) { best = ms; pick = v; }
     }
     c->gather_variant = pick;
+    // the NS gradient's face terms: by face id or the per-entry copies (part 0)
+    best = 1e300;
+    pick = c->grad_variant;
+    for (int v = 0; v < 2; ++v) {
+        c->grad_variant = v;
+        OFX_TRY(median_ms(c, 0, iters, &ms));
+        if (ms < best) { best = ms; pick = v; }
+    }
+    c->grad_variant = pick;
+    // the NS time step's and sweeps' face normals (parts 11 and 12, on the
+    // time step and residual of the current state)
+    if (c->cfg.time_integral == 0) {
+        best = 1e300;
+        pick = c->normal_variant;
+        for (int v = 0; v < 2; ++v) {
+            c->normal_variant = v;
+            double ms2 = 0;
+            OFX_TRY(median_ms(c, 11, iters, &ms));
+            OFX_TRY(median_ms(c, 12, iters, &ms2));
+            if (ms + ms2 < best) { best = ms + ms2; pick = v; }
+        }
+        c->normal_variant = pick;
+    }
     if (c->k.vismodel == 3) {
+        // the same choice for the SA gradient (part 6)
+        best = 1e300;
+        pick = c->sagrad_variant;
+        for (int v = 0; v < 2; ++v) {
+            c->sagrad_variant = v;
+            OFX_TRY(median_ms(c, 6, iters, &ms));
+            if (ms < best) { best = ms; pick = v; }
+        }
+        c->sagrad_variant = pick;
         best = 1e300;
         pick = c->saface_tb;
         for (int t : cand) {

@@ -75,6 +75,24 @@ struct DevMesh {
     // and a wave's loads of entry j are contiguous.
     int ellw = 0;
     int *ellf = nullptr, *elln = nullptr;
+    // Static face terms of the cell kernels, once per (cell, entry) in the
+    // same cell-major layout: three planes of ellw * C records, plane p at
+    // p * ellw * C, entry j of cell i at j * C + i (zeros past the cell's
+    // count).  The gradients' terms, as seen from the cell:
+    //   plane 0 {w_self, w_nb, nx A, ny A}: the Green-Gauss weights of the
+    //     cell and of its neighbour (cl, cr when the cell is the face's left
+    //     cell; cr, cl when it is the right one) and the area-weighted normal
+    //     pointing out of the cell (negated for the right cell);
+    //   plane 1 {nz A, xf - xc, yf - yc, zf - zc}: the normal's z part (out of
+    //     the cell) and the face centre relative to the cell centre (z parts
+    //     zero in 2-D).
+    // The gradients' sums are the same bits as from fgg / frec / fz (a sum is
+    // commutative, and acc - x == acc + (-x) in IEEE arithmetic).  For the
+    // time step and the LU-SGS sweeps, as the face stores them:
+    //   plane 2 {xfn, yfn, zfn, area}: unit normal (left to right) and area.
+    // A cell kernel reads them with contiguous loads that do not wait for its
+    // face ids.
+    dvec4* ellg = nullptr;
     // cell -> face CSR in ascending face id; entry = f if the cell is lc[f], ~f if rc[f]
     int *c2fOff = nullptr, *c2f = nullptr;
     // per cell: bit j set when the neighbour of entry j of its face list has a
@@ -184,6 +202,12 @@ struct ofx_ctx {
     int gather_variant = 1;  // residual gather: 0 lanes per cell, 1 one thread per cell
     int sa_geom_variant = 0;  // SA diffusion, TEST method: 0 stored face records, 1 evaluated from the centres
     int face_split = 0;      // viscous NS face kernel: 1 inviscid and viscous parts on separate waves
+    // cell gradients (NS gradient with limiter; SA gradient): 0 face terms
+    // read by face id (fgg / frec / fz), 1 the per-entry copies (DevMesh::ellg)
+    int grad_variant = 0, sagrad_variant = 0;
+    // NS time step and LU-SGS sweeps: 0 face normals and areas read by face
+    // id, 1 the per-entry copies (DevMesh::ellg plane 2)
+    int normal_variant = 0;
     int gen = 0;  // bumped whenever launch parameters change (mesh, config, tiles)
     bool dist_valid = false;  // wall distance computed for the current mesh / zones
     int fuse_l0 = 0;  // > 0 while a full step fuses the first lower-sweep level into the gathers
@@ -276,6 +300,14 @@ __device__ __forceinline__ void cell_faces(const DevMesh& m, int i, int* ent, in
         ent[j] = in ? m.ellf[(size_t)j * m.C + i] : 0;
         nb[j] = in ? m.elln[(size_t)j * m.C + i] : i;
     }
+}
+
+// The neighbour ids alone (the kernels that take their face terms from
+// DevMesh::ellg need no face id).
+template <int MAXF>
+__device__ __forceinline__ void cell_nbs(const DevMesh& m, int i, int* nb) {
+#pragma unroll
+    for (int j = 0; j < MAXF; ++j) nb[j] = (j < m.ellw) ? m.elln[(size_t)j * m.C + i] : i;
 }
 
 // Logical block of a launch over the cells (faces), `per` items per block.

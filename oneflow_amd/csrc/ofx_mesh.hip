@@ -162,6 +162,44 @@ __global__ void k_face_static(DevMesh m) {
     if (DIM == 3) m.fz[f] = dvec4{m.zfn[f] * ar, fz - zl, fz - zr, 0.0};
 }
 
+// The face terms of the cell kernels once per (cell, entry) (DevMesh::ellg):
+// the gradients' ones as seen from the cell, copied from fgg / frec / fz with
+// the side's selection and sign applied, and the face's unit normal and area
+// as stored.
+template <int DIM>
+__global__ void k_ell_geom(DevMesh m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const int n = m.c2fOff[i + 1] - m.c2fOff[i];
+    const size_t S = (size_t)m.ellw * m.C;
+    for (int j = 0; j < m.ellw; ++j) {
+        const size_t o = (size_t)j * m.C + i;
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // the two records, in order
+        if (j < n) {
+            const int e = m.ellf[o], f = e >= 0 ? e : ~e;
+            const bool own = e >= 0;
+            const dvec4 g = m.fgg[f], rc4 = m.frec[f];
+            const dvec4 z4 = (DIM == 3) ? m.fz[f] : dvec4{0.0, 0.0, 0.0, 0.0};
+            v[0] = own ? g.x : g.y;
+            v[1] = own ? g.y : g.x;
+            v[2] = own ? g.z : -g.z;
+            v[3] = own ? g.w : -g.w;
+            v[4] = (DIM == 3) ? (own ? z4.x : -z4.x) : 0.0;
+            v[5] = own ? rc4.x : rc4.z;
+            v[6] = own ? rc4.y : rc4.w;
+            v[7] = (DIM == 3) ? (own ? z4.y : z4.z) : 0.0;
+        }
+        m.ellg[o] = dvec4{v[0], v[1], v[2], v[3]};
+        m.ellg[S + o] = dvec4{v[4], v[5], v[6], v[7]};
+        dvec4 nrm{0.0, 0.0, 0.0, 0.0};
+        if (j < n) {
+            const int e = m.ellf[o], f = e >= 0 ? e : ~e;
+            nrm = dvec4{m.xfn[f], m.yfn[f], m.zfn[f], m.area[f]};
+        }
+        m.ellg[2 * S + o] = nrm;
+    }
+}
+
 // Viscous face-gradient geometry records (DevMesh::fv0 / fv2) from vis_geom.
 __global__ void k_face_vis_static(DevMesh m) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -652,6 +690,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
         }
     OFX_TRY(A((void**)&m.ellf, sizeof(int) * std::max<size_t>(1, ellf.size())));
     OFX_TRY(A((void**)&m.elln, sizeof(int) * std::max<size_t>(1, elln.size())));
+    OFX_TRY(A((void**)&m.ellg, sizeof(dvec4) * 3 * std::max<size_t>(1, ellf.size())));
     OFX_TRY(A((void**)&m.loOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.upOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.loOff, sizeof(int) * (nlo + 1)));
@@ -690,6 +729,11 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     if (m.dim == 2) k_face_static<2><<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
     else k_face_static<3><<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());
+    if (C) {
+        if (m.dim == 2) k_ell_geom<2><<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
+        else k_ell_geom<3><<<(C + TB - 1) / TB, TB, 0, c->stream>>>(m);
+        OFX_HIP(hipGetLastError());
+    }
     k_face_vis_static<<<(F + TB - 1) / TB, TB, 0, c->stream>>>(m);
     OFX_HIP(hipGetLastError());
     // the wall distance is computed on first use (ensure_dist): a mesh that

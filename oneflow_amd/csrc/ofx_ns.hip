@@ -132,7 +132,9 @@ int ns_boundary(ofx_ctx* c) {
 // part, codes/usolver/src/ULimiter.cpp:597-644): per primitive the largest
 // q_i - min(q_i, q_j) and max(q_i, q_j) - q_i over interior faces, into the
 // limiter's slots of parity par_lim; the residual's gradient kernel reads them.
-template <int DIM, int MAXF, bool QEXT>
+// EN: the faces' normals and areas come from the per-entry copies
+// (DevMesh::ellg plane 2).
+template <int DIM, int MAXF, bool QEXT, bool EN = false>
 __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Consts k0, int par, int par_lim) {
     int i = cell_block(m, TB) * TB + threadIdx.x;
     const int T = m.T;
@@ -157,8 +159,14 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
             if (j < n) {
                 int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
                 for (int q = 0; q < 5; ++q) qn[j][q] = (DIM == 2 && q == 3) ? 0.0 : d.q[q * T + c];
-                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
-                ar[j] = m.area[f];
+                if (EN) {
+                    const dvec4 r4 = m.ellg[(size_t)(2 * m.ellw + j) * m.C + i];
+                    nx[j] = r4.x; ny[j] = r4.y; nz[j] = (DIM == 3) ? r4.z : 0.0;
+                    ar[j] = r4.w;
+                } else {
+                    nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = (DIM == 3) ? m.zfn[f] : 0.0;
+                    ar[j] = m.area[f];
+                }
                 if (visc) { vln[j] = d.visl[c]; vtn[j] = d.vist[c]; }
             }
         }
@@ -245,8 +253,8 @@ __global__ void __launch_bounds__(TB) k_ns_qext(DevMesh m, DevFields d, int par_
     double ext[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < m.C) {
         const int n = m.c2fOff[i + 1] - m.c2fOff[i];
-        int ent[MAXF], nb[MAXF];
-        cell_faces<MAXF>(m, i, ent, nb);
+        int nb[MAXF];
+        cell_nbs<MAXF>(m, i, nb);
         double qi[5], qn[MAXF][5];
         for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
 #pragma unroll
@@ -256,8 +264,7 @@ __global__ void __launch_bounds__(TB) k_ns_qext(DevMesh m, DevFields d, int par_
         for (int e = 0; e < 5; ++e) { mn[e] = qi[e]; mx[e] = qi[e]; }
 #pragma unroll
         for (int j = 0; j < MAXF; ++j) {
-            int f = ent[j] >= 0 ? ent[j] : ~ent[j];
-            if (j < n && f >= m.B)
+            if (j < n && nb[j] < m.C)  // interior faces (a boundary face's neighbour is its ghost, C + f)
                 for (int e = 0; e < 5; ++e) { mn[e] = dmin(mn[e], qn[j][e]); mx[e] = dmax(mx[e], qn[j][e]); }
         }
         for (int e = 0; e < 5; ++e) { mn[e] -= qi[e]; mx[e] -= qi[e]; ext[e] = 0.0 - mn[e]; ext[5 + e] = mx[e]; }
@@ -319,9 +326,12 @@ int ns_timestep(ofx_ctx* c, int iter) {
     const bool qext = c->k.ilim == 2;
     if (qext) c->par_lim ^= 1;
     const int pd = c->par_dt, pl = c->par_lim;
+    const bool en = c->normal_variant == 1;
 #define OFX_TS(D, MF)                                                                                             \
-    (qext ? (k_ns_timestep<D, MF, true><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0)             \
-          : (k_ns_timestep<D, MF, false><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0))
+    (qext ? (en ? (k_ns_timestep<D, MF, true, true><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0)   \
+                : (k_ns_timestep<D, MF, true><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0))        \
+          : (en ? (k_ns_timestep<D, MF, false, true><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0)  \
+                : (k_ns_timestep<D, MF, false><<<nblk(m.C), TB, 0, c->stream>>>(m, c->d, c->k, pd, pl), 0)))
     if (m.dim == 2) {
         if (mf <= 4) OFX_TS(2, 4);
         else if (mf <= 8) OFX_TS(2, 8);
@@ -357,7 +367,10 @@ int ns_timestep(ofx_ctx* c, int iter) {
 // parity `par`.  The face loop is unrolled to MAXF: all loads of the cell's
 // faces are issued before the contributions are added in ascending face
 // order.  DIM == 2 skips the z components and w, identically zero in 2-D.
-template <int DIM, int MAXF, int LIM>
+// EG: the face terms come from the per-entry copies (DevMesh::ellg), loaded
+// with the neighbour ids rather than after the face ids; a boundary entry is
+// one whose neighbour is a ghost (id >= C).
+template <int DIM, int MAXF, int LIM, bool EG = false>
 __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k0, int par) {
     // Every lane stays active to the end (the extrema are read per wave):
     // lanes past the last cell evaluate it and store nothing.
@@ -370,25 +383,46 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
     if (LIM == 2 && m.nz == 1) wave_slot_result<10, true>(d.slots, par, m.nz * RQ_N, RQ_QMIN, ext);
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
-    cell_faces<MAXF>(m, i, ent, nb);
+    if (EG && LIM != 1) cell_nbs<MAXF>(m, i, nb);
+    else cell_faces<MAXF>(m, i, ent, nb);
     double qi[5];
     for (int e = 0; e < 5; ++e) qi[e] = (DIM == 2 && e == 3) ? 0.0 : d.q[e * T + i];
+    // EG: gw = {weight of this cell, weight of the neighbour, outward normal
+    // x A, y A}, gz = outward z A; otherwise the face's record as stored.
+    // dx, dy, dz: face centre - this cell's centre.
     dvec4 gw[MAXF];
     double gz[MAXF], qn[MAXF][5], dx[MAXF], dy[MAXF], dz[MAXF];
+    const size_t S = (size_t)m.ellw * m.C;
+    int imask = 0;  // EG: bit j set for an interior entry
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         if (j < n) {
-            int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-            gw[j] = m.fgg[f];
-            gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
-            for (int e2 = 0; e2 < 5; ++e2) qn[j][e2] = (DIM == 2 && e2 == 3) ? 0.0 : d.q[e2 * T + c];
-            if (LIM != 0) {
-                // face centre relative to this cell's centre
-                dvec4 rc4 = m.frec[f];
-                dx[j] = e >= 0 ? rc4.x : rc4.z;
-                dy[j] = e >= 0 ? rc4.y : rc4.w;
-                dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
+            const int c = nb[j];
+            if (EG) imask |= (c < m.C) << j;
+            if (EG) {
+                const size_t o = (size_t)j * m.C + i;
+                gw[j] = m.ellg[o];
+                if (DIM == 3 || LIM != 0) {
+                    const dvec4 b4 = m.ellg[S + o];
+                    gz[j] = (DIM == 3) ? b4.x : 0.0;
+                    dx[j] = b4.y;
+                    dy[j] = b4.z;
+                    dz[j] = (DIM == 3) ? b4.w : 0.0;
+                } else {
+                    gz[j] = 0.0;
+                }
+            } else {
+                const int e = ent[j], f = e >= 0 ? e : ~e;
+                gw[j] = m.fgg[f];  // as stored: {cl, cr, nx A, ny A} of the face
+                gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
+                if (LIM != 0) {
+                    dvec4 rc4 = m.frec[f];
+                    dx[j] = e >= 0 ? rc4.x : rc4.z;
+                    dy[j] = e >= 0 ? rc4.y : rc4.w;
+                    dz[j] = (DIM == 3) ? (e >= 0 ? m.fz[f].y : m.fz[f].z) : 0.0;
+                }
             }
+            for (int e2 = 0; e2 < 5; ++e2) qn[j][e2] = (DIM == 2 && e2 == 3) ? 0.0 : d.q[e2 * T + c];
         }
     }
     double mn[5], mx[5];
@@ -396,21 +430,35 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
     double ti = temperature(qi[0], qi[4], k.statecoef);
     double acc[6][3];
     for (int s = 0; s < 6; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.0;
+    // EG: with the weights and the normal taken as seen from this cell, the
+    // sums are those of the face records: for the right cell, cl*vn + cr*vi
+    // is w_self*vi + w_nb*vn (a commuted sum) and acc - n*v is acc + (-n)*v,
+    // both the same bits.
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         if (j < n) {
-            bool own = ent[j] >= 0;  // this cell is the face's left cell
-            double cl = gw[j].x, cr = gw[j].y;
-            double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
             double tn = temperature(qn[j][0], qn[j][4], k.statecoef);
-            for (int s = 0; s < 6; ++s) {
-                double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
-                double value = own ? (cl * vi + cr * vn) : (cl * vn + cr * vi);
-                if (own) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
-                else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
+            double fnxa = gw[j].z, fnya = gw[j].w, fnza = gz[j];
+            if (EG) {
+                double ws = gw[j].x, wn = gw[j].y;
+                for (int s = 0; s < 6; ++s) {
+                    double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
+                    double value = ws * vi + wn * vn;
+                    acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value;
+                }
+            } else {
+                bool own = ent[j] >= 0;  // this cell is the face's left cell
+                double cl = gw[j].x, cr = gw[j].y;
+                for (int s = 0; s < 6; ++s) {
+                    double vi = s < 5 ? qi[s] : ti, vn = s < 5 ? qn[j][s] : tn;
+                    double value = own ? (cl * vi + cr * vn) : (cl * vn + cr * vi);
+                    if (own) { acc[s][0] += fnxa * value; acc[s][1] += fnya * value; acc[s][2] += fnza * value; }
+                    else { acc[s][0] -= fnxa * value; acc[s][1] -= fnya * value; acc[s][2] -= fnza * value; }
+                }
             }
-            int f = own ? ent[j] : ~ent[j];
-            if (LIM != 0 && f >= m.B)
+            // interior faces only (a boundary face's neighbour is its ghost, id C + f)
+            const bool interior = EG ? ((imask >> j) & 1) != 0 : (ent[j] >= 0 ? ent[j] : ~ent[j]) >= m.B;
+            if (LIM != 0 && interior)
                 for (int s = 0; s < 5; ++s) { mn[s] = dmin(mn[s], qn[j][s]); mx[s] = dmax(mx[s], qn[j][s]); }
         }
     }
@@ -803,13 +851,17 @@ static void launch_cells(ofx_ctx* c, unsigned parts, bool reuse_ext) {
     bool visc = k.vismodel != 0;
     if (parts & 2u) launch_qext<DIM, MAXF>(c, true);
     if (parts & 1u) {
+        const bool eg = c->grad_variant == 1;
         if (k.ilim == 2) {
             if (!(parts & 2u)) launch_qext<DIM, MAXF>(c, !reuse_ext);
-            k_ns_grad<DIM, MAXF, 2><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            if (eg) k_ns_grad<DIM, MAXF, 2, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
+            else k_ns_grad<DIM, MAXF, 2><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, c->par_lim);
         } else if (k.ilim == 1) {
-            k_ns_grad<DIM, MAXF, 1><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
+            if (eg) k_ns_grad<DIM, MAXF, 1, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
+            else k_ns_grad<DIM, MAXF, 1><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
         } else {
-            k_ns_grad<DIM, MAXF, 0><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
+            if (eg) k_ns_grad<DIM, MAXF, 0, true><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
+            else k_ns_grad<DIM, MAXF, 0><<<nblk(C1), TB, 0, c->stream>>>(c->m, c->d, k, 0);
         }
     }
     if (parts & 4u) launch_face_dim<DIM>(c, visc);
@@ -963,8 +1015,9 @@ __device__ __forceinline__ void ns_lusgs_face(const Consts& k, double xfn, doubl
 // skipped in both sweeps: their ghost increment is zero, so the reference's
 // upper-sweep contribution through them is exactly +0.  The cell's faces are
 // unrolled to MAXF: all neighbour loads are issued first, then the face
-// terms are added in ascending face order.
-template <bool UPPER, int MAXF, int VM>
+// terms are added in ascending face order.  EN: the faces' normals and
+// areas come from the per-entry copies (DevMesh::ellg plane 2).
+template <bool UPPER, int MAXF, int VM, bool EN>
 __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields& d, const Consts& k0, int i, double mindt1,
                                               bool store = true) {
     const int T = m.T;
@@ -991,7 +1044,12 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
     for (int j = 0; j < MAXF; ++j) {
         if (act[j]) {
             int f = ent[j] >= 0 ? ent[j] : ~ent[j], r = nb[j];
-            nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = m.zfn[f]; ar[j] = m.area[f];
+            if (EN) {
+                const dvec4 r4 = m.ellg[(size_t)(2 * m.ellw + j) * m.C + i];
+                nx[j] = r4.x; ny[j] = r4.y; nz[j] = r4.z; ar[j] = r4.w;
+            } else {
+                nx[j] = m.xfn[f]; ny[j] = m.yfn[f]; nz[j] = m.zfn[f]; ar[j] = m.area[f];
+            }
             for (int e = 0; e < 5; ++e) { pj[j][e] = d.q[e * T + r]; dqj[j][e] = d.dq[e * T + r]; }
             if (visc) {
                 vl_[j] = d.visl[r]; vt_[j] = d.vist[r];
@@ -1028,7 +1086,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
 
 // Whole sweep in one workgroup: levels in order, a workgroup barrier between
 // levels (every cell of a level depends only on earlier levels).
-template <bool UPPER, int MAXF, int VM>
+template <bool UPPER, int MAXF, int VM, bool EN>
 __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, Consts k, int par) {
     double mindt = slot_result<false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN);
     const int* off = UPPER ? m.upOff : m.loOff;
@@ -1036,14 +1094,14 @@ __global__ void __launch_bounds__(1024) k_ns_sweep_1wg(DevMesh m, DevFields d, C
     int nl = UPPER ? m.nLevUp : m.nLevLo;
     for (int L = 0; L < nl; ++L) {
         for (int idx = off[L] + threadIdx.x; idx < off[L + 1]; idx += blockDim.x)
-            ns_lusgs_cell<UPPER, MAXF, VM>(m, d, k, ord[idx], mindt);
+            ns_lusgs_cell<UPPER, MAXF, VM, EN>(m, d, k, ord[idx], mindt);
         __syncthreads();
     }
 }
 
 // One level per launch (wide levels on large meshes).
 // base >= 0: the level's cells are base, base + 1, ... (no order lookup).
-template <bool UPPER, int MAXF, int VM>
+template <bool UPPER, int MAXF, int VM, bool EN>
 __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, Consts k, int b, int e, int base,
                                                        int par) {
     // Every lane stays active (the minimum time step is reduced per wave, its
@@ -1055,7 +1113,7 @@ __global__ void __launch_bounds__(TB) k_ns_sweep_level(DevMesh m, DevFields d, C
     double mindt;
     wave_slot_result<1, false>(d.slots, par, m.nz * RQ_N, RQ_DTMIN, &mindt);
     int i = base >= 0 ? base + (idx - b) : (UPPER ? m.upOrder : m.loOrder)[idx];
-    ns_lusgs_cell<UPPER, MAXF, VM>(m, d, k, i, mindt, live);
+    ns_lusgs_cell<UPPER, MAXF, VM, EN>(m, d, k, i, mindt, live);
 }
 
 // Launch policy: few wide levels (multicolour numbering) -> one grid-wide
@@ -1066,15 +1124,15 @@ bool use_level_launches(const std::vector<int>& off) {
     return nl <= 64;
 }
 
-template <int MAXF, int VM>
+template <int MAXF, int VM, bool EN>
 static void launch_lusgs(ofx_ctx* c, int lo_first) {
     const DevMesh& m = c->m;
     for (int pass = 0; pass < 2; ++pass) {
         bool up = pass == 1;
         const std::vector<int>& off = up ? c->sched.upOff : c->sched.loOff;
         if (!use_level_launches(off)) {
-            if (up) k_ns_sweep_1wg<true, MAXF, VM><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
-            else k_ns_sweep_1wg<false, MAXF, VM><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            if (up) k_ns_sweep_1wg<true, MAXF, VM, EN><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
+            else k_ns_sweep_1wg<false, MAXF, VM, EN><<<1, 1024, 0, c->stream>>>(m, c->d, c->k, c->par_dt);
         } else {
             const std::vector<int>& base = up ? c->sched.upBase : c->sched.loBase;
             // upper level 0 holds the cells without a higher neighbour: their
@@ -1083,21 +1141,26 @@ static void launch_lusgs(ofx_ctx* c, int lo_first) {
                 int b = off[L], e = off[L + 1];
                 if (e <= b) continue;
                 if (up)
-                    k_ns_sweep_level<true, MAXF, VM><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L],
+                    k_ns_sweep_level<true, MAXF, VM, EN><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L],
                                                                                        c->par_dt);
                 else
-                    k_ns_sweep_level<false, MAXF, VM><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L],
+                    k_ns_sweep_level<false, MAXF, VM, EN><<<nblk(e - b), TB, 0, c->stream>>>(m, c->d, c->k, b, e, base[L],
                                                                                         c->par_dt);
             }
         }
     }
 }
 
+template <int MAXF, bool EN>
+static void launch_lusgs_vm(ofx_ctx* c, int lo_first) {
+    if (c->k.vismodel == 0) launch_lusgs<MAXF, 0, EN>(c, lo_first);
+    else if (c->k.visSRModel == 1) launch_lusgs<MAXF, 1, EN>(c, lo_first);
+    else launch_lusgs<MAXF, 2, EN>(c, lo_first);
+}
 template <int MAXF>
 static void launch_lusgs_vm(ofx_ctx* c, int lo_first) {
-    if (c->k.vismodel == 0) launch_lusgs<MAXF, 0>(c, lo_first);
-    else if (c->k.visSRModel == 1) launch_lusgs<MAXF, 1>(c, lo_first);
-    else launch_lusgs<MAXF, 2>(c, lo_first);
+    if (c->normal_variant == 1) launch_lusgs_vm<MAXF, true>(c, lo_first);
+    else launch_lusgs_vm<MAXF, false>(c, lo_first);
 }
 
 // The first level of the lower sweep that may be fused into the residual

@@ -78,7 +78,10 @@ int sa_boundary(ofx_ctx* c) {
 // with the Green-Gauss gradient of nu_tilde (UTurbGrad,
 // codes/uturb/src/UTurbGrad.cpp:51-74): each cell sets the ghosts of its own
 // boundary faces (they depend only on the owner) and uses them.
-template <int DIM, int MAXF>
+// EG: the face terms come from the per-entry copies (DevMesh::ellg, as seen
+// from the cell; the same sums bit for bit, see k_ns_grad), a boundary
+// entry being one whose neighbour is a ghost (id C + f).
+template <int DIM, int MAXF, bool EG = false>
 __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts k0) {
     int i = cell_block(m, TBS) * TBS + threadIdx.x;
     if (i >= m.C) return;
@@ -86,24 +89,35 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
     const int T = m.T;
     const int n = m.c2fOff[i + 1] - m.c2fOff[i];
     int ent[MAXF], nb[MAXF];
-    cell_faces<MAXF>(m, i, ent, nb);
+    if (EG) cell_nbs<MAXF>(m, i, nb);
+    else cell_faces<MAXF>(m, i, ent, nb);
     double nui = d.turbq[i];
-    // All entries are loaded (padding: face 0, the cell itself) and masked in
-    // the sums, so that no load sits in a divergent branch; only the ghosts
-    // of the cell's own boundary faces are evaluated and written.
+    // All entries are loaded (padding: face 0 or the cell's entry 0, the
+    // cell itself) and masked in the sums, so that no load sits in a
+    // divergent branch; only the ghosts of the cell's own boundary faces are
+    // evaluated and written.
     dvec4 gw[MAXF];
     double gz[MAXF], vn[MAXF];
+    const size_t S = (size_t)m.ellw * m.C;
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        int e = ent[j], f = e >= 0 ? e : ~e, c = nb[j];
-        gw[j] = m.fgg[f];
-        gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
+        const int c = nb[j];
+        if (EG) {
+            // a padding entry reads entry 0 again (cached), not its zeros
+            const size_t o = (size_t)(j < n ? j : 0) * m.C + i;
+            gw[j] = m.ellg[o];
+            gz[j] = (DIM == 3) ? m.ellg[S + o].x : 0.0;
+        } else {
+            int e = ent[j], f = e >= 0 ? e : ~e;
+            gw[j] = m.fgg[f];
+            gz[j] = (DIM == 3) ? m.fz[f].x : 0.0;
+        }
         vn[j] = d.turbq[c];
     }
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
-        int e = ent[j], f = e >= 0 ? e : ~e;
-        if (j < n && f < m.B) {
+        const int f = EG ? nb[j] - m.C : (ent[j] >= 0 ? ent[j] : ~ent[j]);
+        if (j < n && (EG ? nb[j] >= m.C : f < m.B)) {
             vn[j] = sa_bc_value(m, d, k, f, i, nui);
             d.turbq[nb[j]] = vn[j];
         }
@@ -112,12 +126,18 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         const bool lv = j < n;
-        bool own = ent[j] >= 0;
-        double cl = gw[j].x, cr = gw[j].y;
-        double value = own ? (cl * nui + cr * vn[j]) : (cl * vn[j] + cr * nui);
-        double tx = gw[j].z * value, ty = gw[j].w * value, tz = gz[j] * value;
-        if (own) { ax = ax + live_or_nz(lv, tx); ay = ay + live_or_nz(lv, ty); az = az + live_or_nz(lv, tz); }
-        else { ax = ax - live_or_pz(lv, tx); ay = ay - live_or_pz(lv, ty); az = az - live_or_pz(lv, tz); }
+        if (EG) {
+            double value = gw[j].x * nui + gw[j].y * vn[j];
+            double tx = gw[j].z * value, ty = gw[j].w * value, tz = gz[j] * value;
+            ax = ax + live_or_nz(lv, tx); ay = ay + live_or_nz(lv, ty); az = az + live_or_nz(lv, tz);
+        } else {
+            bool own = ent[j] >= 0;
+            double cl = gw[j].x, cr = gw[j].y;
+            double value = own ? (cl * nui + cr * vn[j]) : (cl * vn[j] + cr * nui);
+            double tx = gw[j].z * value, ty = gw[j].w * value, tz = gz[j] * value;
+            if (own) { ax = ax + live_or_nz(lv, tx); ay = ay + live_or_nz(lv, ty); az = az + live_or_nz(lv, tz); }
+            else { ax = ax - live_or_pz(lv, tx); ay = ay - live_or_pz(lv, ty); az = az - live_or_pz(lv, tz); }
+        }
     }
     double ovol = 1.0 / m.vol[i];
     d.tgrad[i] = ax * ovol;
@@ -389,7 +409,10 @@ __global__ void __launch_bounds__(TBS) k_sa_cell(DevMesh m, DevFields d, Consts 
 template <int DIM, int MAXF>
 static void launch_sa(ofx_ctx* c, unsigned parts) {
     const DevMesh& m = c->m;
-    if (parts & 1u) k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+    if (parts & 1u) {
+        if (c->sagrad_variant == 1) k_sa_grad<DIM, MAXF, true><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+        else k_sa_grad<DIM, MAXF><<<nblks(m.C), TBS, 0, c->stream>>>(m, c->d, c->k);
+    }
     if (parts & 2u) {
         const int nb = (m.F + c->saface_tb - 1) / c->saface_tb;
         if (c->sa_geom_variant == 1 && c->k.iturb_visflux == 2)

@@ -218,7 +218,8 @@ class Solver:
         return a
 
     KERNELS = {"grad": 0, "limiter": 1, "face": 2, "gather": 3, "ns_residual": 4, "sa_residual": 5,
-               "sa_grad": 6, "sa_face": 7, "sa_cell": 8, "residual": 9, "pseudo_step": 10}
+               "sa_grad": 6, "sa_face": 7, "sa_cell": 8, "residual": 9, "pseudo_step": 10,
+               "timestep": 11, "lusgs_ns": 12}
 
     def wall_data(self):               # DUMP_PRESSURE_COEFF
         """[n_wall][5]: x, y, z of each wall face centre, -cp, cf (face order)."""
@@ -245,8 +246,9 @@ class Solver:
         return int(v[0])
 
     def set_kernel_variant(self, kernel, variant):
-        """kernel 2 gather, 3 NS face split: variant 0 or 1 (results identical); kernels 0 and 1
-        (the fused gradient and limiter) have the single form 0."""
+        """Launch variant 0 or 1 of kernel 0 or 1 (the NS gradient with limiter), 2 (gather),
+        3 (NS face split), 4 (SA face geometry), 5 (SA gradient) or 6 (NS time step and
+        LU-SGS sweeps); results are identical."""
         self._check(self._L.ofx_set_kernel_variant(self.h, kernel, variant))
 
     def autotune(self, iters=20):

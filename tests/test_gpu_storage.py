@@ -4,7 +4,10 @@ part in Morton order, interior faces by their cells) against the grid's own
 order, after whole pseudo-steps, on a multicolour-numbered 2-D airfoil with SA,
 the small 3-D swept wing, natural numbering (one-workgroup sweeps) and zones.
 States, eddy viscosity, limiter, time step, forces and wall rows must agree
-bit for bit; norms (sums over cells in storage order) to 1e-13."""
+bit for bit; norms (sums over cells in storage order) to 1e-13.  The spatial
+runs also read the face terms of the gradients, time step and sweeps from
+the per-(cell, face) copies (kernel variants 0, 5 and 6), which are built in
+storage order."""
 import numpy as np
 import pytest
 
@@ -48,6 +51,8 @@ def test_storage_order_changes_no_result(which):
     g, cfg = wing() if which == "wing" else airfoil(which)
     print(f"[storage] {which}: cells={g.n_cells}", flush=True)
     a, b = Solver(g, cfg, storage_order=0), Solver(g, cfg, storage_order=1)
+    for kernel in (0, 5, 6):
+        b.set_kernel_variant(kernel, 1)
     for s in (a, b):
         s.init_freestream()
     na, nb = a.iterate(1, 15), b.iterate(1, 15)
@@ -63,6 +68,8 @@ def test_storage_order_zones():
     c2 = c2.replace(mach_ref=0.5, aoa_degree=3.0)
     a = ZoneSolver([g1, g2], [c1, c2], storage_order=0)
     b = ZoneSolver([g1, g2], [c1, c2], storage_order=1)
+    for kernel in (0, 5, 6):
+        b.set_kernel_variant(kernel, 1)
     for s in (a, b):
         s.init_freestream()
         s.iterate(1, 12, with_norms=False)

@@ -3,7 +3,10 @@
 The residual gather has two kernels (one lane per face; one thread per
 cell), the viscous face kernel a split form (inviscid and viscous parts on
 separate waves), the SA face kernel reads or evaluates the TEST method's
-face geometry, and the face kernels take any face tile.  Every combination must give the
+face geometry, the NS and SA gradients, the NS time step and the LU-SGS
+sweeps read their face terms by face id or from the per-(cell, face) copies,
+and the face kernels take any face tile.
+Every combination must give the
 same state, gradients, limiter and residual bit for bit, here after twelve
 pseudo-time steps from freestream (through the CFL ramp) on 2-D and 3-D grids
 and several option sets.  Variant 0 with the default tile is checked against
@@ -25,12 +28,14 @@ def test_variants_and_tiles_bitwise(grid, opt):
     from oneflow_amd.solver import Solver
     base, _, g = make(grid, opt)
     runs = [base]
-    # (gather, face split, SA face geometry) variants and face tiles
-    for variants, tiles in (((0, 1, 1), (256, 256)), ((1, 0, 1), (64, 128)), ((0, 1, 0), (512, 64)),
-                            ((1, 0, 1), (128, 512))):
+    # (gather, face split, SA face geometry, NS gradient, SA gradient, time
+    # step and sweeps) variants and face tiles
+    for variants, tiles in (((0, 1, 1, 1, 1, 1), (256, 256)), ((1, 0, 1, 1, 0, 0), (64, 128)),
+                            ((0, 1, 0, 0, 1, 1), (512, 64)), ((1, 0, 1, 1, 1, 1), (128, 512))):
         s = Solver(g, base.cfg)
-        for kernel, v in zip((2, 3, 4), variants):
+        for kernel, v in zip((2, 3, 4, 0, 5, 6), variants):
             s.set_kernel_variant(kernel, v)
+        assert s.kernel_variant(1) == variants[3]  # kernels 0 and 1 name the same kernel
         s.set_face_tile(*tiles)
         runs.append(s)
     for s in runs:
@@ -45,11 +50,14 @@ def test_variants_and_tiles_bitwise(grid, opt):
 
 
 
-def test_gradient_and_limiter_have_one_form():
+def test_variant_ids_checked():
     from oneflow_amd.solver import OfxError
     base, _, g = make("quad", "roe_venkat_sa_test")
-    base.set_kernel_variant(0, 0)
+    for kernel in range(7):
+        base.set_kernel_variant(kernel, 0)
     with pytest.raises(OfxError):
-        base.set_kernel_variant(0, 1)
+        base.set_kernel_variant(7, 0)
     with pytest.raises(OfxError):
-        base.set_kernel_variant(1, 1)
+        base.set_kernel_variant(0, 2)
+    with pytest.raises(OfxError):
+        base.kernel_variant(7)
