@@ -629,6 +629,10 @@ def main():
     ap.add_argument("--storage", type=int, default=None, choices=[0, 1],
                     help="device storage order: 0 the grid's (library default), 1 spatial")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-kernel bandwidth reference")
+    ap.add_argument("--launch", default="",
+                    help="c1/c2: fixed launch configuration instead of the autotuner, "
+                         "ns_tile,sa_tile,gather,face_split,sa_geometry,grad,sa_grad,normals (the order of "
+                         "the bench line's launch record)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "batch", "warmstart", "mlp"],
                     help="c2: the 3-D swept wing (default, the largest single-GPU configuration); "
                          "c1: the 2-D NACA0012 grid; batch: the 64-case sweep; "
@@ -826,7 +830,20 @@ def residual_bench(s, grid, args, flux_bytes=8):
             s.set_kernel_variant(kernel, v)
             t[v] = 1e3 * s.time_kernel(part, iters=args.kernel_iters)
         variant_us[name] = t
-    tiles = s.autotune(iters=20)
+    if args.launch:
+        # a fixed launch configuration instead of the autotuner's choice (a
+        # profiled run then times what an unprofiled run chose: under a
+        # kernel-trace profiler every launch of a few microseconds takes about
+        # the same time, and the autotuner's choice among them is arbitrary)
+        v = [int(x) for x in args.launch.split(",")]
+        if len(v) != 8:
+            raise SystemExit("--launch takes ns_tile,sa_tile,gather,face_split,sa_geometry,grad,sa_grad,normals")
+        tiles = (v[0], v[1])
+        s.set_face_tile(*tiles)
+        for kernel, var in zip((2, 3, 4, 0, 5, 6), v[2:]):
+            s.set_kernel_variant(kernel, var)
+    else:
+        tiles = s.autotune(iters=20)
     T, F = s.n_tcell, s.n_faces
     for _ in range(args.warmup):
         s.residual_ns()
@@ -886,9 +903,10 @@ def residual_bench(s, grid, args, flux_bytes=8):
               "cell_steps_per_s": grid.n_cells / (ps_ms * 1e-3)}
     return {"value": grid.n_cells * args.steps / sec, "ms_per_step": ms_per_step, "roofline": roofline,
             "kernels": kernels, "dominant_kernel": dominant, "pseudo_step": pseudo, "copy_kernel_gbs": copy_gbs,
-            "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "face_split": s.kernel_variant(3),
-                       "sa_geometry": s.kernel_variant(4), "grad_entry_terms": s.kernel_variant(0),
-                       "sa_grad_entry_terms": s.kernel_variant(5), "step_entry_normals": s.kernel_variant(6)}}
+            "launch": {"variant_us": variant_us, "face_tiles": list(tiles), "gather": s.kernel_variant(2),
+                       "face_split": s.kernel_variant(3), "sa_geometry": s.kernel_variant(4),
+                       "grad_entry_terms": s.kernel_variant(0), "sa_grad_entry_terms": s.kernel_variant(5),
+                       "step_entry_normals": s.kernel_variant(6), "autotuned": not args.launch}}
 
 
 def oracle_error(grid, cfg, fp32=True):
