@@ -297,8 +297,9 @@ int ofx_set_face_tile(ofx_ctx* ctx, int ns_tile, int sa_tile);
  * variant 1 reads the per-(cell, face) copies stored cell-major at mesh
  * upload (64 bytes per cell face), contiguous per wave.  kernel 6, the NS
  * time step and LU-SGS sweeps: variant 0 (default) reads the faces' normals
- * and areas by face id, variant 1 per-(cell, face) copies of them (32 bytes
- * per cell face).  kernel 2, the residual
+ * and areas by face id and evaluates the viscous radius's centre distance,
+ * variant 1 reads per-(cell, face) copies of the normal, area, distance and
+ * neighbour volume (48 bytes per cell face).  kernel 2, the residual
  * gather: variant 0 one lane per face of a cell, variant 1 (default) one
  * thread per cell.  kernel 3, the viscous FP64 NS face kernel: variant 0
  * (default) one thread per face, variant 1 two waves per 64 faces (inviscid

@@ -197,6 +197,16 @@ __global__ void k_ell_geom(DevMesh m) {
             nrm = dvec4{m.xfn[f], m.yfn[f], m.zfn[f], m.area[f]};
         }
         m.ellg[2 * S + o] = nrm;
+        double dist = 0.0, vnb = 0.0;
+        if (j < n) {
+            const int e = m.ellf[o], c = m.elln[o];
+            const bool own = e >= 0;
+            const double xo = own ? nrm.x : -nrm.x, yo = own ? nrm.y : -nrm.y, zo = own ? nrm.z : -nrm.z;
+            dist = dabs(xo * (m.xcc[c] - m.xcc[i]) + yo * (m.ycc[c] - m.ycc[i]) + zo * (m.zcc[c] - m.zcc[i]));
+            vnb = m.vol[c];
+        }
+        m.elldv[o] = dist;
+        m.elldv[S + o] = vnb;
     }
 }
 
@@ -691,6 +701,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(A((void**)&m.ellf, sizeof(int) * std::max<size_t>(1, ellf.size())));
     OFX_TRY(A((void**)&m.elln, sizeof(int) * std::max<size_t>(1, elln.size())));
     OFX_TRY(A((void**)&m.ellg, sizeof(dvec4) * 3 * std::max<size_t>(1, ellf.size())));
+    OFX_TRY(A((void**)&m.elldv, sizeof(double) * 2 * std::max<size_t>(1, ellf.size())));
     OFX_TRY(A((void**)&m.loOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.upOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.loOff, sizeof(int) * (nlo + 1)));

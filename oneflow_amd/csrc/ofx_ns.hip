@@ -209,9 +209,14 @@ __global__ void __launch_bounds__(TB) k_ns_timestep(DevMesh m, DevFields d, Cons
                         double c3 = 2.0 * dmax(c1, c2) / (k.reynolds * density);
                         sr = (area * area) * c3;
                     } else if (k.visSRModel == 2) {
-                        int l = own ? i : nb[j], r = own ? nb[j] : i;
-                        double zl = (DIM == 3) ? m.zcc[l] : 0.0, zr = (DIM == 3) ? m.zcc[r] : 0.0;
-                        double dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) + zfn * (zr - zl));
+                        double dist;
+                        if (EN) {
+                            dist = m.elldv[(size_t)j * m.C + i];  // the same bits (DevMesh::elldv)
+                        } else {
+                            int l = own ? i : nb[j], r = own ? nb[j] : i;
+                            double zl = (DIM == 3) ? m.zcc[l] : 0.0, zr = (DIM == 3) ? m.zcc[r] : 0.0;
+                            dist = dabs(xfn * (m.xcc[r] - m.xcc[l]) + yfn * (m.ycc[r] - m.ycc[l]) + zfn * (zr - zl));
+                        }
                         double viscosity = visl + vist;
                         double density = 0.5 * (rl + rr);
                         double c1 = 2.0 * viscosity / (density * dist * k.reynolds + SMALL);
@@ -1037,9 +1042,11 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
     }
     double rl = d.q[i], ul = d.q[T + i], vl = d.q[2 * T + i], wl = d.q[3 * T + i], pl = d.q[4 * T + i];
     double xi = 0.0, yi = 0.0, zi = 0.0;
-    if (visc && !sr1) { xi = m.xcc[i]; yi = m.ycc[i]; zi = m.zcc[i]; }
+    if (visc && !sr1 && !EN) { xi = m.xcc[i]; yi = m.ycc[i]; zi = m.zcc[i]; }
     double nx[MAXF], ny[MAXF], nz[MAXF], ar[MAXF], pj[MAXF][5], dqj[MAXF][5], vl_[MAXF], vt_[MAXF];
-    double rx[MAXF], ry[MAXF], rz[MAXF];  // SR model 1: rx = vol of r; otherwise the centroid of r
+    // SR model 1: rx = vol of r; otherwise the centroid of r (EN: rx = the
+    // per-entry distance along the normal)
+    double rx[MAXF], ry[MAXF], rz[MAXF];
 #pragma unroll
     for (int j = 0; j < MAXF; ++j) {
         if (act[j]) {
@@ -1053,7 +1060,8 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
             for (int e = 0; e < 5; ++e) { pj[j][e] = d.q[e * T + r]; dqj[j][e] = d.dq[e * T + r]; }
             if (visc) {
                 vl_[j] = d.visl[r]; vt_[j] = d.vist[r];
-                if (sr1) rx[j] = m.vol[r];
+                if (EN) rx[j] = m.elldv[(size_t)((sr1 ? m.ellw : 0) + j) * m.C + i];  // volume / distance
+                else if (sr1) rx[j] = m.vol[r];
                 else { rx[j] = m.xcc[r]; ry[j] = m.ycc[r]; rz[j] = m.zcc[r]; }
             }
         }
@@ -1069,6 +1077,7 @@ __device__ __forceinline__ void ns_lusgs_cell(const DevMesh& m, const DevFields&
             double volr = 0.0, dist = 0.0;
             if (visc) {
                 if (sr1) volr = rx[j];
+                else if (EN) dist = rx[j];
                 else dist = dabs(xfn * (rx[j] - xi) + yfn * (ry[j] - yi) + zfn * (rz[j] - zi));
             }
             ns_lusgs_face<VM>(k, xfn, yfn, zfn, vfn, ar[j], rl, ul, vl, wl, pl, pj[j], dqj[j], vl_[j], vt_[j], volr,

@@ -22,8 +22,15 @@ pytestmark = pytest.mark.gpu
 FIELDS = ["Q", "TURBQ", "DQDX", "LIMITER", "RES", "TURBRES", "TIMESTEP", "DQ"]
 
 
-@pytest.mark.parametrize("grid", ["quad", "tri", "mixed", "wing"])
-@pytest.mark.parametrize("opt", ["roe_venkat_sa_test", "roe_barth_lam_std", "slau2_o1_lam_aver_iso"])
+# Spectral-radius model 1 (the neighbour's volume in the sweeps) on the 2-D
+# grids: unlimited MUSCL with it diverges on the small wing within a few
+# steps (the oracle's run as well), so there is no state to compare.
+CASES = [(g, o) for g in ("quad", "tri", "mixed", "wing")
+         for o in ("roe_venkat_sa_test", "roe_barth_lam_std", "slau2_o1_lam_aver_iso", "roe_none_sa_aver_iso_sr1")
+         if not (g == "wing" and o == "roe_none_sa_aver_iso_sr1")]
+
+
+@pytest.mark.parametrize("grid,opt", CASES)
 def test_variants_and_tiles_bitwise(grid, opt):
     from oneflow_amd.solver import Solver
     base, _, g = make(grid, opt)
