@@ -316,6 +316,15 @@ __device__ __forceinline__ void cell_nbs(const DevMesh& m, int i, int* nb) {
     for (int j = 0; j < MAXF; ++j) nb[j] = (j < m.ellw) ? m.elln[(size_t)j * m.C + i] : i;
 }
 
+// Whether cell i has a boundary face.  A cell's faces are listed in
+// ascending face id and boundary faces (ids below B, whose left cell the
+// cell is) come first, so its first entry tells: one contiguous load that
+// does not depend on the cell's list offsets.
+__device__ __forceinline__ bool has_bface(const DevMesh& m, int i) {
+    const int e = m.ellf[i];
+    return e >= 0 && e < m.B;
+}
+
 // Logical block of a launch over the cells (faces), `per` items per block.
 __device__ __forceinline__ int cell_block(const DevMesh& m, int per) {
     return xcd_remap_seg((int)blockIdx.x, (int)gridDim.x, per, m.segc, m.nsegc);

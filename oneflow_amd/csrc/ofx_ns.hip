@@ -1260,6 +1260,7 @@ __device__ __forceinline__ void ns_cell_boundary(const DevMesh& m, const DevFiel
     double t = temperature(prim[0], prim[4], kz.statecoef);
     d.tempr[i] = t;
     d.visl[i] = sutherland(t, kz.ts_suth);
+    if (!has_bface(m, i)) return;
     for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
         int f = m.c2f[kk];
         if (f < 0 || f >= m.B) break;

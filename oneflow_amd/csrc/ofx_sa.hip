@@ -555,6 +555,7 @@ __global__ void __launch_bounds__(TBS) k_sa_update_fused(DevMesh m, DevFields d,
     d.turbq[i] = qNew;
     double vist = sa_vist_cell(d, k, m.T, i);
     d.vist[i] = vist;
+    if (!has_bface(m, i)) return;
     for (int kk = m.c2fOff[i]; kk < m.c2fOff[i + 1]; ++kk) {
         int f = m.c2f[kk];
         if (f < 0 || f >= m.B) break;  // boundary faces first; right-cell entries are ~f < 0
