@@ -356,6 +356,12 @@ int ofx_mlp_get_params(ofx_mlp* net, float* host);
 int ofx_mlp_set_params(ofx_mlp* net, const float* host);
 /* y[n][5] = net(x[n][9]), n <= 2^20 per call. */
 int ofx_mlp_forward(ofx_mlp* net, const float* x, int n, float* y);
+/* How ofx_mlp_forward and ofx_mlp_forward_noalloc evaluate the network (no
+ * counterpart in the reference; results do not depend on it): variant 1
+ * (default) all layers in one kernel, 128 samples per workgroup with the
+ * hidden activations kept in LDS; variant 0 one GEMM kernel per layer, the
+ * activations through device memory (the training pass's form). */
+int ofx_mlp_set_forward_variant(ofx_mlp* net, int variant);
 /* ofx_mlp_forward without any allocation.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, serving loops).  x_dev [n][9] and y_dev [n][5] are DEVICE pointers

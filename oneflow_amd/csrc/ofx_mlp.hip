@@ -141,6 +141,104 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int ld
             }
 }
 
+// The network's weights and biases as the one-kernel forward pass reads them.
+struct MlpFwd {
+    const bf16* wt[NLAYER];  // transposed bf16 weights [kout][kin]
+    const float* b[NLAYER];
+};
+
+// The whole forward pass of 128 samples in one workgroup: the activations of
+// every hidden layer stay in LDS (128 x 256 bf16, 64 KiB), so HBM sees only the
+// packed inputs, the weights (from L2) and the outputs.  Eight waves; wave w
+// computes rows 64(w/4)..+63 and columns 64(w%4)..+63 of a hidden layer (4 x 4
+// tiles of v_mfma_f32_16x16x32_bf16, operands as in k_gemm: A from LDS, Bt
+// straight from the weights), then all waves write their tiles back over the
+// layer's input once every wave has read it.  The output layer needs its first
+// 16 (of 64 padded) columns only: wave w computes rows 16w..16w+15 of them.
+// Every product, sum order and epilogue is k_gemm's (the MFMA chain over k in
+// ascending 32-wide steps from zero, the 16-wide input layer through one MFMA
+// with k 16..31 zero), so the outputs are those of the layer-by-layer pass
+// bit for bit.  LDS rows are 512 bytes with no padding; the 16-byte vector v
+// of row r is stored at v ^ (r & 15), so the 16 rows an MFMA operand load
+// reads fall in different banks.  Rows at or past Mp (the last workgroup when
+// Mp is not a multiple of 128) read zero inputs and are not stored.
+__device__ __forceinline__ int swz(int r, int k) { return (((k >> 3) ^ (r & 15)) << 3) | (k & 7); }
+
+__global__ void __launch_bounds__(512) k_mlp_forward(const bf16* __restrict__ x16, int Mp, MlpFwd P,
+                                                     float* __restrict__ y) {
+    constexpr int BM = 128;
+    __shared__ __align__(16) bf16 sH[BM][NHID];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.x * BM;
+    const int r0 = 64 * (w >> 2), c0 = 64 * (w & 3);  // this wave's tile of a hidden layer
+    const int lr = lane & 15, lk = 8 * (lane >> 4);    // operand row within a 16-tile; k offset of the lane's 8
+    const v8bf zero8 = {};
+    v4f acc[4][4];
+    // bias + SiLU of this wave's tile into the activation rows
+    auto store_silu = [&](const float* bias) {
+        for (int j = 0; j < 4; ++j) {
+            const int col = c0 + 16 * j + lr;
+            const float bc = bias[col];
+            for (int i = 0; i < 4; ++i)
+                for (int r = 0; r < 4; ++r) {
+                    const int row = r0 + 16 * i + 4 * (lane >> 4) + r;
+                    sH[row][swz(row, col)] = f2bf(silu(acc[i][j][r] + bc));
+                }
+        }
+    };
+    // layer 0: k = 16 packed inputs, k 16..31 of the MFMA zero
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    {
+        v8bf b[4];
+        for (int j = 0; j < 4; ++j)
+            b[j] = lk < NIN_P ? *(const v8bf*)&P.wt[0][(size_t)(c0 + 16 * j + lr) * NIN_P + lk] : zero8;
+        for (int i = 0; i < 4; ++i) {
+            const int row = m0 + r0 + 16 * i + lr;
+            const v8bf a = (lk < NIN_P && row < Mp) ? *(const v8bf*)&x16[(size_t)row * NIN_P + lk] : zero8;
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    store_silu(P.b[0]);
+    __syncthreads();
+    // hidden layers 1..4: 256 -> 256
+    for (int l = 1; l < NLAYER - 1; ++l) {
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        const bf16* wt = P.wt[l];
+        for (int ks = 0; ks < NHID; ks += 32) {
+            v8bf a[4], b[4];
+            for (int j = 0; j < 4; ++j) b[j] = *(const v8bf*)&wt[(size_t)(c0 + 16 * j + lr) * NHID + ks + lk];
+            for (int i = 0; i < 4; ++i) {
+                const int row = r0 + 16 * i + lr;
+                a[i] = *(const v8bf*)&sH[row][swz(row, ks + lk)];
+            }
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();  // every wave has read this layer's input
+        store_silu(P.b[l]);
+        __syncthreads();
+    }
+    // output layer: columns 0..15 of the padded 64, rows 16w..16w+15
+    {
+        const int L = NLAYER - 1;
+        v4f o = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < NHID; ks += 32) {
+            const v8bf b = *(const v8bf*)&P.wt[L][(size_t)lr * NHID + ks + lk];
+            const int row = 16 * w + lr;
+            const v8bf a = *(const v8bf*)&sH[row][swz(row, ks + lk)];
+            o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, o, 0, 0, 0);
+        }
+        const float bc = P.b[L][lr];
+        for (int r = 0; r < 4; ++r) {
+            const int row = m0 + 16 * w + 4 * (lane >> 4) + r;
+            if (row < Mp) y[(size_t)row * NOUT_P + lr] = o[r] + bc;
+        }
+    }
+}
+
 // out[c][r] = in[r][c] for an R x Cc bf16 matrix; rows R..Rp-1 of the output's
 // transposed shape (columns of `in` beyond Cc) are zero-filled up to Cp.
 __global__ void k_transpose(const bf16* in, int R, int Cc, bf16* out, int Cp) {
@@ -308,6 +406,7 @@ struct ofx_mlp {
     float *dyf = nullptr, *gz = nullptr;  // fp32 gradient of the current layer's pre-activation
     bf16 *gzb = nullptr, *gzb2 = nullptr;
     float *part = nullptr, *lpart = nullptr, *loss = nullptr, *cpart = nullptr;
+    int fwd_variant = 1;  // inference forward pass: 0 layer by layer, 1 one kernel (k_mlp_forward)
     std::vector<void*> allocs;
 };
 
@@ -424,8 +523,19 @@ int colsum(ofx_mlp* n, const float* g, int M, int N, float* out) {
     return 0;
 }
 
+// All layers in one kernel (k_mlp_forward): packed inputs x16 [Mp][16] ->
+// padded outputs y [Mp][64] (columns 0..15 written).
+int forward_fused(const ofx_mlp* n, hipStream_t st, const bf16* x16, int Mp, float* y) {
+    MlpFwd P;
+    for (int l = 0; l < NLAYER; ++l) { P.wt[l] = n->wt[l]; P.b[l] = n->b[l]; }
+    k_mlp_forward<<<(Mp + 127) / 128, 512, 0, st>>>(x16, Mp, P, y);
+    OFX_HIP(hipGetLastError());
+    return 0;
+}
+
 // keep_z: store the pre-activations (a backward pass follows).
 int forward(ofx_mlp* n, int Mp, bool keep_z) {
+    if (!keep_z && n->fwd_variant == 1) return forward_fused(n, n->stream, n->h[0], Mp, n->y);
     for (int l = 0; l < NLAYER - 1; ++l)
         OFX_TRY(gemm<EPI_BIAS_SILU>(n, n->h[l], n->kin[l], n->wt[l], n->kin[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
                                     n->h[l + 1], n->kout[l], n->b[l], nullptr, keep_z ? n->z[l] : nullptr));
@@ -669,18 +779,28 @@ int ofx_mlp_forward_noalloc(ofx_mlp* n, const float* x_dev, int nsamp, float* y_
     const int Mp = (nsamp + 63) / 64 * 64;
     k_pack_inputs<<<(Mp * NIN_P + 255) / 256, 256, 0, st>>>(x_dev, nsamp, Mp, in);
     OFX_HIP(hipGetLastError());
-    const bf16* h = in;
-    for (int l = 0; l < NLAYER - 1; ++l) {
-        OFX_TRY(gemm_on<EPI_BIAS_SILU>(st, h, n->kin[l], n->wt[l], n->kin[l], Mp, n->kout[l], n->kin[l], 1, nullptr,
-                                       act[l & 1], n->kout[l], n->b[l], nullptr, nullptr));
-        h = act[l & 1];
+    if (n->fwd_variant == 1) {
+        OFX_TRY(forward_fused(n, st, in, Mp, yp));
+    } else {
+        const bf16* h = in;
+        for (int l = 0; l < NLAYER - 1; ++l) {
+            OFX_TRY(gemm_on<EPI_BIAS_SILU>(st, h, n->kin[l], n->wt[l], n->kin[l], Mp, n->kout[l], n->kin[l], 1,
+                                           nullptr, act[l & 1], n->kout[l], n->b[l], nullptr, nullptr));
+            h = act[l & 1];
+        }
+        const int L = NLAYER - 1;
+        OFX_TRY(gemm_on<EPI_BIAS>(st, h, n->kin[L], n->wt[L], n->kin[L], Mp, n->kout[L], n->kin[L], 1, yp, nullptr,
+                                  n->kout[L], n->b[L], nullptr, nullptr));
     }
-    const int L = NLAYER - 1;
-    OFX_TRY(gemm_on<EPI_BIAS>(st, h, n->kin[L], n->wt[L], n->kin[L], Mp, n->kout[L], n->kin[L], 1, yp, nullptr,
-                              n->kout[L], n->b[L], nullptr, nullptr));
     size_t cnt = (size_t)nsamp * NOUT;
     k_pack_outputs<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(yp, nsamp, y_dev);
     OFX_HIP(hipGetLastError());
+    return 0;
+}
+
+int ofx_mlp_set_forward_variant(ofx_mlp* n, int variant) {
+    if (!n || variant < 0 || variant > 1) { set_error("mlp_set_forward_variant: variant 0 or 1"); return OFX_EINVAL; }
+    n->fwd_variant = variant;
     return 0;
 }
 

@@ -403,8 +403,8 @@ __global__ void __launch_bounds__(TB) k_ns_grad(DevMesh m, DevFields d, Consts k
     for (int j = 0; j < MAXF; ++j) {
         if (j < n) {
             const int c = nb[j];
-            if (EG) imask |= (c < m.C) << j;
             if (EG) {
+                imask |= (c < m.C) << j;
                 const size_t o = (size_t)j * m.C + i;
                 gw[j] = m.ellg[o];
                 if (DIM == 3 || LIM != 0) {

@@ -30,6 +30,7 @@ def _lib():
         L.ofx_mlp_set_params.argtypes = [P, fp]
         L.ofx_mlp_forward.argtypes = [P, fp, ctypes.c_int, fp]
         L.ofx_mlp_forward_noalloc.argtypes = [P, P, ctypes.c_int, P, P]
+        L.ofx_mlp_set_forward_variant.argtypes = [P, ctypes.c_int]
         L.ofx_mlp_gradients.argtypes = [P, fp, fp, ctypes.c_int, fp, fp]
         L.ofx_mlp_train.argtypes = [P, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_ulonglong, fp]
@@ -73,6 +74,10 @@ class Mlp:
             self._check(self._L.ofx_mlp_forward(self.h, _fp(xs), xs.shape[0], _fp(ys)))
             y[i:i + step] = ys
         return y
+
+    def set_forward_variant(self, variant) -> None:
+        """1 (default): all layers in one kernel; 0: one GEMM kernel per layer. Same results."""
+        self._check(self._L.ofx_mlp_set_forward_variant(self.h, int(variant)))
 
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, np.float32)

@@ -36,6 +36,24 @@ def test_forward_matches_oracle():
     assert e <= 2e-3, e
 
 
+@pytest.mark.parametrize("n", [1050, 70_000, 70_050])
+def test_one_kernel_forward_equals_layered(n):
+    """The one-kernel forward pass (activations in LDS) against one GEMM
+    kernel per layer: the same bits.  1050 and 70,050 samples leave the last
+    128-sample workgroup half filled; 70,000 takes the layered pass's large
+    tiles."""
+    rng = np.random.default_rng(7)
+    p = random_params(rng, scale=1.0).astype(np.float32)
+    x = rng.normal(0, 1, (n, NIN)).astype(np.float32)
+    m = net(p)
+    m.set_forward_variant(0)
+    y0 = m.forward(x)
+    m.set_forward_variant(1)
+    y1 = m.forward(x)
+    print(f"\n[mlp] one-kernel forward, n={n}: max |diff| {float(np.max(np.abs(y1 - y0))):.3e}", flush=True)
+    assert np.array_equal(y0, y1)
+
+
 def test_gradients_match_oracle():
     rng = np.random.default_rng(2)
     p = random_params(rng, scale=1.0).astype(np.float32)
