@@ -36,9 +36,14 @@ __device__ __forceinline__ bf16 f2bf(float f) {
     return (bf16)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((uint32_t)h) << 16); }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (v_rcp_f32, 1 ulp)
+// instead of a correctly rounded division: the epilogues evaluate one per
+// output element, and a division is ten instructions against one.  The
+// values feed BF16 activations, whose rounding is 2^-9 relative.
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
+    float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -147,30 +152,32 @@ struct MlpFwd {
     const float* b[NLAYER];
 };
 
-// The whole forward pass of 128 samples in one workgroup: the activations of
-// every hidden layer stay in LDS (128 x 256 bf16, 64 KiB), so HBM sees only the
-// packed inputs, the weights (from L2) and the outputs.  Eight waves; wave w
-// computes rows 64(w/4)..+63 and columns 64(w%4)..+63 of a hidden layer (4 x 4
-// tiles of v_mfma_f32_16x16x32_bf16, operands as in k_gemm: A from LDS, Bt
-// straight from the weights), then all waves write their tiles back over the
-// layer's input once every wave has read it.  The output layer needs its first
-// 16 (of 64 padded) columns only: wave w computes rows 16w..16w+15 of them.
+// The whole forward pass of 64 samples in one workgroup: the activations of
+// every hidden layer stay in LDS (64 x 256 bf16, 32 KiB), so HBM sees only the
+// packed inputs, the weights (from L2) and the outputs, and several
+// workgroups share a CU, so one's layer barriers do not idle the MFMA units.
+// Four waves; wave w computes all 64 rows and columns 64w..64w+63 of a
+// hidden layer (4 x 4 tiles of v_mfma_f32_16x16x32_bf16, operands as in
+// k_gemm: A from LDS, Bt straight from the weights), then all waves write
+// their tiles back over the layer's input once every wave has read it.  The
+// output layer needs its first 16 (of 64 padded) columns only: wave w
+// computes rows 16w..16w+15 of them.
 // Every product, sum order and epilogue is k_gemm's (the MFMA chain over k in
 // ascending 32-wide steps from zero, the 16-wide input layer through one MFMA
 // with k 16..31 zero), so the outputs are those of the layer-by-layer pass
 // bit for bit.  LDS rows are 512 bytes with no padding; the 16-byte vector v
 // of row r is stored at v ^ (r & 15), so the 16 rows an MFMA operand load
-// reads fall in different banks.  Rows at or past Mp (the last workgroup when
-// Mp is not a multiple of 128) read zero inputs and are not stored.
+// reads fall in different banks.  Mp is a multiple of 64; rows at or past it
+// would read zero inputs and not be stored.
 __device__ __forceinline__ int swz(int r, int k) { return (((k >> 3) ^ (r & 15)) << 3) | (k & 7); }
 
-__global__ void __launch_bounds__(512) k_mlp_forward(const bf16* __restrict__ x16, int Mp, MlpFwd P,
+__global__ void __launch_bounds__(256) k_mlp_forward(const bf16* __restrict__ x16, int Mp, MlpFwd P,
                                                      float* __restrict__ y) {
-    constexpr int BM = 128;
+    constexpr int BM = 64;
     __shared__ __align__(16) bf16 sH[BM][NHID];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.x * BM;
-    const int r0 = 64 * (w >> 2), c0 = 64 * (w & 3);  // this wave's tile of a hidden layer
+    const int r0 = 0, c0 = 64 * w;  // this wave's tile of a hidden layer
     const int lr = lane & 15, lk = 8 * (lane >> 4);    // operand row within a 16-tile; k offset of the lane's 8
     const v8bf zero8 = {};
     v4f acc[4][4];
@@ -201,24 +208,40 @@ __global__ void __launch_bounds__(512) k_mlp_forward(const bf16* __restrict__ x1
     }
     store_silu(P.b[0]);
     __syncthreads();
-    // hidden layers 1..4: 256 -> 256
-    for (int l = 1; l < NLAYER - 1; ++l) {
+    // hidden layers 1..4: 256 -> 256, 8 k steps of 32 each.  The weights are
+    // loaded from L2 two steps ahead (across layer ends: they do not depend
+    // on the activations) and the activations from LDS one step ahead, into
+    // register rings indexed by compile-time step numbers (both loops
+    // unrolled).
+    v8bf B[3][4], Am[2][4];
+    auto load_b = [&](const bf16* wt, int ks, v8bf* dst) {
+        for (int j = 0; j < 4; ++j) dst[j] = *(const v8bf*)&wt[(size_t)(c0 + 16 * j + lr) * NHID + ks + lk];
+    };
+    auto load_a = [&](int ks, v8bf* dst) {
+        for (int i = 0; i < 4; ++i) {
+            const int row = r0 + 16 * i + lr;
+            dst[i] = *(const v8bf*)&sH[row][swz(row, ks + lk)];
+        }
+    };
+    constexpr int NSTEP = NHID / 32, NHL = NLAYER - 2;  // k steps per layer, hidden 256 -> 256 layers
+    load_b(P.wt[1], 0, B[0]);
+    load_b(P.wt[1], 32, B[1]);
+#pragma unroll
+    for (int h = 0; h < NHL; ++h) {
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-        const bf16* wt = P.wt[l];
-        for (int ks = 0; ks < NHID; ks += 32) {
-            v8bf a[4], b[4];
-            for (int j = 0; j < 4; ++j) b[j] = *(const v8bf*)&wt[(size_t)(c0 + 16 * j + lr) * NHID + ks + lk];
-            for (int i = 0; i < 4; ++i) {
-                const int row = r0 + 16 * i + lr;
-                a[i] = *(const v8bf*)&sH[row][swz(row, ks + lk)];
-            }
+        load_a(0, Am[0]);
+#pragma unroll
+        for (int st = 0; st < NSTEP; ++st) {
+            const int g = h * NSTEP + st, g2 = g + 2;
+            if (g2 < NHL * NSTEP) load_b(P.wt[1 + g2 / NSTEP], 32 * (g2 % NSTEP), B[g2 % 3]);
+            if (st + 1 < NSTEP) load_a(32 * (st + 1), Am[(st + 1) & 1]);
             for (int i = 0; i < 4; ++i)
                 for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[st & 1][i], B[g % 3][j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();  // every wave has read this layer's input
-        store_silu(P.b[l]);
+        store_silu(P.b[1 + h]);
         __syncthreads();
     }
     // output layer: columns 0..15 of the padded 64, rows 16w..16w+15
@@ -528,7 +551,7 @@ int colsum(ofx_mlp* n, const float* g, int M, int N, float* out) {
 int forward_fused(const ofx_mlp* n, hipStream_t st, const bf16* x16, int Mp, float* y) {
     MlpFwd P;
     for (int l = 0; l < NLAYER; ++l) { P.wt[l] = n->wt[l]; P.b[l] = n->b[l]; }
-    k_mlp_forward<<<(Mp + 127) / 128, 512, 0, st>>>(x16, Mp, P, y);
+    k_mlp_forward<<<(Mp + 63) / 64, 256, 0, st>>>(x16, Mp, P, y);
     OFX_HIP(hipGetLastError());
     return 0;
 }

@@ -2,7 +2,7 @@
 
 oracle/mlp_oracle.py rounds to BF16 at the same points as the GPU and is
 exact FP64 elsewhere, so the GPU differs from it only by FP32 accumulation
-order, its exp, and the rare BF16 rounding flips these cause.  Stated bounds:
+order, its exp and reciprocal, and the rare BF16 rounding flips these cause.  Stated bounds:
 outputs within 2e-3 normwise relative, loss within 1e-4 relative, every
 layer's weight and bias gradient within 5e-3 normwise relative.  A short
 training run must cut the loss on a smooth regression target tenfold.
