@@ -689,7 +689,8 @@ MLP_MACS = 16 * 256 + 4 * 256 * 256 + 256 * 64
 
 def mlp_main(args):
     """The warm-start network alone: `steps` forward passes of --mlp-samples
-    samples (host timed, PCIe copies of inputs and outputs included) and
+    samples in each form (one kernel, the default; one GEMM per layer), host
+    timed with the PCIe copies of inputs and outputs included, and
     --mlp-steps training steps of --mlp-batch samples (forward, backward and
     Adam; one call).  Kernel durations come from rocprofv3 --kernel-trace;
     FLOP counts use the padded layer shapes."""
@@ -698,11 +699,15 @@ def mlp_main(args):
     n = args.mlp_samples
     x = rng.normal(0.0, 1.0, (n, 9)).astype(np.float32)
     net = Mlp(seed=1)
-    net.forward(x[:4096])
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        net.forward(x)
-    fwd_s = (time.perf_counter() - t0) / args.steps
+    fwd = {}
+    for variant in (0, 1):
+        net.set_forward_variant(variant)
+        net.forward(x[:4096])
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            net.forward(x)
+        fwd[variant] = (time.perf_counter() - t0) / args.steps
+    fwd_s = fwd[1]
     nt = 1 << 16
     xt = x[:nt]
     yt = np.tanh(xt[:, :5]).astype(np.float32)
@@ -719,6 +724,7 @@ def mlp_main(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": "mlp", "samples": n, "train_batch": args.mlp_batch},
         "forward_tflops_host": flop_fwd / fwd_s / 1e12,
+        "forward_layered_ms_host": 1e3 * fwd[0],
         "train": {"ms_per_step": 1e3 * train_s, "tflops_host": flop_step / train_s / 1e12,
                   "loss_first": float(losses[0]), "loss_last": float(losses[-1])}}), flush=True)
 
