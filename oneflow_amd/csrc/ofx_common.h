@@ -93,11 +93,13 @@ struct DevMesh {
     // A cell kernel reads them with contiguous loads that do not wait for its
     // face ids.
     dvec4* ellg = nullptr;
-    // Two more per-entry planes of doubles (same layout): 0 the distance
+    // Three more per-entry planes of doubles (same layout): 0 the distance
     // |n_out . (x_nb - x_c)| of the neighbour's centre from the cell's along
     // the normal out of the cell (the viscous spectral radius of models 2
     // and 3, evaluated as the sweeps evaluate it; equal bit for bit to the
-    // time step's left-to-right form), 1 the neighbour's volume (model 1).
+    // time step's left-to-right form), 1 the neighbour's volume (model 1),
+    // 2 nz A out of the cell again (plane 1's first value), so that the SA
+    // gradient reads 40 bytes per entry rather than two records.
     double* elldv = nullptr;
     // cell -> face CSR in ascending face id; entry = f if the cell is lc[f], ~f if rc[f]
     int *c2fOff = nullptr, *c2f = nullptr;

@@ -207,6 +207,7 @@ __global__ void k_ell_geom(DevMesh m) {
         }
         m.elldv[o] = dist;
         m.elldv[S + o] = vnb;
+        m.elldv[2 * S + o] = v[4];
     }
 }
 
@@ -701,7 +702,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(A((void**)&m.ellf, sizeof(int) * std::max<size_t>(1, ellf.size())));
     OFX_TRY(A((void**)&m.elln, sizeof(int) * std::max<size_t>(1, elln.size())));
     OFX_TRY(A((void**)&m.ellg, sizeof(dvec4) * 3 * std::max<size_t>(1, ellf.size())));
-    OFX_TRY(A((void**)&m.elldv, sizeof(double) * 2 * std::max<size_t>(1, ellf.size())));
+    OFX_TRY(A((void**)&m.elldv, sizeof(double) * 3 * std::max<size_t>(1, ellf.size())));
     OFX_TRY(A((void**)&m.loOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.upOrder, sizeof(int) * C));
     OFX_TRY(A((void**)&m.loOff, sizeof(int) * (nlo + 1)));

@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(TBS) k_sa_grad(DevMesh m, DevFields d, Consts 
             // a padding entry reads entry 0 again (cached), not its zeros
             const size_t o = (size_t)(j < n ? j : 0) * m.C + i;
             gw[j] = m.ellg[o];
-            gz[j] = (DIM == 3) ? m.ellg[S + o].x : 0.0;
+            gz[j] = (DIM == 3) ? m.elldv[2 * S + o] : 0.0;
         } else {
             int e = ent[j], f = e >= 0 ? e : ~e;
             gw[j] = m.fgg[f];
