@@ -43,6 +43,7 @@ constexpr int MAXSEG = 9;  // cell segments <= 8 (colours); faces: boundary + 8
 // 128-bit memory operations pairs kept in registers (HIP's double4 struct is
 // copied through private memory when the compiler cannot split it).
 typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 struct DevMesh {
     int dim = 2, nNodes = 0, C = 0, F = 0, B = 0, T = 0;
@@ -60,8 +61,11 @@ struct DevMesh {
     //          the left / right cell and the area-weighted normal;
     //   frec = {xf - xl, yf - yl, xf - xr, yf - yr}: face centre relative to
     //          the left and right cell centres (reconstruction, limiter);
-    //   fz   = {nz A, zf - zl, zf - zr, 0} (3-D only).
+    //   fz   = {nz A, zf - zl, zf - zr, 0} (3-D only);
+    //   fzr  = {zf - zl, zf - zr} again (3-D only): the face flux's
+    //          reconstruction reads these two and not the 32-byte fz.
     dvec4 *fgg = nullptr, *frec = nullptr, *fz = nullptr;
+    dvec2* fzr = nullptr;
     //   fv0  = {dx, dy, dz, 1/|d|}: unit centre-to-centre vector of the STD
     //          viscous face gradient (VisGeom, method 0);
     //   fv2  = 5 per face: {d1, d2, angle1, angle2}, {dxnl, dynl, dznl, dxnr},

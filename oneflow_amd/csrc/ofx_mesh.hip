@@ -159,7 +159,10 @@ __global__ void k_face_static(DevMesh m) {
     double ar = m.area[f];
     m.fgg[f] = dvec4{cl, cr, m.xfn[f] * ar, m.yfn[f] * ar};
     m.frec[f] = dvec4{fx - xl, fy - yl, fx - xr, fy - yr};
-    if (DIM == 3) m.fz[f] = dvec4{m.zfn[f] * ar, fz - zl, fz - zr, 0.0};
+    if (DIM == 3) {
+        m.fz[f] = dvec4{m.zfn[f] * ar, fz - zl, fz - zr, 0.0};
+        m.fzr[f] = dvec2{fz - zl, fz - zr};
+    }
 }
 
 // The face terms of the cell kernels once per (cell, entry) (DevMesh::ellg):
@@ -682,6 +685,7 @@ int mesh_build(ofx_ctx* c, const double* x, const double* y, const double* z, co
     OFX_TRY(A((void**)&m.fgg, sizeof(dvec4) * F));
     OFX_TRY(A((void**)&m.frec, sizeof(dvec4) * F));
     OFX_TRY(A((void**)&m.fz, sizeof(dvec4) * (m.dim == 3 ? F : 1)));
+    OFX_TRY(A((void**)&m.fzr, sizeof(dvec2) * (m.dim == 3 ? F : 1)));
     OFX_TRY(A((void**)&m.fv0, sizeof(dvec4) * F));
     OFX_TRY(A((void**)&m.fv2, sizeof(dvec4) * 5 * (size_t)F));
     OFX_TRY(A((void**)&m.lc, sizeof(int) * F));

@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(512) k_ns_face(DevMesh m, DevFields d, Consts 
     } else {
         dvec4 rc4 = m.frec[f];
         double dxl = rc4.x, dyl = rc4.y, dxr = rc4.z, dyr = rc4.w, dzl = 0.0, dzr = 0.0;
-        if (DIM == 3) { dvec4 z4 = m.fz[f]; dzl = z4.y; dzr = z4.z; }
+        if (DIM == 3) { dvec2 z2 = m.fzr[f]; dzl = z2.x; dzr = z2.y; }
         for (int e = 0; e < 5; ++e) {
             if (DIM == 2 && e == 3) {
                 // w and its gradient are zero in 2-D: the reconstruction leaves w = 0
