@@ -561,7 +561,9 @@ __device__ __forceinline__ void ns_gather_cell(const DevMesh& m, const DevFields
     // Every entry is loaded, padding included (it names face 0), so that no
     // load sits in a divergent branch, and all loads come before the first
     // store (which the compiler cannot move them across): the first n
-    // entries are summed.
+    // entries are summed.  The viscous flux of the continuity equation is
+    // coeff * 0.0, a signed zero, on every face, and a sum that starts at +0
+    // and only adds is never -0, so adding it changes no bit: it is not read.
     double fi[5][MAXF], fv[5][MAXF];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
@@ -570,7 +572,7 @@ __device__ __forceinline__ void ns_gather_cell(const DevMesh& m, const DevFields
             int e = ent[j], f = e >= 0 ? e : ~e;
             double x = F32 ? (double)inv32[s * F + f] : d.invflux[s * F + f];
             fi[s][j] = (e >= 0) ? -x : x;
-            if (VISC) {
+            if (VISC && s > 0) {
                 double y = F32 ? (double)vis32[s * F + f] : d.visflux[s * F + f];
                 fv[s][j] = (e >= 0) ? -y : y;
             }
@@ -582,7 +584,7 @@ __device__ __forceinline__ void ns_gather_cell(const DevMesh& m, const DevFields
 #pragma unroll
         for (int j = 0; j < MAXF; ++j)
             sum = sum + live_or_nz(j < n, fi[s][j]);
-        if (VISC) {
+        if (VISC && s > 0) {
 #pragma unroll
             for (int j = 0; j < MAXF; ++j)
                 sum = sum + live_or_nz(j < n, fv[s][j]);
@@ -782,8 +784,8 @@ __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
             double x = F32 ? (double)inv32[s * F + f] : d.invflux[s * F + f];
             fi[s] = (e >= 0) ? -x : x;
         }
-        if (VISC)
-            for (int s = 0; s < 5; ++s) {
+        if (VISC)  // the continuity equation's viscous flux is a signed zero (see ns_gather_cell)
+            for (int s = 1; s < 5; ++s) {
                 double x = F32 ? (double)vis32[s * F + f] : d.visflux[s * F + f];
                 fv[s] = (e >= 0) ? -x : x;
             }
@@ -794,7 +796,7 @@ __global__ void __launch_bounds__(TB) k_ns_gather(DevMesh m, DevFields d) {
         double sum = 0.0;
 #pragma unroll
         for (int jj = 0; jj < LPC; ++jj) { double y = __shfl(fi[s], base + jj, 64); if (jj < n) sum = sum + y; }
-        if (VISC) {
+        if (VISC && s > 0) {
 #pragma unroll
             for (int jj = 0; jj < LPC; ++jj) { double y = __shfl(fv[s], base + jj, 64); if (jj < n) sum = sum + y; }
         }
