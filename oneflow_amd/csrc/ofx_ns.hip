@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(TB) k_ns_gather_cf(DevMesh m, DevFields d, Con
 
 // Viscous face flux (UNsVisFlux::CalcFlux, codes/uns/src/UNsVisFlux.cpp:127-440)
 // for face-gradient method VM: face gradients of u, v, (w,) T, Newtonian
-// stress and heat flux, stored as visflux[5][F].
+// stress and heat flux, stored as visflux[5][F] (rows 1..4; row 0 is zero and not stored).
 template <int DIM, bool F32, int VM>
 __device__ __forceinline__ void ns_face_visc(const DevMesh& m, const DevFields& d, const Consts& k, int f, int l,
                                              int r, int rg, const double* ql, const double* qr, double xfn,
@@ -651,15 +651,15 @@ __device__ __forceinline__ void ns_face_visc(const DevMesh& m, const DevFields& 
     double fv3 = xfn * txz + yfn * tyz + zfn * tzz;
     double fv4 = qfv[1] * fv1 + qfv[2] * fv2 + qfv[3] * fv3 + qNormal;
     double coeff = -k.oreynolds * area;
+    // Row 0, the continuity equation's coeff * 0.0, is a signed zero that the
+    // gathers do not read (adding it changes no bit), so it is not stored.
     if (F32) {
         float* out = reinterpret_cast<float*>(d.visflux);
-        out[0 * m.F + f] = (float)(coeff * 0.0);
         out[1 * m.F + f] = (float)(coeff * fv1);
         out[2 * m.F + f] = (float)(coeff * fv2);
         out[3 * m.F + f] = (float)(coeff * fv3);
         out[4 * m.F + f] = (float)(coeff * fv4);
     } else {
-        d.visflux[0 * m.F + f] = coeff * 0.0;
         d.visflux[1 * m.F + f] = coeff * fv1;
         d.visflux[2 * m.F + f] = coeff * fv2;
         d.visflux[3 * m.F + f] = coeff * fv3;
